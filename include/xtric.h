@@ -1,0 +1,219 @@
+/*
+ * xtric.h -- C ABI of the MI355X (gfx950) triangle-counting library
+ *            (xxtric_amd/lib/libxtric.so).
+ *
+ * Plain C types only: no HIP, torch or C++ types cross this boundary.  Every
+ * function returns an int status (XT_OK == 0); xt_last_error(ctx) holds the
+ * message of the last failure on that context.  A context is used by one host
+ * thread at a time (calls on one context are serialised by an internal mutex).
+ *
+ * Counting semantics (all entry points): the input is a simple undirected graph
+ * in CSR form -- symmetric adjacency, each row strictly ascending (no duplicate
+ * entries); self-loops are allowed and never close a triangle.  Each triangle
+ * u < v < w is counted once and, for per-vertex output, credited to u, its
+ * smallest vertex id (xxtric.hpp:160-206).  Counts are exact uint64.
+ */
+#ifndef XTRIC_H
+#define XTRIC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------- */
+#define XT_OK 0
+#define XT_ERR_ARG (-1)       /* bad argument (null pointer, negative size ...) */
+#define XT_ERR_HIP (-2)       /* HIP runtime error */
+#define XT_ERR_NOMEM (-3)     /* device allocation failed / over budget */
+#define XT_ERR_INPUT (-4)     /* adjacency unsorted, duplicated, out of range or asymmetric */
+#define XT_ERR_CAPACITY (-5)  /* batch / server capacity exceeded */
+#define XT_ERR_NOTFOUND (-6)  /* unknown ticket / slot */
+#define XT_ERR_STATE (-7)     /* call not valid in the object's current state */
+#define XT_PENDING 1          /* xt_serve_poll: request not finished yet */
+
+/* ---- counting flags ----------------------------------------------------- */
+#define XT_ORIENT_DEGREE 0x0u  /* orient u->v iff (deg u, u) < (deg v, v) (default) */
+#define XT_ORIENT_ID 0x1u      /* orient u->v iff u < v, the reference's rule */
+#define XT_PER_VERTEX 0x2u     /* also produce per-vertex counts (min-id credit) */
+#define XT_VALIDATE 0x4u       /* check rows strictly ascending and ids in range */
+#define XT_CHECK_SYMMETRY 0x8u /* with XT_VALIDATE: also check (u,v) => (v,u) */
+
+typedef struct xt_ctx xt_ctx;
+typedef struct xt_graph xt_graph;
+
+/* Per-phase device times of the last count on a context, from HIP events on
+ * the stream the kernels ran on (milliseconds). */
+typedef struct xt_timing {
+  double orient_ms;   /* degree keys + out-degree count + scans + oriented fill */
+  double count_ms;    /* intersection + reduction kernels */
+  double total_ms;    /* first to last event of the count */
+  int64_t ne_plus;    /* oriented edges |E+| */
+  int64_t n_deferred; /* edges handed to the wave-per-edge kernel */
+  int64_t n_block;    /* edges handed to the block-per-edge kernel */
+} xt_timing;
+
+const char* xt_version(void);
+
+/* Context: owns device scratch, one HIP stream, events.  device = HIP ordinal. */
+int xt_ctx_create(int device, xt_ctx** out);
+int xt_ctx_destroy(xt_ctx* ctx);
+const char* xt_last_error(const xt_ctx* ctx);
+int xt_last_timing(const xt_ctx* ctx, xt_timing* out);
+/* Tuning knobs (normally set by the autotuner): small-edge cutoff and chunk
+ * sizes; 0 keeps the current value. */
+int xt_ctx_tune(xt_ctx* ctx, int small_cut, int block_cut, int reserved);
+
+/* ---- device memory plumbing ------------------------------------------------
+ * The CudaMalloc / CudaMemcpyHtoD / CudaMemcpyDtoH / CudaFree /
+ * CudaDeviceSynchronize macros the reference engine uses around its launchers
+ * (counting.cuh:40-147, xxtric-gpu.hpp:116-150,266-284), for callers that hold
+ * no HIP runtime of their own.  kind: XT_H2D, XT_D2H or XT_D2D. */
+#define XT_H2D 1
+#define XT_D2H 2
+#define XT_D2D 3
+int xt_dev_alloc(xt_ctx* ctx, size_t bytes, void** out);
+int xt_dev_free(xt_ctx* ctx, void* p);
+int xt_memcpy(xt_ctx* ctx, void* dst, const void* src, size_t bytes, int kind);
+int xt_dev_sync(xt_ctx* ctx);
+
+/* ---- single-device count of a host CSR ---------------------------------
+ * Replaces TriangleCountingSingle(Graph*) + count() (xxtric-gpu.hpp:27-156,
+ * 192-270): offsets/tails are GraphElem (int64) host arrays as in
+ * Graph::edge_indices_ / Edge::tail_ (graph.hpp:267-269).  Host buffers are
+ * borrowed for the duration of the call.  per_vertex (nullable, nv entries)
+ * requires XT_PER_VERTEX. */
+int xt_count_csr(xt_ctx* ctx, int64_t nv, int64_t ne, const int64_t* offsets,
+                 const int64_t* tails, uint32_t flags, uint64_t* total,
+                 uint64_t* per_vertex);
+
+/* ---- lower-FFI drop-ins (device pointers, caller-owned buffers) ---------
+ * Same arguments and ownership as the reference launchers; the count is
+ * ADDED to *d_local_count (the caller zeroes it, xxtric-gpu.hpp:150) and the
+ * call is asynchronous on `stream` (a hipStream_t, NULL = the context's). */
+
+/* counting.cuh:259-265 single_gpu_count (counting.cu:537-577): whole-graph count from a device
+ * int64 CSR (symmetric, sorted rows). */
+int xt_single_gpu_count(xt_ctx* ctx, int64_t nv, const int64_t* d_indices,
+                        const int64_t* d_edge_list, int64_t* d_local_count,
+                        void* stream);
+
+/* counting.cuh:229-245 single_gpu_count_masked (counting.cu:297-359): nedges pairs (e1, m) in
+ * d_edge_mask (int64, 2 per edge, built as xxtric-gpu.hpp:91-107); counts
+ * |{n in (m, e1) : d_edge_list[n] in N(d_edge_list[m])}| summed over pairs. */
+int xt_single_gpu_count_masked(xt_ctx* ctx, int64_t nedges, const int64_t* d_edge_mask,
+                               const int64_t* d_indices, const int64_t* d_edge_list,
+                               int64_t* d_local_count, void* stream);
+
+/* counting.cuh:314-323 local_count_gpu_inner (counting.cu:640-657): counts
+ * |{i in [start,end) : d_edge_list[i] in d_edge_list[jstart..jend)}|. */
+int xt_local_count_gpu_inner(xt_ctx* ctx, int64_t start, int64_t end, int64_t jstart,
+                             int64_t jend, const int64_t* d_edge_list,
+                             int64_t* d_local_count, void* stream);
+
+/* ---- device-resident graphs (CSR kept in HBM: int64 offsets, int32 ids) -- */
+/* Upload a host CSR; tails_bytes = 4 (int32) or 8 (int64). */
+int xt_graph_from_csr(xt_ctx* ctx, int64_t nv, int64_t ne, const int64_t* offsets,
+                      const void* tails, int tails_bytes, uint32_t flags, xt_graph** out);
+/* Seeded R-MAT generated and cleaned on the device (self-loops dropped,
+ * duplicates merged, symmetrised).  permute != 0 relabels vertices by a
+ * seeded bijection of [0, 2^scale). */
+int xt_graph_rmat(xt_ctx* ctx, int scale, int edgefactor, double a, double b, double c,
+                  uint64_t seed, int permute, xt_graph** out);
+/* Seeded G(n,p): edge {i<j} iff (mix64(mix64(seed) + i*n + j) >> 32) < thr64. */
+int xt_graph_er(xt_ctx* ctx, int64_t n, uint64_t seed, uint64_t thr64, xt_graph** out);
+int xt_graph_info(const xt_graph* g, int64_t* nv, int64_t* ne);
+/* Copy the CSR back to host arrays (offsets nv+1, tails ne). */
+int xt_graph_download(xt_ctx* ctx, const xt_graph* g, int64_t* offsets, int32_t* tails);
+/* Count a resident graph: orientation + intersection + reduction on device.
+ * per_vertex (host, nullable) requires XT_PER_VERTEX. */
+int xt_graph_count(xt_ctx* ctx, xt_graph* g, uint32_t flags, uint64_t* total,
+                   uint64_t* per_vertex);
+/* Algorithmic byte figure of one count of g under `flags` orientation:
+ * 8(n+1) + 4|E+| + 4 * sum over oriented (u,v) of min(d+(u), d+(v)). */
+int xt_graph_alg_bytes(xt_ctx* ctx, xt_graph* g, uint32_t flags, uint64_t* bytes,
+                       uint64_t* sum_min, int64_t* ne_plus);
+int xt_graph_destroy(xt_ctx* ctx, xt_graph* g);
+
+/* ---- host ingest (C++ in the library; no device work) -------------------- */
+/* Reference RGG (graph.hpp:745-1374, one rank): LCG coordinates seeded with
+ * reseeder(seed) (utils.hpp:37-44), edge iff sqrt(dx*dx+dy*dy) <= r, r from
+ * graph.hpp:789-791 when radius <= 0.  Cell-grid search, same edge set as the
+ * reference's all-pairs loop.  Returns ne in *ne; fills tails when cap >= ne. */
+int xt_rgg_generate(int64_t nv, uint32_t seed, double radius, int64_t* offsets,
+                    int64_t* tails, int64_t cap, int64_t* ne);
+double xt_rgg_radius(int64_t nv);
+/* vite binary file (graph.hpp:294-395); layout32 selects the USE_32_BIT_GRAPH
+ * layout.  Header only when offsets == NULL. */
+int xt_vite_read(const char* path, int layout32, int64_t* nv, int64_t* ne,
+                 int64_t* offsets, int64_t* tails);
+int xt_vite_write(const char* path, int layout32, int64_t nv, const int64_t* offsets,
+                  const int64_t* tails);
+/* Tab/space separated edge list "src dst [w]" -> symmetric sorted CSR
+ * (cugraph/example-inputs format).  Two calls: tails == NULL sizes it. */
+int xt_edgelist_to_csr(const char* path, int64_t* nv, int64_t* ne, int64_t* offsets,
+                       int64_t* tails);
+
+/* ---- batching: many small graphs counted by one persistent launch ------- */
+typedef struct xt_batch xt_batch;
+typedef struct xt_batch_timing {
+  double upload_ms;  /* host->device staging of added graphs */
+  double run_ms;     /* device time of the batched count (events) */
+  int64_t n_graphs, n_vertices, n_edges, n_tiles;
+} xt_batch_timing;
+#define XT_BATCH_PERSISTENT 0 /* one persistent pipeline over all graphs */
+#define XT_BATCH_HIPGRAPH 1   /* same pipeline, captured once and replayed */
+#define XT_BATCH_PER_GRAPH 2  /* baseline: one launch sequence per graph */
+int xt_batch_create(xt_ctx* ctx, int64_t max_vertices, int64_t max_edges,
+                    int64_t max_graphs, xt_batch** out);
+/* Append one graph (int32 ids local to the graph, symmetric sorted rows). */
+int xt_batch_add(xt_batch* b, int64_t nv, int64_t ne, const int64_t* offsets,
+                 const int32_t* tails, int64_t* slot);
+/* Append the seeded mix of graphs [first, first+count) (RGG even, ER odd). */
+int xt_batch_add_generated(xt_batch* b, uint64_t base_seed, int64_t first, int64_t count,
+                           int64_t nmin, int64_t nmax);
+int xt_batch_run(xt_batch* b, int mode, uint32_t flags);
+int xt_batch_results(xt_batch* b, uint64_t* totals, int64_t n);
+int xt_batch_timing_get(const xt_batch* b, xt_batch_timing* out);
+int xt_batch_clear(xt_batch* b);
+int xt_batch_destroy(xt_batch* b);
+/* Parameters of batch graph g (log-uniform n in [nmin,nmax], RGG even / ER odd). */
+int xt_batch_params(uint64_t base_seed, int64_t g, int64_t nmin, int64_t nmax,
+                    int64_t* n, int* kind, uint64_t* gseed, uint64_t* er_thr64);
+/* Host CSR of batch graph g (two calls: tails == NULL sizes it). */
+int xt_batch_graph_csr(uint64_t base_seed, int64_t g, int64_t nmin, int64_t nmax,
+                       int64_t* nv, int64_t* ne, int64_t* offsets, int32_t* tails);
+
+/* ---- serving: continuous batching of arriving graphs -------------------- */
+typedef struct xt_server xt_server;
+typedef struct xt_serve_config {
+  int64_t hbm_budget_bytes; /* admission limit for resident graphs (0 = 8 GiB) */
+  int64_t max_batch_graphs; /* graphs folded into one launch (0 = 4096) */
+  int64_t max_inflight;     /* batches in flight on the device (0 = 2) */
+  int64_t max_queue;        /* pending requests before submit fails (0 = 1<<20) */
+  double max_wait_us;       /* batching window after the first queued request */
+} xt_serve_config;
+typedef struct xt_serve_stats {
+  int64_t submitted, completed, rejected, batches;
+  double p50_us, p99_us, mean_us, max_us; /* arrival -> result latency */
+  double graphs_per_s;                    /* completed / (last done - first arrival) */
+  int64_t peak_resident_bytes;
+} xt_serve_stats;
+int xt_serve_create(xt_ctx* ctx, const xt_serve_config* cfg, xt_server** out);
+/* Non-blocking; the CSR is copied into server-owned host memory before return.
+ * arrival_us < 0 stamps the arrival with the server clock. */
+int xt_serve_submit(xt_server* s, int64_t nv, int64_t ne, const int64_t* offsets,
+                    const int32_t* tails, double arrival_us, uint64_t* ticket);
+/* XT_OK with the count when done, XT_PENDING while queued or running. */
+int xt_serve_poll(xt_server* s, uint64_t ticket, uint64_t* total, double* latency_us);
+int xt_serve_drain(xt_server* s, double timeout_s);
+int xt_serve_stats_get(xt_server* s, xt_serve_stats* out);
+double xt_serve_now_us(const xt_server* s);
+int xt_serve_destroy(xt_server* s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* XTRIC_H */

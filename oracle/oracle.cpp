@@ -1,0 +1,457 @@
+// oracle.cpp -- CPU restatement of the xxtric counting path (TEST INFRASTRUCTURE).
+//
+// Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load
+// this library, and only as the checker.  The product never links it.
+//
+// Compiled with -ffp-contract=off: the RGG distance test must round exactly as
+// the reference's host build does (x86-64 baseline ISA, no FMA contraction).
+#include "oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <random>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+// ---------------------------------------------------------------------------
+// Reference constants (deftypes.hpp:7,14-16).
+static const double kPi = 3.14159;
+static const int64_t kMlcg = 2147483647;  // 2^31 - 1
+static const int64_t kAlcg = 16807;       // 7^5
+static const int64_t kBlcg = 0;
+
+extern "C" uint32_t or_reseeder(uint32_t seed) {
+  // utils.hpp:37-44: std::seed_seq{seed}, one generated 32-bit word.
+  std::seed_seq seq({seed});
+  std::vector<std::uint32_t> out(1);
+  seq.generate(out.begin(), out.end());
+  return out[0];
+}
+
+extern "C" double or_rgg_radius_ref(int64_t nv) {
+  // graph.hpp:789-791, same casts and operation order.
+  double rc = std::sqrt((double)std::log(nv) / (double)(kPi * nv));
+  double rt = std::sqrt((double)2.0736 / (double)nv);
+  return (rc + rt) / (double)2.0;
+}
+
+extern "C" void or_rgg_lcg_coords(int64_t n, uint32_t seed, double* X, double* Y) {
+  // utils.hpp:248-268 (x0 = reseeder(seed) at rank 0, nprocs = 1 so no
+  // parallel prefix), 365-394 (generate), 399-420 (rescale with lo = 0,
+  // range = 1/nprocs = 1).  The LCG object holds 2n numbers: X | Y.
+  const int64_t len = 2 * n;
+  std::vector<int64_t> r(len);
+  r[0] = (int64_t)or_reseeder(seed);
+  for (int64_t i = 1; i < len; i++) r[i] = (r[i - 1] * kAlcg + kBlcg) % kMlcg;
+  const double mult = 1.0 / (double)(1.0 + (double)(kMlcg - 1));
+  const double range = 1.0 / (double)1;
+  for (int64_t i = 0; i < n; i++) X[i] = (double)((double)std::fabs((double)r[i]) * mult);
+  for (int64_t j = 0; j < n; j++) {
+    double d = (double)((double)std::fabs((double)r[n + j]) * mult);
+    Y[j] = 0.0 + (double)(range * d);
+  }
+}
+
+extern "C" int64_t or_rgg_lcg(int64_t n, uint32_t seed, double radius, int64_t* off,
+                              int64_t* tails, int64_t cap, int nthreads) {
+  // graph.hpp:919-948: every pair i<j with sqrt(dx*dx+dy*dy) <= rn_ becomes
+  // the two directed entries (i,j) and (j,i); 1289-1341: CSR sorted by (src,dst).
+  std::vector<double> X(n), Y(n);
+  or_rgg_lcg_coords(n, seed, X.data(), Y.data());
+  const double rn = radius > 0 ? radius : or_rgg_radius_ref(n);
+  std::vector<std::vector<int64_t>> up(n);
+  if (nthreads < 1) nthreads = 1;
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads)
+  for (int64_t i = 0; i < n; i++) {
+    for (int64_t j = i + 1; j < n; j++) {
+      double dx = X[i] - X[j];
+      double dy = Y[i] - Y[j];
+      double ed = std::sqrt(dx * dx + dy * dy);
+      if (ed <= rn) up[i].push_back(j);
+    }
+  }
+  std::vector<int64_t> deg(n, 0);
+  for (int64_t i = 0; i < n; i++) {
+    deg[i] += (int64_t)up[i].size();
+    for (int64_t j : up[i]) deg[j]++;
+  }
+  off[0] = 0;
+  for (int64_t i = 0; i < n; i++) off[i + 1] = off[i] + deg[i];
+  const int64_t ne = off[n];
+  if (tails && ne <= cap) {
+    // Row j holds {i < j} in increasing i, then {j' > j} in increasing j'.
+    std::vector<int64_t> cur(off, off + n);
+    for (int64_t i = 0; i < n; i++)
+      for (int64_t j : up[i]) tails[cur[j]++] = i;
+    for (int64_t i = 0; i < n; i++)
+      for (int64_t j : up[i]) tails[cur[i]++] = j;
+  }
+  return ne;
+}
+
+// ---------------------------------------------------------------------------
+// Reference CPU count, xxtric.hpp:60-208.
+template <typename T>
+static inline bool check_edgelist(const int64_t* off, const T* tails, int64_t v, int64_t w) {
+  // xxtric.hpp:516-545: first/last shortcut, then bisection with MID().
+  int64_t beg = off[v], end = off[v + 1];
+  int64_t mid = beg + (end - beg) / 2;
+  const int64_t first = tails[beg];
+  const int64_t last = tails[end - 1];
+  if (w == last || w == first) return true;
+  if (w > first && w < last) {
+    while (end > beg) {
+      const int64_t mv = tails[mid];
+      if (mv == w) return true;
+      if (mv < w)
+        beg = mid + 1;
+      else
+        end = mid;
+      mid = beg + (end - beg) / 2;
+    }
+  }
+  return false;
+}
+
+template <typename T>
+static uint64_t count_reference(int64_t nv, const int64_t* off, const T* tails,
+                                uint64_t* per_vertex, int nthreads, int64_t vstart,
+                                int64_t vstride) {
+  // erange_ (xxtric.hpp:60-75): min and max neighbour of each vertex; degree-1
+  // rows keep (0,0).  Degree-0 rows: the reference reads the neighbouring
+  // rows' entries here (get_edge(e0-1)); that value is never consulted because
+  // no edge points at an isolated vertex, so this restatement leaves (0,0).
+  std::vector<int64_t> erange(2 * nv, 0);
+  for (int64_t i = 0; i < nv; i++) {
+    const int64_t e0 = off[i], e1 = off[i + 1];
+    if (e0 + 1 == e1 || e0 == e1) continue;
+    erange[2 * i] = tails[e0];
+    erange[2 * i + 1] = tails[e1 - 1];
+  }
+  if (vstride < 1) vstride = 1;
+  if (nthreads < 1) nthreads = 1;
+  const int64_t nsteps = vstart < nv ? (nv - vstart + vstride - 1) / vstride : 0;
+  uint64_t total = 0;
+  // OpenMP schedule(static) as xxtric.hpp:84; the serial branch (149-208) is
+  // the same loop with one thread.  The total is summed in 64 bits (the
+  // OpenMP branch's std::accumulate(..., 0) at :147 sums in int and wraps).
+#pragma omp parallel for schedule(static) num_threads(nthreads) reduction(+ : total)
+  for (int64_t k = 0; k < nsteps; k++) {
+    const int64_t i = vstart + k * vstride;
+    const int64_t e0 = off[i], e1 = off[i + 1];
+    uint64_t t = 0;
+    if (e0 + 1 != e1) {
+      for (int64_t m = e0; m < e1 - 1; m++) {
+        const int64_t v = tails[m];
+        if (i >= v) continue;  // xxtric.hpp:165 -- triangle i<v<w counted at i
+        const int64_t vmin = erange[2 * v], vmax = erange[2 * v + 1];
+        for (int64_t n = m + 1; n < e1; n++) {
+          const int64_t w = tails[n];
+          if (!(w <= vmax)) break;     // edge_within_max, xxtric.hpp:583-588
+          if (!(w >= vmin)) continue;  // edge_above_min, xxtric.hpp:576-581
+          if (check_edgelist(off, tails, v, w)) t += 1;
+        }
+      }
+    }
+    if (per_vertex) per_vertex[i] = t;
+    total += t;
+  }
+  return total;
+}
+
+extern "C" uint64_t or_count_reference(int64_t nv, const int64_t* off, const int64_t* tails,
+                                       uint64_t* per_vertex, int nthreads, int64_t vstart,
+                                       int64_t vstride) {
+  return count_reference<int64_t>(nv, off, tails, per_vertex, nthreads, vstart, vstride);
+}
+
+extern "C" uint64_t or_count_reference32(int64_t nv, const int64_t* off, const int32_t* tails,
+                                         uint64_t* per_vertex, int nthreads, int64_t vstart,
+                                         int64_t vstride) {
+  return count_reference<int32_t>(nv, off, tails, per_vertex, nthreads, vstart, vstride);
+}
+
+// ---------------------------------------------------------------------------
+// Independent exact counter (not a reference restatement): orientation by
+// (degree, id), sorted-merge intersection of out-lists, credit min(u, v, w).
+extern "C" uint64_t or_count_degree_ordered(int64_t nv, const int64_t* off, const int32_t* tails,
+                                            uint64_t* per_vertex, int nthreads) {
+  if (nthreads < 1) nthreads = 1;
+  auto deg = [&](int64_t x) { return off[x + 1] - off[x]; };
+  auto less = [&](int64_t a, int64_t b) {
+    const int64_t da = deg(a), db = deg(b);
+    return da < db || (da == db && a < b);
+  };
+  std::vector<int64_t> ooff(nv + 1, 0);
+#pragma omp parallel for schedule(dynamic, 1024) num_threads(nthreads)
+  for (int64_t u = 0; u < nv; u++) {
+    int64_t c = 0;
+    for (int64_t e = off[u]; e < off[u + 1]; e++)
+      if (less(u, tails[e])) c++;
+    ooff[u + 1] = c;
+  }
+  for (int64_t u = 0; u < nv; u++) ooff[u + 1] += ooff[u];
+  std::vector<int32_t> ot(ooff[nv]);
+#pragma omp parallel for schedule(dynamic, 1024) num_threads(nthreads)
+  for (int64_t u = 0; u < nv; u++) {
+    int64_t p = ooff[u];
+    for (int64_t e = off[u]; e < off[u + 1]; e++)
+      if (less(u, tails[e])) ot[p++] = tails[e];
+  }
+  if (per_vertex) std::memset(per_vertex, 0, sizeof(uint64_t) * nv);
+  uint64_t total = 0;
+#pragma omp parallel for schedule(dynamic, 256) num_threads(nthreads) reduction(+ : total)
+  for (int64_t u = 0; u < nv; u++) {
+    for (int64_t a = ooff[u]; a < ooff[u + 1]; a++) {
+      const int64_t v = ot[a];
+      int64_t i = ooff[u], ie = ooff[u + 1], j = ooff[v], je = ooff[v + 1];
+      const int64_t uv = u < v ? u : v;
+      uint64_t at_uv = 0;
+      while (i < ie && j < je) {
+        const int64_t x = ot[i], y = ot[j];
+        if (x < y)
+          i++;
+        else if (y < x)
+          j++;
+        else {
+          total++;
+          if (per_vertex) {
+            if (x < uv) {
+#pragma omp atomic
+              per_vertex[x] += 1;
+            } else {
+              at_uv++;
+            }
+          }
+          i++;
+          j++;
+        }
+      }
+      if (per_vertex && at_uv) {
+#pragma omp atomic
+        per_vertex[uv] += at_uv;
+      }
+    }
+  }
+  return total;
+}
+
+// ---------------------------------------------------------------------------
+// vite binary format, graph.hpp:294-395 (reader) with the layout at 321-323,
+// 333-334, 361-362; Edge = {GraphElem tail_; GraphWeight weight_} (graph.hpp:20-25).
+template <typename E, typename W>
+static int vite_write(const char* path, int64_t nv, const int64_t* off, const int64_t* tails) {
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return -1;
+  E hdr[2] = {(E)nv, (E)off[nv]};
+  int ok = std::fwrite(hdr, sizeof(E), 2, f) == 2;
+  std::vector<E> o(nv + 1);
+  for (int64_t i = 0; i <= nv; i++) o[i] = (E)off[i];
+  ok = ok && std::fwrite(o.data(), sizeof(E), nv + 1, f) == (size_t)(nv + 1);
+  struct Rec {
+    E tail;
+    W w;
+  };
+  const int64_t ne = off[nv];
+  std::vector<Rec> buf(1 << 16);
+  for (int64_t e = 0; e < ne && ok;) {
+    int64_t k = std::min<int64_t>(ne - e, (int64_t)buf.size());
+    for (int64_t t = 0; t < k; t++) {
+      buf[t].tail = (E)tails[e + t];
+      buf[t].w = (W)1.0;
+    }
+    ok = std::fwrite(buf.data(), sizeof(Rec), k, f) == (size_t)k;
+    e += k;
+  }
+  std::fclose(f);
+  return ok ? 0 : -1;
+}
+
+template <typename E, typename W>
+static int vite_read(const char* path, int64_t* nv, int64_t* ne, int64_t* off, int64_t* tails) {
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return -1;
+  E hdr[2];
+  if (std::fread(hdr, sizeof(E), 2, f) != 2) {
+    std::fclose(f);
+    return -2;
+  }
+  *nv = hdr[0];
+  *ne = hdr[1];
+  if (!off) {
+    std::fclose(f);
+    return 0;
+  }
+  std::vector<E> o(*nv + 1);
+  if (std::fread(o.data(), sizeof(E), *nv + 1, f) != (size_t)(*nv + 1)) {
+    std::fclose(f);
+    return -3;
+  }
+  for (int64_t i = 0; i <= *nv; i++) off[i] = (int64_t)o[i] - (int64_t)o[0];
+  struct Rec {
+    E tail;
+    W w;
+  };
+  std::vector<Rec> buf(1 << 16);
+  for (int64_t e = 0; e < *ne;) {
+    int64_t k = std::min<int64_t>(*ne - e, (int64_t)buf.size());
+    if (std::fread(buf.data(), sizeof(Rec), k, f) != (size_t)k) {
+      std::fclose(f);
+      return -4;
+    }
+    for (int64_t t = 0; t < k; t++) tails[e + t] = (int64_t)buf[t].tail;
+    e += k;
+  }
+  std::fclose(f);
+  return 0;
+}
+
+extern "C" int or_vite_write(const char* path, int64_t nv, const int64_t* off,
+                             const int64_t* tails, int layout32) {
+  return layout32 ? vite_write<int32_t, float>(path, nv, off, tails)
+                  : vite_write<int64_t, double>(path, nv, off, tails);
+}
+
+extern "C" int or_vite_read(const char* path, int layout32, int64_t* nv, int64_t* ne,
+                            int64_t* off, int64_t* tails) {
+  return layout32 ? vite_read<int32_t, float>(path, nv, ne, off, tails)
+                  : vite_read<int64_t, double>(path, nv, ne, off, tails);
+}
+
+// ---------------------------------------------------------------------------
+// This build's own generators.  The mixing function is the SplitMix64 output
+// function (Steele, Lea, Flood, "Fast splittable pseudorandom number
+// generators", OOPSLA 2014; constants as in Vigna's public splitmix64.c).
+extern "C" uint64_t or_mix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+extern "C" void or_rmat_thresholds(double a, double b, double c, uint32_t* t) {
+  const double s = 4294967296.0;
+  t[0] = (uint32_t)std::floor(a * s);
+  t[1] = (uint32_t)std::floor((a + b) * s);
+  t[2] = (uint32_t)std::floor((a + b + c) * s);
+}
+
+static inline uint32_t rmat_perm(uint32_t x, int scale, uint64_t key) {
+  // Bijection on [0, 2^scale): three rounds of odd multiply + add mod 2^scale
+  // followed by an xor-shift right (each step is invertible on scale bits).
+  const uint64_t mask = (scale >= 64) ? ~0ULL : ((1ULL << scale) - 1);
+  const int h = (scale + 1) / 2;
+  uint64_t y = x;
+  for (int r = 0; r < 3; r++) {
+    const uint64_t m = or_mix64(key + 1000 + r) | 1ULL;
+    const uint64_t k = or_mix64(key + 2000 + r);
+    y = (y * m + k) & mask;
+    y ^= y >> h;
+  }
+  return (uint32_t)y;
+}
+
+extern "C" void or_rmat_tuples(int scale, uint64_t seed, const uint32_t* thr, int permute,
+                               int64_t start, int64_t count, uint32_t* src, uint32_t* dst) {
+  const uint64_t key = or_mix64(seed);
+  for (int64_t t = 0; t < count; t++) {
+    const uint64_t e = (uint64_t)(start + t);
+    uint32_t u = 0, v = 0;
+    uint64_t w = 0;
+    for (int l = 0; l < scale; l++) {
+      if ((l & 1) == 0) w = or_mix64(key + e * 32 + (uint64_t)(l >> 1));
+      const uint32_t r = (l & 1) ? (uint32_t)(w >> 32) : (uint32_t)w;
+      uint32_t ub, vb;
+      if (r < thr[0]) {
+        ub = 0; vb = 0;
+      } else if (r < thr[1]) {
+        ub = 0; vb = 1;
+      } else if (r < thr[2]) {
+        ub = 1; vb = 0;
+      } else {
+        ub = 1; vb = 1;
+      }
+      u = (u << 1) | ub;
+      v = (v << 1) | vb;
+    }
+    if (permute) {
+      u = rmat_perm(u, scale, key);
+      v = rmat_perm(v, scale, key);
+    }
+    src[t] = u;
+    dst[t] = v;
+  }
+}
+
+extern "C" int64_t or_rmat_csr(int scale, int edgefactor, uint64_t seed, const uint32_t* thr,
+                               int permute, int64_t* off, int32_t* tails, int64_t cap) {
+  const int64_t nv = 1LL << scale;
+  const int64_t nt = (int64_t)edgefactor * nv;
+  std::vector<uint32_t> s(nt), d(nt);
+  or_rmat_tuples(scale, seed, thr, permute, 0, nt, s.data(), d.data());
+  std::vector<uint64_t> keys;
+  keys.reserve(2 * nt);
+  for (int64_t t = 0; t < nt; t++) {
+    if (s[t] == d[t]) continue;
+    keys.push_back(((uint64_t)s[t] << 32) | d[t]);
+    keys.push_back(((uint64_t)d[t] << 32) | s[t]);
+  }
+  std::sort(keys.begin(), keys.end());
+  keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+  std::fill(off, off + nv + 1, 0);
+  for (uint64_t k : keys) off[(k >> 32) + 1]++;
+  for (int64_t i = 0; i < nv; i++) off[i + 1] += off[i];
+  const int64_t ne = (int64_t)keys.size();
+  if (tails && ne <= cap)
+    for (int64_t e = 0; e < ne; e++) tails[e] = (int32_t)(keys[e] & 0xffffffffULL);
+  return ne;
+}
+
+extern "C" int64_t or_er_csr(int64_t n, uint64_t seed, uint64_t thr64, int64_t* off,
+                             int32_t* tails, int64_t cap) {
+  const uint64_t key = or_mix64(seed);
+  std::vector<std::vector<int32_t>> up(n);
+  for (int64_t i = 0; i < n; i++)
+    for (int64_t j = i + 1; j < n; j++)
+      if ((or_mix64(key + (uint64_t)i * (uint64_t)n + (uint64_t)j) >> 32) < thr64)
+        up[i].push_back((int32_t)j);
+  std::vector<int64_t> deg(n, 0);
+  for (int64_t i = 0; i < n; i++) {
+    deg[i] += (int64_t)up[i].size();
+    for (int32_t j : up[i]) deg[j]++;
+  }
+  off[0] = 0;
+  for (int64_t i = 0; i < n; i++) off[i + 1] = off[i] + deg[i];
+  const int64_t ne = off[n];
+  if (tails && ne <= cap) {
+    std::vector<int64_t> cur(off, off + n);
+    for (int64_t i = 0; i < n; i++)
+      for (int32_t j : up[i]) tails[cur[j]++] = (int32_t)i;
+    for (int64_t i = 0; i < n; i++)
+      for (int32_t j : up[i]) tails[cur[i]++] = j;
+  }
+  return ne;
+}
+
+extern "C" void or_batch_params(uint64_t base_seed, int64_t g, int64_t nmin, int64_t nmax,
+                                int64_t* n, int* kind, uint64_t* gseed, uint64_t* er_thr64) {
+  const uint64_t gk = or_mix64(or_mix64(base_seed) + (uint64_t)g);
+  const double u1 = (double)(or_mix64(gk + 1) >> 11) * 0x1.0p-53;
+  const double ln0 = std::log((double)nmin), ln1 = std::log((double)nmax);
+  int64_t nn = (int64_t)std::llround(std::exp(ln0 + u1 * (ln1 - ln0)));
+  if (nn < nmin) nn = nmin;
+  if (nn > nmax) nn = nmax;
+  *n = nn;
+  *kind = (int)(g & 1);
+  *gseed = gk;
+  const double u2 = (double)(or_mix64(gk + 2) >> 11) * 0x1.0p-53;
+  const double deg = 4.0 + 60.0 * u2;
+  const double p = nn > 1 ? deg / (double)(nn - 1) : 0.0;
+  *er_thr64 = p >= 1.0 ? (1ULL << 32) : (uint64_t)std::floor(p * 4294967296.0);
+}

@@ -1,0 +1,93 @@
+/*
+ * oracle.h -- CPU restatement of the xxtric counting path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (xxtric_amd/) links,
+ * loads or calls this library; only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py do, and there only as the checker.
+ *
+ * Every function restates a piece of the reference (sg0/xxtric) and cites the
+ * file:line it follows, or is one of this build's own generators (R-MAT, ER),
+ * restated here on the host so the device generators can be checked against it.
+ *
+ * Parity pins (what checks each function; see tests/test_oracle_*.py):
+ *   or_count_reference      golden counts from the reference binary (oracle/_ref),
+ *                           brute force O(n^3), closed forms (K_n, bipartite, grid),
+ *                           scipy row sums of (U*U).*U for per-vertex counts.
+ *   or_count_degree_ordered equality with or_count_reference (global + per-vertex)
+ *                           and with the closed forms above.
+ *   or_rgg_lcg              golden edge and triangle counts from the reference binary.
+ *   or_vite_read/write      byte-layout round trip and reading by the reference binary.
+ *   or_rmat_csr / or_er_csr the build's own generators: pinned by invariants
+ *                           (symmetry, no self-loops, sorted unique rows, edge-count
+ *                           bounds) and by golden counts of the reference binary
+ *                           reading their output as vite files.
+ */
+#ifndef XXTRIC_ORACLE_H
+#define XXTRIC_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* std::seed_seq{seed}.generate() first word (utils.hpp:37-44 reseeder). */
+uint32_t or_reseeder(uint32_t seed);
+
+/* RGG radius of graph.hpp:789-791. */
+double or_rgg_radius_ref(int64_t nv);
+
+/* LCG coordinates of utils.hpp:245-430 at one rank: X[0..n), Y[0..n). */
+void or_rgg_lcg_coords(int64_t n, uint32_t seed, double* X, double* Y);
+
+/* O(n^2) RGG of graph.hpp:919-948 + CSR of 1289-1341 (one rank, LCG mode).
+ * radius <= 0 selects the reference formula.  Returns the number of directed
+ * entries ne; fills off[n+1] always and tails[0..ne) when ne <= cap. */
+int64_t or_rgg_lcg(int64_t n, uint32_t seed, double radius, int64_t* off,
+                   int64_t* tails, int64_t cap, int nthreads);
+
+/* Reference CPU count (xxtric.hpp:60-208, check_edgelist 516-545, pruning
+ * 569-588) over vertices vstart, vstart+vstride, ... .  per_vertex (nullable)
+ * receives tcount[i] (min-vertex attribution) for processed vertices.
+ * nthreads > 1 uses OpenMP schedule(static) as xxtric.hpp:84. */
+uint64_t or_count_reference(int64_t nv, const int64_t* off, const int64_t* tails,
+                            uint64_t* per_vertex, int nthreads, int64_t vstart,
+                            int64_t vstride);
+uint64_t or_count_reference32(int64_t nv, const int64_t* off, const int32_t* tails,
+                              uint64_t* per_vertex, int nthreads, int64_t vstart,
+                              int64_t vstride);
+
+/* Independent exact counter: (degree, id) orientation + sorted merge,
+ * per-vertex credit to the smallest id of each triangle. */
+uint64_t or_count_degree_ordered(int64_t nv, const int64_t* off, const int32_t* tails,
+                                 uint64_t* per_vertex, int nthreads);
+
+/* vite binary edge list (graph.hpp:294-395): int64 nv, ne, (nv+1) offsets,
+ * ne x {int64 tail, double weight};  layout32: int32 / float (deftypes.hpp:40-45). */
+int or_vite_write(const char* path, int64_t nv, const int64_t* off,
+                  const int64_t* tails, int layout32);
+/* Reads the header only when off == NULL; returns 0 on success. */
+int or_vite_read(const char* path, int layout32, int64_t* nv, int64_t* ne,
+                 int64_t* off, int64_t* tails);
+
+/* ---- this build's own seeded generators (host twins of the device ones) ---- */
+uint64_t or_mix64(uint64_t z);
+void or_rmat_thresholds(double a, double b, double c, uint32_t* t);
+/* Raw R-MAT tuples [start, start+count) (before cleaning). */
+void or_rmat_tuples(int scale, uint64_t seed, const uint32_t* thr, int permute,
+                    int64_t start, int64_t count, uint32_t* src, uint32_t* dst);
+/* Cleaned symmetric CSR: self-loops dropped, duplicates merged, rows sorted. */
+int64_t or_rmat_csr(int scale, int edgefactor, uint64_t seed, const uint32_t* thr,
+                    int permute, int64_t* off, int32_t* tails, int64_t cap);
+/* G(n,p) with p given as a 32.32 fixed-point threshold: edge {i<j} iff
+ * (mix64(key + i*n + j) >> 32) < thr64. */
+int64_t or_er_csr(int64_t n, uint64_t seed, uint64_t thr64, int64_t* off,
+                  int32_t* tails, int64_t cap);
+
+/* Batch graph parameters of graph g (log-uniform n, RGG even / ER odd). */
+void or_batch_params(uint64_t base_seed, int64_t g, int64_t nmin, int64_t nmax,
+                     int64_t* n, int* kind, uint64_t* gseed, uint64_t* er_thr64);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,105 @@
+"""CPU tests of the product library that need no GPU: exports, host ingest, vite I/O.
+
+The host ingest of libxtric (cell-grid RGG, vite reader/writer, edge-list
+converter, batch mix) is checked against the oracle and the golden fixtures.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle_py as O
+import xxtric_amd
+from helpers import csr_sha256
+from xxtric_amd import capi
+
+
+def test_library_exports_every_declared_function():
+    L = capi.lib()
+    names = capi.header_functions()
+    assert len(names) >= 40
+    missing = [n for n in names if getattr(L, n, None) is None]
+    assert missing == []
+    out = subprocess.run(["nm", "-D", "--defined-only", capi.LIB_PATH], capture_output=True,
+                         text=True).stdout
+    exported = {ln.split()[-1] for ln in out.splitlines() if " T " in ln}
+    assert set(names) <= exported
+
+
+def test_library_is_built_for_gfx950():
+    # the embedded device code object names its target (amdgcn-amd-amdhsa--gfx950)
+    blob = open(capi.LIB_PATH, "rb").read()
+    assert b"amdgcn-amd-amdhsa--gfx950" in blob
+
+
+def test_no_gpu_fails_loudly_here():
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("a GPU node may be present")
+    with pytest.raises(xxtric_amd.XtError):
+        xxtric_amd.Context(0)
+
+
+@pytest.mark.parametrize("idx", range(5))
+def test_rgg_cell_grid_matches_reference_all_pairs(golden, idx):
+    g = golden["rgg_lcg"][idx]
+    off, tails = xxtric_amd.rgg_generate(g["n"], seed=g["seed"])
+    assert off[-1] == g["ne"] and csr_sha256(off, tails) == g["csr_sha256"]
+
+
+@pytest.mark.parametrize("n,seed,radius", [(3000, 7, 0.0), (5000, 3, 8.5612e-3), (2000, 11, 0.05)])
+def test_rgg_cell_grid_other_seeds_and_radii(n, seed, radius):
+    o1, t1 = xxtric_amd.rgg_generate(n, seed=seed, radius=radius)
+    o2, t2 = O.rgg_lcg(n, seed=seed, radius=radius)
+    assert (o1 == o2).all() and (t1 == t2).all()
+
+
+def test_rgg_radius_matches_oracle():
+    for n in (256, 4096, 100000, 10 ** 7):
+        assert xxtric_amd.rgg_radius(n) == O.rgg_radius_ref(n)
+
+
+@pytest.mark.parametrize("g", list(range(64)))
+def test_batch_params_match_oracle(g):
+    assert xxtric_amd.batch_params(2026, g)[:4] == O.batch_params(2026, g)
+
+
+@pytest.mark.parametrize("idx", range(8))
+def test_batch_graphs_match_golden(golden, idx):
+    g = golden["batch"][idx]
+    off, tails = xxtric_amd.batch_graph_csr(g["base_seed"], g["index"])
+    assert csr_sha256(off, tails) == g["csr_sha256"]
+
+
+@pytest.mark.parametrize("layout32", [False, True])
+def test_vite_product_and_oracle_agree(tmp_path, layout32):
+    off, tails = O.rmat_csr(10, 16, 1, True)
+    p1 = str(tmp_path / "prod.bin")
+    p2 = str(tmp_path / "orac.bin")
+    xxtric_amd.vite_write(p1, off, tails, layout32=layout32)
+    O.vite_write(p2, off, tails, layout32=layout32)
+    assert open(p1, "rb").read() == open(p2, "rb").read()
+    o, t = xxtric_amd.vite_read(p2, layout32=layout32)
+    assert (o == off).all() and (t == tails).all()
+
+
+def test_vite_reader_rebases_offsets(tmp_path):
+    # graph.hpp:386-388: offsets are made relative to the first one
+    p = str(tmp_path / "g.bin")
+    nv, ne = 3, 4
+    with open(p, "wb") as f:
+        np.array([nv, ne], np.int64).tofile(f)
+        np.array([10, 12, 13, 14], np.int64).tofile(f)
+        rec = np.zeros(ne, dtype=[("t", np.int64), ("w", np.float64)])
+        rec["t"] = [1, 2, 0, 0]
+        rec.tofile(f)
+    o, t = xxtric_amd.vite_read(p)
+    assert list(o) == [0, 2, 3, 4] and list(t) == [1, 2, 0, 0]
+
+
+def test_edgelist_converter(tmp_path):
+    p = tmp_path / "e.tsv"
+    p.write_text("# comment\n1\t2\t1\n2\t3\t1\n3\t1\t1\n3\t4\t1\n2\t1\t1\n")
+    off, tails = xxtric_amd.edgelist_to_csr(str(p))
+    assert list(off) == [0, 0, 2, 4, 7, 8]
+    assert list(tails) == [2, 3, 1, 3, 1, 2, 4, 3]

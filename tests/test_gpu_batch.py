@@ -1,0 +1,127 @@
+"""GPU: batched counting (persistent pipeline, hipGraph replay, per-graph baseline).
+
+Every per-graph total must equal the oracle's count of that graph, whatever
+the submission order, the kernel split and the launch mode.
+"""
+import numpy as np
+import pytest
+
+import oracle_py as O
+import xxtric_amd
+from xxtric_amd import XT_BATCH_HIPGRAPH, XT_BATCH_PER_GRAPH, XT_BATCH_PERSISTENT
+
+pytestmark = pytest.mark.gpu
+
+SEED = 2026
+
+
+def _say(*a):
+    print("[case]", *a, flush=True)
+
+
+def _expected(gs, nmax=20000):
+    out = []
+    for g in gs:
+        off, tails = xxtric_amd.batch_graph_csr(SEED, g, 200, nmax)
+        out.append(O.count_degree_ordered(off, tails))
+    return np.array(out, np.uint64)
+
+
+def test_generated_batch_matches_oracle(ctx, golden):
+    n = 48
+    b = xxtric_amd.Batch(ctx, 4_000_000, 40_000_000, 4096)
+    _say("batch generated", n)
+    b.add_generated(SEED, 0, n)
+    b.run(XT_BATCH_PERSISTENT)
+    got = b.results()
+    exp = _expected(range(n))
+    assert (got == exp).all(), np.nonzero(got != exp)[0]
+    for g in golden["batch"]:
+        assert int(got[g["index"]]) == g["triangles"]
+    b.close()
+
+
+def test_modes_agree_and_graph_replays(ctx):
+    n = 40
+    b = xxtric_amd.Batch(ctx, 4_000_000, 40_000_000, 4096)
+    b.add_generated(SEED, 100, n)
+    _say("persistent")
+    b.run(XT_BATCH_PERSISTENT)
+    ref = b.results().copy()
+    _say("per-graph")
+    b.run(XT_BATCH_PER_GRAPH)
+    assert (b.results() == ref).all()
+    for rep in range(4):  # replays interleaved with eager work
+        _say("hipgraph replay", rep)
+        b.run(XT_BATCH_HIPGRAPH)
+        assert (b.results() == ref).all()
+        b.run(XT_BATCH_PERSISTENT)
+        assert (b.results() == ref).all()
+    assert (ref == _expected(range(100, 100 + n))).all()
+    b.close()
+
+
+def test_permuted_order_and_split(ctx):
+    gs = list(range(200, 232))
+    graphs = [xxtric_amd.batch_graph_csr(SEED, g) for g in gs]
+    exp = _expected(gs)
+    perm = np.random.default_rng(5).permutation(len(gs))
+    for cut in (1, 16, 64, 1 << 30):
+        ctx.tune(small_cut=cut)
+        try:
+            b = xxtric_amd.Batch(ctx, 2_000_000, 20_000_000, 1024)
+            for k in perm:
+                b.add(*graphs[k])
+            _say("permuted batch small_cut", cut)
+            b.run(XT_BATCH_PERSISTENT)
+            got = b.results()
+            assert (got == exp[perm]).all()
+            b.close()
+        finally:
+            ctx.tune(small_cut=64)
+
+
+def test_solo_equals_in_batch(ctx):
+    off, tails = xxtric_amd.batch_graph_csr(SEED, 7)
+    solo = ctx.count_csr(off, tails.astype(np.int64))
+    b = xxtric_amd.Batch(ctx, 2_000_000, 20_000_000, 1024)
+    b.add_generated(SEED, 300, 10)
+    b.add(off, tails)
+    b.add_generated(SEED, 400, 10)
+    _say("solo vs batch")
+    b.run(XT_BATCH_PERSISTENT)
+    assert int(b.results()[10]) == solo
+    b.close()
+
+
+def test_mixed_sizes_with_rmat(ctx):
+    g = ctx.rmat(14, 16, 3, True)
+    off, tails = g.download()
+    solo = g.count()
+    g.close()
+    b = xxtric_amd.Batch(ctx, 2_000_000, 20_000_000, 1024)
+    b.add_generated(SEED, 0, 5)
+    b.add(off, tails)
+    b.add_generated(SEED, 5, 5)
+    _say("batch with rmat14")
+    b.run(XT_BATCH_PERSISTENT)
+    got = b.results()
+    assert int(got[5]) == solo == O.count_degree_ordered(off, tails)
+    assert (np.delete(got, 5) == _expected(range(10))).all()
+    b.close()
+
+
+def test_capacity_errors(ctx):
+    b = xxtric_amd.Batch(ctx, 1000, 10000, 2)
+    off, tails = xxtric_amd.batch_graph_csr(SEED, 5)   # 1139 vertices: over the vertex capacity
+    with pytest.raises(xxtric_amd.XtError) as e:
+        b.add(off, tails)
+    assert e.value.code == -5
+    small = (np.array([0, 2, 4, 6], np.int64), np.array([1, 2, 0, 2, 0, 1], np.int32))
+    b.add(*small)
+    b.add(*small)
+    with pytest.raises(xxtric_amd.XtError):
+        b.add(*small)                                   # graph slots exhausted
+    b.run(XT_BATCH_PERSISTENT)
+    assert list(b.results()) == [1, 1]
+    b.close()

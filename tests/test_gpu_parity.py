@@ -1,0 +1,239 @@
+"""GPU parity: the HIP path against the oracle and the reference's golden counts.
+
+Bar: bit-exact global and per-vertex counts (integer work).  Each case prints
+its name before the device call so a failing run names it.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_py as O
+import xxtric_amd
+from helpers import bipartite, complete, csr_from_edges, csr_sha256, grid, random_graph, wheel
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def _check_csr(ctx, off, tails, name, per_vertex=True):
+    exp, exp_pv = O.count_reference(off, tails, per_vertex=True)
+    for orient in ("degree", "id"):
+        _say(name, orient, "nv=%d ne=%d" % (len(off) - 1, len(tails)))
+        tot, pv = ctx.count_csr(off, tails, per_vertex=True, orient=orient, validate=True)
+        assert tot == exp, (name, orient, tot, exp)
+        assert (pv == exp_pv).all(), (name, orient, np.nonzero(pv != exp_pv)[0][:10])
+        assert ctx.count_csr(off, tails, orient=orient) == exp
+    return exp
+
+
+@pytest.mark.parametrize("idx", range(5))
+def test_rgg_lcg_golden(ctx, golden, idx):
+    g = golden["rgg_lcg"][idx]
+    off, tails = xxtric_amd.rgg_generate(g["n"], seed=g["seed"])
+    assert csr_sha256(off, tails) == g["csr_sha256"]
+    assert _check_csr(ctx, off, tails, "rgg%d" % g["n"]) == g["triangles"]
+
+
+@pytest.mark.parametrize("idx", range(8))
+def test_rmat_golden_host_csr(ctx, golden, idx):
+    g = golden["rmat"][idx]
+    off, tails = O.rmat_csr(g["scale"], g["edgefactor"], g["seed"], bool(g["permute"]))
+    if g["scale"] <= 14:
+        assert _check_csr(ctx, off, tails.astype(np.int64), "rmat%d" % g["scale"]) == g["triangles"]
+    else:
+        _say("rmat host", g["scale"])
+        exp, exp_pv = O.count_degree_ordered(off, tails, per_vertex=True)
+        tot, pv = ctx.count_csr(off, tails.astype(np.int64), per_vertex=True)
+        assert tot == exp == g["triangles"] and (pv == exp_pv).all()
+
+
+@pytest.mark.parametrize("idx", range(8))
+def test_rmat_device_generator(ctx, golden, idx):
+    g = golden["rmat"][idx]
+    _say("rmat device gen", g["scale"], g["seed"], g["permute"])
+    dg = ctx.rmat(g["scale"], g["edgefactor"], g["seed"], bool(g["permute"]))
+    off, tails = dg.download()
+    assert dg.ne == g["ne"]
+    assert csr_sha256(off, tails) == g["csr_sha256"]
+    tot, pv = dg.count(per_vertex=True)
+    assert tot == g["triangles"]
+    exp_tot, exp_pv = O.count_degree_ordered(off, tails, per_vertex=True)
+    assert (pv == exp_pv).all()
+    assert dg.count(orient="id") == g["triangles"]
+    dg.close()
+
+
+@pytest.mark.parametrize("idx", range(3))
+def test_er_device_generator(ctx, golden, idx):
+    g = golden["er"][idx]
+    _say("er device gen", g["n"])
+    dg = ctx.er(g["n"], g["seed"], g["thr64"])
+    off, tails = dg.download()
+    assert csr_sha256(off, tails) == g["csr_sha256"]
+    assert dg.count() == g["triangles"]
+    dg.close()
+
+
+@pytest.mark.parametrize("idx", range(8))
+def test_batch_graphs_golden(ctx, golden, idx):
+    g = golden["batch"][idx]
+    off, tails = xxtric_amd.batch_graph_csr(g["base_seed"], g["index"])
+    assert csr_sha256(off, tails) == g["csr_sha256"]
+    assert _check_csr(ctx, off, tails.astype(np.int64), "batch%d" % idx) == g["triangles"]
+
+
+@pytest.mark.parametrize("name,graph", [
+    ("K3", complete(3)), ("K40", complete(40)), ("K300", complete(300)),
+    ("bip", bipartite(30, 40)), ("grid", grid(20, 30)), ("wheel", wheel(500)),
+    ("path", csr_from_edges(5, [(0, 1), (1, 2), (2, 3), (3, 4)])),
+    ("loops", csr_from_edges(6, [(0, 0), (0, 1), (1, 2), (0, 2), (2, 2), (5, 5), (3, 4)])),
+    ("isolated", csr_from_edges(50, [(10, 20), (20, 30), (10, 30), (40, 41)])),
+    ("single_edge", csr_from_edges(2, [(0, 1)])),
+    ("no_edges", (np.zeros(8, np.int64), np.zeros(0, np.int64))),
+    ("rand", random_graph(300, 0.2, 7)),
+])
+def test_small_graphs(ctx, name, graph):
+    _check_csr(ctx, graph[0], graph[1], name)
+
+
+def test_empty_graph(ctx):
+    _say("empty")
+    assert ctx.count_csr(np.zeros(1, np.int64), np.zeros(0, np.int64)) == 0
+
+
+def test_star_hub(ctx):
+    n = 200001  # one hub joined to every other vertex: a 200000-entry row, 0 triangles
+    off = np.zeros(n + 1, np.int64)
+    off[1] = n - 1
+    off[2:] = n - 1 + np.arange(1, n)
+    tails = np.concatenate([np.arange(1, n), np.zeros(n - 1, np.int64)])
+    _say("star")
+    assert ctx.count_csr(off, tails, validate=True) == 0
+    # add a long path among the leaves: n-2 triangles, all through the hub
+    e = [(0, i) for i in range(1, n)] + [(i, i + 1) for i in range(1, n - 1)]
+    o2, t2 = csr_from_edges(n, e)
+    tot, pv = ctx.count_csr(o2, t2, per_vertex=True)
+    assert tot == n - 2 and int(pv[0]) == n - 2
+
+
+@pytest.mark.parametrize("small_cut", [1, 4, 64, 1 << 30])
+def test_kernel_split_does_not_change_counts(ctx, small_cut):
+    off, tails = O.rmat_csr(14, 16, 1, True)
+    exp, exp_pv = O.count_degree_ordered(off, tails, per_vertex=True)
+    ctx.tune(small_cut=small_cut)
+    try:
+        _say("small_cut", small_cut)
+        tot, pv = ctx.count_csr(off, tails.astype(np.int64), per_vertex=True)
+        assert tot == exp and (pv == exp_pv).all()
+        t = ctx.timing()
+        if small_cut == 1:
+            assert t["n_deferred"] > 0
+    finally:
+        ctx.tune(small_cut=64)
+
+
+def test_invalid_inputs_are_rejected(ctx):
+    off, tails = csr_from_edges(4, [(0, 1), (1, 2), (0, 2), (2, 3)])
+    bad = tails.copy()
+    bad[0], bad[1] = bad[1], bad[0]                   # unsorted row 0
+    _say("invalid: unsorted")
+    with pytest.raises(xxtric_amd.XtError) as e:
+        ctx.count_csr(off, bad, validate=True)
+    assert e.value.code == -4
+    dup_off = off.copy()
+    dup = tails.copy()
+    dup[1] = dup[0]                                   # duplicate entry in row 0
+    with pytest.raises(xxtric_amd.XtError):
+        ctx.count_csr(dup_off, dup, validate=True)
+    oob = tails.copy()
+    oob[-1] = 99                                      # id out of range: rejected even without validation
+    with pytest.raises(xxtric_amd.XtError):
+        ctx.count_csr(off, oob)
+    asym_off = np.array([0, 2, 3, 4, 4], np.int64)
+    asym = np.array([1, 2, 2, 0], np.int64)           # (1,2) without (2,1) ...
+    with pytest.raises(xxtric_amd.XtError):
+        ctx.count_csr(asym_off, asym, symmetry=True)
+    badoff = off.copy()
+    badoff[2] = badoff[3] + 1
+    with pytest.raises(xxtric_amd.XtError):
+        ctx.count_csr(badoff, tails)
+    # the context still works afterwards
+    assert ctx.count_csr(off, tails) == 1
+
+
+def test_reference_ffi_shapes(ctx):
+    """xt_single_gpu_count / _masked / local_count_gpu_inner on device int64 arrays."""
+    off, tails = O.rmat_csr(12, 16, 1, True)
+    tails = tails.astype(np.int64)
+    exp = O.count_reference(off, tails)
+    zero = np.zeros(1, np.int64)
+    d_ind = ctx.to_device(off)
+    d_el = ctx.to_device(tails)
+    d_cnt = ctx.to_device(zero)
+    _say("ffi single_gpu_count")
+    ctx.single_gpu_count(len(off) - 1, d_ind.ptr, d_el.ptr, d_cnt.ptr)
+    ctx.single_gpu_count(len(off) - 1, d_ind.ptr, d_el.ptr, d_cnt.ptr)
+    ctx.sync()
+    assert int(d_cnt.download(np.int64, 1)[0]) == 2 * exp   # adds into the caller's counter
+    # host-built edge mask as xxtric-gpu.hpp:91-107: (e1, m) for i < N[m], m != e1-1
+    mask = []
+    nv = len(off) - 1
+    for i in range(nv):
+        e0, e1 = off[i], off[i + 1]
+        if e0 + 1 == e1:
+            continue
+        for m in range(e0, e1):
+            if i >= tails[m]:
+                continue
+            if m != e1 - 1:
+                mask += [e1, m]
+    d_mask = ctx.to_device(np.array(mask, np.int64))
+    d_cnt.upload(zero)
+    _say("ffi masked")
+    ctx.single_gpu_count_masked(len(mask) // 2, d_mask.ptr, d_ind.ptr, d_el.ptr, d_cnt.ptr)
+    ctx.sync()
+    assert int(d_cnt.download(np.int64, 1)[0]) == exp
+    # one launch per oriented edge, as USE_SPLIT_KERNELS (xxtric-gpu.hpp:204-227), on a small graph
+    o2, t2 = random_graph(60, 0.3, 3)
+    exp2 = O.count_reference(o2, t2)
+    d_el2 = ctx.to_device(np.ascontiguousarray(t2, np.int64))
+    d_cnt.upload(zero)
+    _say("ffi inner")
+    for i in range(len(o2) - 1):
+        e0, e1 = o2[i], o2[i + 1]
+        for m in range(e0, e1 - 1):
+            v = t2[m]
+            if i >= v:
+                continue
+            ctx.local_count_gpu_inner(int(m + 1), int(e1), int(o2[v]), int(o2[v + 1]), d_el2.ptr,
+                                      d_cnt.ptr)
+    ctx.sync()
+    assert int(d_cnt.download(np.int64, 1)[0]) == exp2
+    for b in (d_ind, d_el, d_cnt, d_mask, d_el2):
+        b.close()
+
+
+def test_resident_graph_and_alg_bytes(ctx):
+    off, tails = O.rmat_csr(12, 16, 1, True)
+    g = ctx.graph(off, tails, validate=True, symmetry=True)
+    exp = O.count_degree_ordered(off, tails)
+    _say("resident")
+    assert g.count() == exp and g.count(orient="id") == exp
+    b, sm, nep = g.alg_bytes()
+    assert nep == len(tails) // 2
+    # independent sum of min(d+(u), d+(v)) under (degree, id) orientation
+    n = len(off) - 1
+    deg = np.diff(off)
+    rows = np.repeat(np.arange(n), deg)
+    key_u = deg[rows] * (1 << 32) + rows
+    key_v = deg[tails] * (1 << 32) + tails
+    keep = key_v > key_u
+    dplus = np.bincount(rows[keep], minlength=n)
+    s = np.minimum(dplus[rows[keep]], dplus[tails[keep]]).sum()
+    assert sm == s and b == 8 * (n + 1) + 4 * nep + 4 * s
+    g.close()

@@ -1,0 +1,125 @@
+"""Batched counting of many small graphs and the continuous-batching server.
+
+Thin wrappers over xt_batch_* and xt_serve_* of libxtric.so.
+"""
+import ctypes
+
+import numpy as np
+
+from . import capi
+from .capi import XtError
+
+
+class Batch:
+    """Many graphs counted by one persistent device pipeline (xt_batch)."""
+
+    def __init__(self, ctx, max_vertices, max_edges, max_graphs):
+        self.ctx = ctx
+        self._L = ctx._L
+        h = ctypes.c_void_p()
+        ctx._check(self._L.xt_batch_create(ctx.h, max_vertices, max_edges, max_graphs,
+                                           ctypes.byref(h)))
+        self.h = h
+        self.n = 0
+
+    def add(self, off, tails):
+        off = np.ascontiguousarray(off, np.int64)
+        tails = np.ascontiguousarray(tails, np.int32)
+        slot = ctypes.c_int64()
+        self.ctx._check(self._L.xt_batch_add(self.h, len(off) - 1, len(tails), capi.ptr(off),
+                                             capi.ptr(tails), ctypes.byref(slot)))
+        self.n += 1
+        return slot.value
+
+    def add_generated(self, base_seed, first, count, nmin=200, nmax=20000):
+        self.ctx._check(self._L.xt_batch_add_generated(self.h, base_seed, first, count, nmin, nmax))
+        self.n += count
+
+    def run(self, mode=capi.XT_BATCH_PERSISTENT, orient="degree"):
+        f = capi.XT_ORIENT_ID if orient == "id" else capi.XT_ORIENT_DEGREE
+        self.ctx._check(self._L.xt_batch_run(self.h, mode, f))
+
+    def results(self):
+        out = np.zeros(self.n, np.uint64)
+        self.ctx._check(self._L.xt_batch_results(self.h, capi.ptr(out), self.n))
+        return out
+
+    def timing(self):
+        t = capi.XtBatchTiming()
+        self.ctx._check(self._L.xt_batch_timing_get(self.h, ctypes.byref(t)))
+        return {k: getattr(t, k) for k, _ in t._fields_}
+
+    def clear(self):
+        self.ctx._check(self._L.xt_batch_clear(self.h))
+        self.n = 0
+
+    def close(self):
+        if self.h:
+            self._L.xt_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching request loop (xt_serve)."""
+
+    def __init__(self, ctx, hbm_budget_bytes=0, max_batch_graphs=0, max_inflight=0, max_queue=0,
+                 max_wait_us=0.0):
+        self.ctx = ctx
+        self._L = ctx._L
+        cfg = capi.XtServeConfig(hbm_budget_bytes, max_batch_graphs, max_inflight, max_queue,
+                                 max_wait_us)
+        h = ctypes.c_void_p()
+        ctx._check(self._L.xt_serve_create(ctx.h, ctypes.byref(cfg), ctypes.byref(h)))
+        self.h = h
+
+    def submit(self, off, tails, arrival_us=-1.0):
+        off = np.ascontiguousarray(off, np.int64)
+        tails = np.ascontiguousarray(tails, np.int32)
+        t = ctypes.c_uint64()
+        rc = self._L.xt_serve_submit(self.h, len(off) - 1, len(tails), capi.ptr(off), capi.ptr(tails),
+                                     arrival_us, ctypes.byref(t))
+        if rc != capi.XT_OK:
+            raise XtError(rc, self._L.xt_last_error(self.ctx.h).decode())
+        return t.value
+
+    def poll(self, ticket):
+        total = ctypes.c_uint64()
+        lat = ctypes.c_double()
+        rc = self._L.xt_serve_poll(self.h, ticket, ctypes.byref(total), ctypes.byref(lat))
+        if rc == capi.XT_PENDING:
+            return None
+        if rc != capi.XT_OK:
+            raise XtError(rc, "xt_serve_poll")
+        return total.value, lat.value
+
+    def drain(self, timeout_s=60.0):
+        rc = self._L.xt_serve_drain(self.h, timeout_s)
+        if rc != capi.XT_OK:
+            raise XtError(rc, "xt_serve_drain")
+
+    def now_us(self):
+        return self._L.xt_serve_now_us(self.h)
+
+    def stats(self):
+        s = capi.XtServeStats()
+        rc = self._L.xt_serve_stats_get(self.h, ctypes.byref(s))
+        if rc != capi.XT_OK:
+            raise XtError(rc, "xt_serve_stats_get")
+        return {k: getattr(s, k) for k, _ in s._fields_}
+
+    def close(self):
+        if self.h:
+            self._L.xt_serve_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,447 @@
+// api.hip -- extern "C" entry points of libxtric (context, single-graph counts,
+// reference lower-FFI shapes, resident graphs).
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "xt_internal.hpp"
+
+namespace xt {
+
+int launch_masked(xt_ctx* ctx, int64_t nedges, const int64_t* mask, const int64_t* ind,
+                  const int64_t* L, int64_t* out, hipStream_t st);
+int launch_inner(xt_ctx* ctx, int64_t start, int64_t end, int64_t jstart, int64_t jend,
+                 const int64_t* L, int64_t* out, hipStream_t st);
+int launch_add_total(xt_ctx* ctx, int64_t* out, hipStream_t st);
+
+int set_err(xt_ctx* ctx, int code, const std::string& msg) {
+  if (ctx) ctx->err = msg;
+  return code;
+}
+
+int hip_err(xt_ctx* ctx, hipError_t e, const char* what) {
+  const int code = (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? XT_ERR_NOMEM : XT_ERR_HIP;
+  return set_err(ctx, code, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+int ensure(xt_ctx* ctx, DevBuf& b, size_t bytes) {
+  if (bytes == 0) bytes = 8;
+  if (b.bytes >= bytes) return XT_OK;
+  if (b.p) {
+    // Work queued on any stream (a caller's stream included) may still read it.
+    XT_HIP(ctx, hipDeviceSynchronize());
+    XT_HIP(ctx, hipFree(b.p));
+    b.p = nullptr;
+    b.bytes = 0;
+  }
+  const size_t want = (bytes + ((size_t)1 << 21) - 1) & ~(((size_t)1 << 21) - 1);
+  hipError_t e = hipMalloc(&b.p, want);
+  if (e != hipSuccess) {
+    b.p = nullptr;
+    return set_err(ctx, XT_ERR_NOMEM, std::string("device allocation of ") + std::to_string(want) +
+                                          " bytes failed: " + hipGetErrorString(e));
+  }
+  b.bytes = want;
+  return XT_OK;
+}
+
+static int check_host_offsets(xt_ctx* ctx, int64_t nv, int64_t ne, const int64_t* off) {
+  if (nv < 0 || ne < 0 || !off) return set_err(ctx, XT_ERR_ARG, "negative size or null offsets");
+  if (nv >= (1LL << 31) - 1) return set_err(ctx, XT_ERR_ARG, "nv must be below 2^31-1 (int32 device ids)");
+  if (off[0] != 0 || off[nv] != ne) return set_err(ctx, XT_ERR_INPUT, "offsets must start at 0 and end at ne");
+  for (int64_t i = 0; i < nv; i++)
+    if (off[i] > off[i + 1]) return set_err(ctx, XT_ERR_INPUT, "offsets are not monotone");
+  return XT_OK;
+}
+
+template <typename T>
+int count_device_csr(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne, const int64_t* off,
+                     const T* tails, uint32_t flags, hipStream_t st, bool timed) {
+  const int mode = (flags & XT_ORIENT_ID) ? kOrientId : kOrientDegree;
+  XT_TRY(ensure(ctx, s.counters, kNumSlots * sizeof(uint64_t)));
+  XT_TRY(fill_u64(ctx, s.counters.as<uint64_t>(), kNumSlots, 0, st));
+  uint64_t* pv = nullptr;
+  if (flags & XT_PER_VERTEX) {
+    XT_TRY(ensure(ctx, s.pv, (size_t)(nv > 0 ? nv : 1) * sizeof(uint64_t)));
+    pv = s.pv.as<uint64_t>();
+  }
+  // Grow every scratch buffer before the first event so no allocation (and
+  // its implicit synchronisation) falls inside the timed window.
+  XT_TRY(ensure(ctx, s.ooff, (size_t)(nv + 1) * sizeof(int64_t)));
+  XT_TRY(ensure(ctx, s.otails, (size_t)(ne > 0 ? ne : 1) * sizeof(int32_t)));
+  if (timed) XT_HIP(ctx, hipEventRecord(ctx->ev[kEvStart], st));
+  XT_TRY(orient<T>(ctx, s, nv, ne, off, tails, mode, st));
+  if (timed) XT_HIP(ctx, hipEventRecord(ctx->ev[kEvOriented], st));
+  if (pv) XT_TRY(fill_u64(ctx, pv, nv, 0, st));
+  XT_TRY(count_oriented(ctx, s, nv, ne, pv, nullptr, nullptr, st));
+  if (timed) XT_HIP(ctx, hipEventRecord(ctx->ev[kEvCounted], st));
+  return XT_OK;
+}
+
+template int count_device_csr<int32_t>(xt_ctx*, Scratch&, int64_t, int64_t, const int64_t*, const int32_t*,
+                                       uint32_t, hipStream_t, bool);
+template int count_device_csr<int64_t>(xt_ctx*, Scratch&, int64_t, int64_t, const int64_t*, const int64_t*,
+                                       uint32_t, hipStream_t, bool);
+
+// Reads the count, the input-error slots and the timing after a count.
+static int finish_count(xt_ctx* ctx, int64_t nv, uint32_t flags, hipStream_t st, bool timed,
+                        uint64_t* total, uint64_t* per_vertex) {
+  uint64_t h[kNumSlots];
+  XT_HIP(ctx, hipMemcpyAsync(h, ctx->s.counters.p, sizeof(h), hipMemcpyDeviceToHost, st));
+  if (per_vertex && (flags & XT_PER_VERTEX) && nv > 0)
+    XT_HIP(ctx, hipMemcpyAsync(per_vertex, ctx->s.pv.p, (size_t)nv * sizeof(uint64_t),
+                               hipMemcpyDeviceToHost, st));
+  XT_HIP(ctx, hipStreamSynchronize(st));
+  if (h[kBadIds]) return set_err(ctx, XT_ERR_INPUT, "neighbour id out of range [0, nv)");
+  if (total) *total = h[kTotal];
+  int64_t nep = 0;
+  if (nv > 0)
+    XT_HIP(ctx, hipMemcpy(&nep, ctx->s.ooff.as<int64_t>() + nv, sizeof(int64_t), hipMemcpyDeviceToHost));
+  ctx->timing.ne_plus = nep;
+  ctx->timing.n_deferred = (int64_t)h[kDeferred];
+  ctx->timing.n_block = (int64_t)h[kBlockEdges];
+  if (timed) {
+    float a = 0, b = 0, c = 0;
+    XT_HIP(ctx, hipEventElapsedTime(&a, ctx->ev[kEvStart], ctx->ev[kEvOriented]));
+    XT_HIP(ctx, hipEventElapsedTime(&b, ctx->ev[kEvOriented], ctx->ev[kEvCounted]));
+    XT_HIP(ctx, hipEventElapsedTime(&c, ctx->ev[kEvStart], ctx->ev[kEvCounted]));
+    ctx->timing.orient_ms = a;
+    ctx->timing.count_ms = b;
+    ctx->timing.total_ms = c;
+  }
+  return XT_OK;
+}
+
+// Host int64 ids -> int32 for the device layout.
+static void narrow_ids(const int64_t* in, int32_t* out, int64_t n) {
+  for (int64_t i = 0; i < n; i++) out[i] = (int32_t)in[i];
+}
+
+static int upload_csr(xt_ctx* ctx, int64_t nv, int64_t ne, const int64_t* off, const void* tails,
+                      int tails_bytes, int64_t* d_off, int32_t* d_tails, hipStream_t st) {
+  XT_HIP(ctx, hipMemcpyAsync(d_off, off, (size_t)(nv + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
+  if (ne == 0) return XT_OK;
+  if (tails_bytes == 4) {
+    XT_HIP(ctx, hipMemcpyAsync(d_tails, tails, (size_t)ne * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    XT_HIP(ctx, hipStreamSynchronize(st));
+    return XT_OK;
+  }
+  const int64_t* t64 = static_cast<const int64_t*>(tails);
+  for (int64_t e = 0; e < ne; e++)
+    if (t64[e] < 0 || t64[e] >= nv) return set_err(ctx, XT_ERR_INPUT, "neighbour id out of range [0, nv)");
+  std::vector<int32_t> tmp((size_t)ne);
+  narrow_ids(t64, tmp.data(), ne);
+  XT_HIP(ctx, hipMemcpyAsync(d_tails, tmp.data(), (size_t)ne * sizeof(int32_t), hipMemcpyHostToDevice, st));
+  XT_HIP(ctx, hipStreamSynchronize(st));
+  return XT_OK;
+}
+
+}  // namespace xt
+
+using namespace xt;
+
+extern "C" {
+
+const char* xt_version(void) { return "xtric 0.1 (gfx950)"; }
+
+int xt_ctx_create(int device, xt_ctx** out) {
+  if (!out) return XT_ERR_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return XT_ERR_HIP;
+  if (device < 0 || device >= ndev) return XT_ERR_ARG;
+  if (hipSetDevice(device) != hipSuccess) return XT_ERR_HIP;
+  xt_ctx* ctx = new xt_ctx();
+  ctx->device = device;
+  int ncu = 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu > 0)
+    ctx->num_cus = ncu;
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete ctx;
+    return XT_ERR_HIP;
+  }
+  for (int i = 0; i < kNumEv; i++)
+    if (hipEventCreate(&ctx->ev[i]) != hipSuccess) {
+      delete ctx;
+      return XT_ERR_HIP;
+    }
+  if (ensure(ctx, ctx->s.counters, kNumSlots * sizeof(uint64_t)) != XT_OK) {
+    delete ctx;
+    return XT_ERR_NOMEM;
+  }
+  *out = ctx;
+  return XT_OK;
+}
+
+int xt_ctx_destroy(xt_ctx* ctx) {
+  if (!ctx) return XT_ERR_ARG;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  release(ctx->s);
+  for (int i = 0; i < kNumEv; i++)
+    if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return XT_OK;
+}
+
+const char* xt_last_error(const xt_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int xt_last_timing(const xt_ctx* ctx, xt_timing* out) {
+  if (!ctx || !out) return XT_ERR_ARG;
+  *out = ctx->timing;
+  return XT_OK;
+}
+
+int xt_ctx_tune(xt_ctx* ctx, int small_cut, int block_cut, int reserved) {
+  (void)reserved;
+  if (!ctx) return XT_ERR_ARG;
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  if (small_cut > 0) ctx->small_cut = small_cut;
+  if (block_cut > 0) ctx->block_cut = block_cut;
+  return XT_OK;
+}
+
+int xt_dev_alloc(xt_ctx* ctx, size_t bytes, void** out) {
+  if (!ctx || !out) return XT_ERR_ARG;
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  XT_HIP(ctx, hipSetDevice(ctx->device));
+  *out = nullptr;
+  XT_HIP(ctx, hipMalloc(out, bytes > 0 ? bytes : 1));
+  return XT_OK;
+}
+
+int xt_dev_free(xt_ctx* ctx, void* p) {
+  if (!ctx) return XT_ERR_ARG;
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  XT_HIP(ctx, hipSetDevice(ctx->device));
+  if (p) {
+    XT_HIP(ctx, hipDeviceSynchronize());
+    XT_HIP(ctx, hipFree(p));
+  }
+  return XT_OK;
+}
+
+int xt_memcpy(xt_ctx* ctx, void* dst, const void* src, size_t bytes, int kind) {
+  if (!ctx || (bytes > 0 && (!dst || !src))) return XT_ERR_ARG;
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  XT_HIP(ctx, hipSetDevice(ctx->device));
+  hipMemcpyKind k = kind == XT_H2D ? hipMemcpyHostToDevice
+                    : kind == XT_D2H ? hipMemcpyDeviceToHost
+                    : kind == XT_D2D ? hipMemcpyDeviceToDevice
+                                     : hipMemcpyDefault;
+  if (kind < XT_H2D || kind > XT_D2D) return set_err(ctx, XT_ERR_ARG, "bad copy kind");
+  if (bytes == 0) return XT_OK;
+  XT_HIP(ctx, hipDeviceSynchronize());
+  XT_HIP(ctx, hipMemcpy(dst, src, bytes, k));
+  return XT_OK;
+}
+
+int xt_dev_sync(xt_ctx* ctx) {
+  if (!ctx) return XT_ERR_ARG;
+  XT_HIP(ctx, hipSetDevice(ctx->device));
+  XT_HIP(ctx, hipDeviceSynchronize());
+  return XT_OK;
+}
+
+int xt_count_csr(xt_ctx* ctx, int64_t nv, int64_t ne, const int64_t* offsets, const int64_t* tails,
+                 uint32_t flags, uint64_t* total, uint64_t* per_vertex) {
+  if (!ctx) return XT_ERR_ARG;
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  XT_HIP(ctx, hipSetDevice(ctx->device));
+  if (!total || (ne > 0 && !tails)) return set_err(ctx, XT_ERR_ARG, "null output or tails");
+  if (per_vertex && !(flags & XT_PER_VERTEX)) return set_err(ctx, XT_ERR_ARG, "per_vertex needs XT_PER_VERTEX");
+  XT_TRY(check_host_offsets(ctx, nv, ne, offsets));
+  hipStream_t st = ctx->stream;
+  XT_TRY(ensure(ctx, ctx->s.in_off, (size_t)(nv + 1) * sizeof(int64_t)));
+  XT_TRY(ensure(ctx, ctx->s.in_tails, (size_t)(ne > 0 ? ne : 1) * sizeof(int32_t)));
+  int64_t* d_off = ctx->s.in_off.as<int64_t>();
+  int32_t* d_tails = ctx->s.in_tails.as<int32_t>();
+  XT_TRY(upload_csr(ctx, nv, ne, offsets, tails, 8, d_off, d_tails, st));
+  if (flags & XT_VALIDATE) XT_TRY(validate<int32_t>(ctx, nv, ne, d_off, d_tails, (flags & XT_CHECK_SYMMETRY) != 0, st));
+  XT_TRY(count_device_csr<int32_t>(ctx, ctx->s, nv, ne, d_off, d_tails, flags, st, true));
+  return finish_count(ctx, nv, flags, st, true, total, per_vertex);
+}
+
+int xt_single_gpu_count(xt_ctx* ctx, int64_t nv, const int64_t* d_indices, const int64_t* d_edge_list,
+                        int64_t* d_local_count, void* stream) {
+  if (!ctx) return XT_ERR_ARG;
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  XT_HIP(ctx, hipSetDevice(ctx->device));
+  if (nv < 0 || !d_indices || !d_local_count) return set_err(ctx, XT_ERR_ARG, "bad arguments");
+  if (nv == 0) return XT_OK;
+  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+  int64_t ne = 0;
+  XT_HIP(ctx, hipMemcpyAsync(&ne, d_indices + nv, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  XT_HIP(ctx, hipStreamSynchronize(st));
+  if (ne < 0) return set_err(ctx, XT_ERR_INPUT, "negative edge count in d_indices[nv]");
+  if (ne > 0 && !d_edge_list) return set_err(ctx, XT_ERR_ARG, "null d_edge_list");
+  // Offsets must be a valid index (and rows sorted) before the chunked
+  // kernels search them; the reference launcher assumes the same.
+  XT_TRY(validate<int64_t>(ctx, nv, ne, d_indices, d_edge_list, false, st));
+  XT_TRY(count_device_csr<int64_t>(ctx, ctx->s, nv, ne, d_indices, d_edge_list, XT_ORIENT_DEGREE, st, false));
+  uint64_t bad = 0;
+  XT_HIP(ctx, hipMemcpyAsync(&bad, ctx->s.counters.as<uint64_t>() + kBadIds, sizeof(bad),
+                             hipMemcpyDeviceToHost, st));
+  XT_HIP(ctx, hipStreamSynchronize(st));
+  if (bad) return set_err(ctx, XT_ERR_INPUT, "neighbour id out of range [0, nv)");
+  return launch_add_total(ctx, d_local_count, st);
+}
+
+int xt_single_gpu_count_masked(xt_ctx* ctx, int64_t nedges, const int64_t* d_edge_mask,
+                               const int64_t* d_indices, const int64_t* d_edge_list,
+                               int64_t* d_local_count, void* stream) {
+  if (!ctx) return XT_ERR_ARG;
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  XT_HIP(ctx, hipSetDevice(ctx->device));
+  if (nedges < 0 || (nedges > 0 && (!d_edge_mask || !d_indices || !d_edge_list)) || !d_local_count)
+    return set_err(ctx, XT_ERR_ARG, "bad arguments");
+  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+  return launch_masked(ctx, nedges, d_edge_mask, d_indices, d_edge_list, d_local_count, st);
+}
+
+int xt_local_count_gpu_inner(xt_ctx* ctx, int64_t start, int64_t end, int64_t jstart, int64_t jend,
+                             const int64_t* d_edge_list, int64_t* d_local_count, void* stream) {
+  if (!ctx) return XT_ERR_ARG;
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  XT_HIP(ctx, hipSetDevice(ctx->device));
+  if (!d_edge_list || !d_local_count || start < 0 || jstart < 0) return set_err(ctx, XT_ERR_ARG, "bad arguments");
+  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+  return launch_inner(ctx, start, end, jstart, jend, d_edge_list, d_local_count, st);
+}
+
+// ---- resident graphs ----------------------------------------------------
+int xt_graph_from_csr(xt_ctx* ctx, int64_t nv, int64_t ne, const int64_t* offsets, const void* tails,
+                      int tails_bytes, uint32_t flags, xt_graph** out) {
+  if (!ctx || !out) return XT_ERR_ARG;
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  XT_HIP(ctx, hipSetDevice(ctx->device));
+  *out = nullptr;
+  if (tails_bytes != 4 && tails_bytes != 8) return set_err(ctx, XT_ERR_ARG, "tails_bytes must be 4 or 8");
+  if (ne > 0 && !tails) return set_err(ctx, XT_ERR_ARG, "null tails");
+  XT_TRY(check_host_offsets(ctx, nv, ne, offsets));
+  xt_graph* g = new xt_graph();
+  g->nv = nv;
+  g->ne = ne;
+  hipError_t e = hipMalloc(&g->off, (size_t)(nv + 1) * sizeof(int64_t));
+  if (e == hipSuccess) e = hipMalloc(&g->tails, (size_t)(ne > 0 ? ne : 1) * sizeof(int32_t));
+  if (e != hipSuccess) {
+    if (g->off) (void)hipFree(g->off);
+    delete g;
+    return hip_err(ctx, e, "hipMalloc(graph)");
+  }
+  int rc = upload_csr(ctx, nv, ne, offsets, tails, tails_bytes, g->off, g->tails, ctx->stream);
+  if (rc == XT_OK && (flags & XT_VALIDATE))
+    rc = validate<int32_t>(ctx, nv, ne, g->off, g->tails, (flags & XT_CHECK_SYMMETRY) != 0, ctx->stream);
+  if (rc != XT_OK) {
+    (void)hipFree(g->off);
+    (void)hipFree(g->tails);
+    delete g;
+    return rc;
+  }
+  *out = g;
+  return XT_OK;
+}
+
+int xt_graph_rmat(xt_ctx* ctx, int scale, int edgefactor, double a, double b, double c, uint64_t seed,
+                  int permute, xt_graph** out) {
+  if (!ctx || !out) return XT_ERR_ARG;
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  XT_HIP(ctx, hipSetDevice(ctx->device));
+  *out = nullptr;
+  if (!(a >= 0 && b >= 0 && c >= 0 && a + b + c <= 1.0)) return set_err(ctx, XT_ERR_ARG, "bad R-MAT probabilities");
+  const double s = 4294967296.0;
+  const double t[3] = {a * s, (a + b) * s, (a + b + c) * s};
+  uint32_t thr[3];
+  for (int i = 0; i < 3; i++) thr[i] = t[i] >= s ? 0xFFFFFFFFu : (uint32_t)t[i];
+  xt_graph* g = new xt_graph();
+  int rc = gen_rmat(ctx, scale, edgefactor, thr, seed, permute, g);
+  if (rc != XT_OK) {
+    if (g->off) (void)hipFree(g->off);
+    if (g->tails) (void)hipFree(g->tails);
+    delete g;
+    return rc;
+  }
+  *out = g;
+  return XT_OK;
+}
+
+int xt_graph_er(xt_ctx* ctx, int64_t n, uint64_t seed, uint64_t thr64, xt_graph** out) {
+  if (!ctx || !out) return XT_ERR_ARG;
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  XT_HIP(ctx, hipSetDevice(ctx->device));
+  *out = nullptr;
+  xt_graph* g = new xt_graph();
+  int rc = gen_er(ctx, n, seed, thr64, g);
+  if (rc != XT_OK) {
+    if (g->off) (void)hipFree(g->off);
+    if (g->tails) (void)hipFree(g->tails);
+    delete g;
+    return rc;
+  }
+  *out = g;
+  return XT_OK;
+}
+
+int xt_graph_info(const xt_graph* g, int64_t* nv, int64_t* ne) {
+  if (!g) return XT_ERR_ARG;
+  if (nv) *nv = g->nv;
+  if (ne) *ne = g->ne;
+  return XT_OK;
+}
+
+int xt_graph_download(xt_ctx* ctx, const xt_graph* g, int64_t* offsets, int32_t* tails) {
+  if (!ctx || !g) return XT_ERR_ARG;
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  XT_HIP(ctx, hipSetDevice(ctx->device));
+  if (offsets)
+    XT_HIP(ctx, hipMemcpy(offsets, g->off, (size_t)(g->nv + 1) * sizeof(int64_t), hipMemcpyDeviceToHost));
+  if (tails && g->ne > 0)
+    XT_HIP(ctx, hipMemcpy(tails, g->tails, (size_t)g->ne * sizeof(int32_t), hipMemcpyDeviceToHost));
+  return XT_OK;
+}
+
+int xt_graph_count(xt_ctx* ctx, xt_graph* g, uint32_t flags, uint64_t* total, uint64_t* per_vertex) {
+  if (!ctx || !g || !total) return XT_ERR_ARG;
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  XT_HIP(ctx, hipSetDevice(ctx->device));
+  if (per_vertex && !(flags & XT_PER_VERTEX)) return set_err(ctx, XT_ERR_ARG, "per_vertex needs XT_PER_VERTEX");
+  hipStream_t st = ctx->stream;
+  XT_TRY(count_device_csr<int32_t>(ctx, ctx->s, g->nv, g->ne, g->off, g->tails, flags, st, true));
+  return finish_count(ctx, g->nv, flags, st, true, total, per_vertex);
+}
+
+int xt_graph_alg_bytes(xt_ctx* ctx, xt_graph* g, uint32_t flags, uint64_t* bytes, uint64_t* sum_min,
+                       int64_t* ne_plus) {
+  if (!ctx || !g) return XT_ERR_ARG;
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  XT_HIP(ctx, hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  const int mode = (flags & XT_ORIENT_ID) ? kOrientId : kOrientDegree;
+  XT_TRY(ensure(ctx, ctx->s.counters, kNumSlots * sizeof(uint64_t)));
+  XT_TRY(fill_u64(ctx, ctx->s.counters.as<uint64_t>(), kNumSlots, 0, st));
+  XT_TRY(orient<int32_t>(ctx, ctx->s, g->nv, g->ne, g->off, g->tails, mode, st));
+  XT_TRY(alg_stats(ctx, ctx->s, g->nv, g->ne, st));
+  uint64_t sm = 0;
+  int64_t nep = 0;
+  XT_HIP(ctx, hipMemcpyAsync(&sm, ctx->s.counters.as<uint64_t>() + kStatMin, sizeof(sm),
+                             hipMemcpyDeviceToHost, st));
+  if (g->nv > 0)
+    XT_HIP(ctx, hipMemcpyAsync(&nep, ctx->s.ooff.as<int64_t>() + g->nv, sizeof(nep), hipMemcpyDeviceToHost, st));
+  XT_HIP(ctx, hipStreamSynchronize(st));
+  if (sum_min) *sum_min = sm;
+  if (ne_plus) *ne_plus = nep;
+  if (bytes) *bytes = 8ULL * (uint64_t)(g->nv + 1) + 4ULL * (uint64_t)nep + 4ULL * sm;
+  return XT_OK;
+}
+
+int xt_graph_destroy(xt_ctx* ctx, xt_graph* g) {
+  if (!g) return XT_ERR_ARG;
+  if (ctx) (void)hipStreamSynchronize(ctx->stream);
+  if (g->off) (void)hipFree(g->off);
+  if (g->tails) (void)hipFree(g->tails);
+  delete g;
+  return XT_OK;
+}
+
+}  // extern "C"

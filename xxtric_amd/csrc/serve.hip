@@ -1,0 +1,324 @@
+// serve.hip -- continuous batching of arriving graphs (host runtime, C++).
+//
+// Requests (host CSRs) are queued by xt_serve_submit.  A worker thread owns
+// `max_inflight` batch slots, each an xt_batch with its own HIP stream and
+// pre-sized HBM arena.  Whenever a slot is free it admits queued requests in
+// arrival order while their HBM footprint fits the budget and the slot's
+// arena, uploads them and launches the persistent batch pipeline on the slot's
+// stream; slots in flight run concurrently on the device.  Finished slots are
+// read back, their requests completed (latency = completion - arrival) and
+// their footprint released, which admits the next requests.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <deque>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "xt_internal.hpp"
+
+namespace xt {
+int batch_launch(xt_batch* b, int mode, uint32_t flags);
+int batch_poll(xt_batch* b);
+int batch_wait(xt_batch* b);
+int batch_read(xt_batch* b, uint64_t* totals, int64_t n);
+int batch_add_host(xt_batch* b, int64_t nv, int64_t ne, const int64_t* off, const int32_t* tails);
+}  // namespace xt
+
+using namespace xt;
+
+namespace {
+
+// HBM bytes one graph occupies in a slot arena: union + local CSR, vertex map,
+// orientation and scan scratch (per vertex) and oriented ids (per entry).
+constexpr int64_t kBytesPerVertex = 40;
+constexpr int64_t kBytesPerEntry = 12;
+
+enum ReqState : int { kQueued = 0, kRunning = 1, kDone = 2 };
+
+struct Req {
+  uint64_t ticket = 0;
+  int64_t nv = 0, ne = 0;
+  std::vector<int64_t> off;
+  std::vector<int32_t> tails;
+  double arrival_us = 0, done_us = 0;
+  uint64_t total = 0;
+  int64_t footprint = 0;
+  int state = kQueued;
+};
+
+struct ServeSlot {
+  xt_batch* b = nullptr;
+  std::vector<Req*> reqs;
+  int64_t V = 0, E = 0;
+  bool inflight = false;
+};
+
+}  // namespace
+
+struct xt_server {
+  xt_ctx* ctx = nullptr;
+  xt_serve_config cfg = {};
+  int64_t slot_v = 0, slot_e = 0, slot_g = 0;
+  std::vector<ServeSlot> slots;
+  std::mutex mu;
+  std::condition_variable cv, cv_done;
+  std::deque<Req*> queue;
+  std::unordered_map<uint64_t, Req*> live;
+  uint64_t next_ticket = 1;
+  int64_t resident = 0, peak_resident = 0;
+  std::vector<double> latencies;
+  double first_arrival = -1, last_done = 0;
+  int64_t submitted = 0, completed = 0, rejected = 0, batches = 0;
+  int worker_err = XT_OK;
+  bool stop = false;
+  std::thread worker;
+  std::chrono::steady_clock::time_point t0;
+
+  double now() const {
+    return std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
+  }
+};
+
+namespace {
+
+void complete_slot(xt_server* s, ServeSlot& sl) {
+  std::vector<uint64_t> tot(sl.reqs.size());
+  int rc = batch_read(sl.b, tot.data(), (int64_t)tot.size());
+  const double t = s->now();
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (rc != XT_OK) s->worker_err = rc;
+  for (size_t i = 0; i < sl.reqs.size(); i++) {
+    Req* r = sl.reqs[i];
+    r->total = tot[i];
+    r->done_us = t;
+    r->state = kDone;
+    s->resident -= r->footprint;
+    s->latencies.push_back(t - r->arrival_us);
+    s->completed++;
+    s->last_done = t;
+  }
+  sl.reqs.clear();
+  sl.V = sl.E = 0;
+  sl.inflight = false;
+  (void)xt_batch_clear(sl.b);
+  s->cv_done.notify_all();
+}
+
+void worker_main(xt_server* s) {
+  (void)hipSetDevice(s->ctx->device);
+  while (true) {
+    bool progressed = false;
+    for (ServeSlot& sl : s->slots) {
+      if (sl.inflight && batch_poll(sl.b) == 1) {
+        complete_slot(s, sl);
+        progressed = true;
+      }
+    }
+    // Fill one free slot from the queue (arrival order), then launch it.
+    ServeSlot* free_slot = nullptr;
+    for (ServeSlot& sl : s->slots)
+      if (!sl.inflight) {
+        free_slot = &sl;
+        break;
+      }
+    if (free_slot) {
+      std::vector<Req*> take;
+      {
+        std::unique_lock<std::mutex> lk(s->mu);
+        if (s->stop && s->queue.empty()) break;
+        if (!s->queue.empty() && s->cfg.max_wait_us > 0) {
+          // batching window: let a short queue fill unless a slot's worth is waiting
+          const double wait_until = s->queue.front()->arrival_us + s->cfg.max_wait_us;
+          if ((int64_t)s->queue.size() < s->slot_g && s->now() < wait_until && !s->stop) {
+            s->cv.wait_for(lk, std::chrono::microseconds(20));
+            continue;
+          }
+        }
+        int64_t V = free_slot->V, E = free_slot->E, G = (int64_t)free_slot->reqs.size();
+        while (!s->queue.empty()) {
+          Req* r = s->queue.front();
+          if (G + 1 > s->slot_g || V + r->nv > s->slot_v || E + r->ne > s->slot_e) break;
+          if (s->resident + r->footprint > s->cfg.hbm_budget_bytes) break;  // wait for memory
+          s->queue.pop_front();
+          s->resident += r->footprint;
+          s->peak_resident = std::max(s->peak_resident, s->resident);
+          r->state = kRunning;
+          take.push_back(r);
+          V += r->nv;
+          E += r->ne;
+          G++;
+        }
+      }
+      for (Req* r : take) {
+        int rc = batch_add_host(free_slot->b, r->nv, r->ne, r->off.data(), r->tails.data());
+        if (rc != XT_OK) {
+          std::lock_guard<std::mutex> lk(s->mu);
+          s->worker_err = rc;
+        }
+        free_slot->reqs.push_back(r);
+        free_slot->V += r->nv;
+        free_slot->E += r->ne;
+        std::vector<int64_t>().swap(r->off);  // host copy no longer needed
+        std::vector<int32_t>().swap(r->tails);
+      }
+      if (!free_slot->reqs.empty()) {
+        int rc = batch_launch(free_slot->b, XT_BATCH_PERSISTENT, XT_ORIENT_DEGREE);
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (rc != XT_OK) s->worker_err = rc;
+        free_slot->inflight = true;
+        s->batches++;
+        progressed = true;
+      }
+    }
+    if (!progressed) {
+      std::unique_lock<std::mutex> lk(s->mu);
+      bool any_inflight = false;
+      for (ServeSlot& sl : s->slots) any_inflight |= sl.inflight;
+      if (s->stop && s->queue.empty() && !any_inflight) break;
+      s->cv.wait_for(lk, std::chrono::microseconds(any_inflight ? 20 : 200));
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int xt_serve_create(xt_ctx* ctx, const xt_serve_config* cfg, xt_server** out) {
+  if (!ctx || !out) return XT_ERR_ARG;
+  *out = nullptr;
+  xt_server* s = new xt_server();
+  s->ctx = ctx;
+  if (cfg) s->cfg = *cfg;
+  if (s->cfg.hbm_budget_bytes <= 0) s->cfg.hbm_budget_bytes = 8LL << 30;
+  if (s->cfg.max_batch_graphs <= 0) s->cfg.max_batch_graphs = 4096;
+  if (s->cfg.max_inflight <= 0) s->cfg.max_inflight = 2;
+  if (s->cfg.max_queue <= 0) s->cfg.max_queue = 1 << 20;
+  if (s->cfg.max_wait_us < 0) s->cfg.max_wait_us = 0;
+  // Each slot's arena gets an equal share of the budget.
+  const int64_t share = s->cfg.hbm_budget_bytes / s->cfg.max_inflight;
+  s->slot_e = std::max<int64_t>(1024, share / (2 * kBytesPerEntry + kBytesPerVertex / 2));
+  s->slot_v = std::max<int64_t>(256, s->slot_e / 4);
+  s->slot_g = s->cfg.max_batch_graphs;
+  s->slots.resize(s->cfg.max_inflight);
+  for (ServeSlot& sl : s->slots) {
+    int rc = xt_batch_create(ctx, s->slot_v, s->slot_e, s->slot_g, &sl.b);
+    if (rc != XT_OK) {
+      for (ServeSlot& o : s->slots)
+        if (o.b) xt_batch_destroy(o.b);
+      delete s;
+      return rc;
+    }
+  }
+  s->t0 = std::chrono::steady_clock::now();
+  s->worker = std::thread(worker_main, s);
+  *out = s;
+  return XT_OK;
+}
+
+int xt_serve_submit(xt_server* s, int64_t nv, int64_t ne, const int64_t* offsets, const int32_t* tails,
+                    double arrival_us, uint64_t* ticket) {
+  if (!s || !ticket || nv < 1 || ne < 0 || !offsets || (ne > 0 && !tails)) return XT_ERR_ARG;
+  const int64_t fp = kBytesPerVertex * nv + kBytesPerEntry * ne;
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->submitted++;
+  if (nv > s->slot_v || ne > s->slot_e || fp > s->cfg.hbm_budget_bytes ||
+      (int64_t)s->queue.size() >= s->cfg.max_queue) {
+    s->rejected++;
+    return set_err(s->ctx, XT_ERR_CAPACITY, "request rejected: larger than a batch slot, the HBM budget or the queue");
+  }
+  Req* r = new Req();
+  r->ticket = s->next_ticket++;
+  r->nv = nv;
+  r->ne = ne;
+  r->off.assign(offsets, offsets + nv + 1);
+  r->tails.assign(tails, tails + ne);
+  r->arrival_us = arrival_us >= 0 ? arrival_us : s->now();
+  r->footprint = fp;
+  if (s->first_arrival < 0 || r->arrival_us < s->first_arrival) s->first_arrival = r->arrival_us;
+  s->live[r->ticket] = r;
+  s->queue.push_back(r);
+  *ticket = r->ticket;
+  s->cv.notify_all();
+  return XT_OK;
+}
+
+int xt_serve_poll(xt_server* s, uint64_t ticket, uint64_t* total, double* latency_us) {
+  if (!s) return XT_ERR_ARG;
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (s->worker_err != XT_OK) return s->worker_err;
+  auto it = s->live.find(ticket);
+  if (it == s->live.end()) return XT_ERR_NOTFOUND;
+  Req* r = it->second;
+  if (r->state != kDone) return XT_PENDING;
+  if (total) *total = r->total;
+  if (latency_us) *latency_us = r->done_us - r->arrival_us;
+  s->live.erase(it);
+  delete r;
+  return XT_OK;
+}
+
+int xt_serve_drain(xt_server* s, double timeout_s) {
+  if (!s) return XT_ERR_ARG;
+  std::unique_lock<std::mutex> lk(s->mu);
+  const bool ok = s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), [&] {
+    return s->completed + s->rejected >= s->submitted || s->worker_err != XT_OK;
+  });
+  if (s->worker_err != XT_OK) return s->worker_err;
+  return ok ? XT_OK : XT_ERR_STATE;
+}
+
+int xt_serve_stats_get(xt_server* s, xt_serve_stats* out) {
+  if (!s || !out) return XT_ERR_ARG;
+  std::lock_guard<std::mutex> lk(s->mu);
+  xt_serve_stats st = {};
+  st.submitted = s->submitted;
+  st.completed = s->completed;
+  st.rejected = s->rejected;
+  st.batches = s->batches;
+  st.peak_resident_bytes = s->peak_resident;
+  if (!s->latencies.empty()) {
+    std::vector<double> v = s->latencies;
+    std::sort(v.begin(), v.end());
+    auto q = [&](double p) {
+      size_t k = (size_t)std::ceil(p * (double)v.size()) - 1;
+      return v[std::min(k, v.size() - 1)];
+    };
+    st.p50_us = q(0.50);
+    st.p99_us = q(0.99);
+    double sum = 0;
+    for (double x : v) sum += x;
+    st.mean_us = sum / (double)v.size();
+    st.max_us = v.back();
+    const double span = s->last_done - s->first_arrival;
+    st.graphs_per_s = span > 0 ? (double)s->completed / (span * 1e-6) : 0.0;
+  }
+  *out = st;
+  return XT_OK;
+}
+
+double xt_serve_now_us(const xt_server* s) { return s ? s->now() : 0.0; }
+
+int xt_serve_destroy(xt_server* s) {
+  if (!s) return XT_ERR_ARG;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+    s->cv.notify_all();
+  }
+  if (s->worker.joinable()) s->worker.join();
+  for (ServeSlot& sl : s->slots) {
+    if (sl.b) {
+      (void)batch_wait(sl.b);
+      xt_batch_destroy(sl.b);
+    }
+  }
+  for (auto& kv : s->live) delete kv.second;
+  delete s;
+  return XT_OK;
+}
+
+}  // extern "C"

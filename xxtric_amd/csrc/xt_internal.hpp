@@ -1,0 +1,158 @@
+// xt_internal.hpp -- shared internals of libxtric (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <string>
+
+#include "../../include/xtric.h"
+
+namespace xt {
+
+// Launch geometry shared by the chunked kernels: one 256-thread workgroup
+// (four wave64s) owns a contiguous chunk of kChunk CSR entries, eight per
+// thread.  kOffCap bounds the vertex span staged in LDS per chunk.
+constexpr int kBlock = 256;
+constexpr int kPerThread = 8;
+constexpr int kChunk = kBlock * kPerThread;  // 2048
+constexpr int kOffCap = 2048;
+
+// Counter slots in Scratch::counters (uint64).
+enum Slot : int {
+  kTotal = 0,      // global triangle count
+  kDeferred = 1,   // edges appended to the wave-kernel list
+  kBlockEdges = 2, // edges appended to the block-kernel list
+  kBadIds = 3,     // neighbour ids outside [0, nv)
+  kBadRows = 4,    // rows not strictly ascending
+  kAsym = 5,       // entries without their reverse entry
+  kBadOff = 6,     // offsets not monotone / wrong ends
+  kStatMin = 7,    // sum of min(d+(u), d+(v)) (statistics kernel)
+  kTileQueue = 8,  // next tile of the persistent batch intersection kernel
+  kNumSlots = 16
+};
+
+// Orientation rules.
+enum Orient : int { kOrientDegree = 0, kOrientId = 1 };
+
+// A grow-only device buffer owned by a context.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  template <typename T>
+  T* as() const { return static_cast<T*>(p); }
+};
+
+// Phase events recorded on the context stream.
+enum Ev : int { kEvStart = 0, kEvOriented, kEvCounted, kEvEnd, kNumEv };
+
+struct Scratch {
+  DevBuf deg;       // uint32[nv]          degree of each vertex
+  DevBuf dplus;     // uint32[nv]          out-degree of each vertex
+  DevBuf tailcnt;   // uint32[nchunks]     flagged entries of each chunk's last vertex
+  DevBuf qscan;     // int64[nchunks+1]    exclusive scan of tailcnt
+  DevBuf ooff;      // int64[nv+1]         oriented offsets
+  DevBuf otails;    // int32[ne/2]         oriented out-lists, ascending ids
+  DevBuf counters;  // uint64[16]          total, deferred count, block count, flags
+  DevBuf defer;     // int64[..]           oriented edges deferred to the wave kernel
+  DevBuf pv;        // uint64[nv]          per-vertex counts
+  DevBuf scan_tmp;  // int64[..]           block sums of the scans
+  DevBuf in_off;    // int64[nv+1]         staged input offsets (xt_count_csr)
+  DevBuf in_tails;  // int32[ne]           staged input ids (xt_count_csr)
+};
+
+}  // namespace xt
+
+struct xt_ctx {
+  int device = 0;
+  int num_cus = 256;
+  hipStream_t stream = nullptr;
+  std::string err;
+  std::recursive_mutex mu;
+  xt::Scratch s;
+  hipEvent_t ev[xt::kNumEv] = {};
+  xt_timing timing = {};
+  int small_cut = 64;     // |shorter list| above which an edge goes to the wave kernel
+  int block_cut = 4096;   // |shorter list| above which an edge goes to the block kernel
+  int persist_blocks = 8; // persistent workgroups per CU (batch tile queue)
+};
+
+struct xt_graph {
+  int64_t nv = 0;
+  int64_t ne = 0;
+  int64_t* off = nullptr;   // int64[nv+1]
+  int32_t* tails = nullptr; // int32[ne]
+};
+
+namespace xt {
+
+int set_err(xt_ctx* ctx, int code, const std::string& msg);
+int hip_err(xt_ctx* ctx, hipError_t e, const char* what);
+// Ensure a scratch buffer holds at least `bytes`; contents are not preserved.
+int ensure(xt_ctx* ctx, DevBuf& b, size_t bytes);
+
+// ---- pipeline pieces (kernels live in the .hip files) ----
+// Exclusive scan of n uint32 -> int64 out[0..n] (out[n] = total).
+int scan_u32_to_i64(xt_ctx* ctx, Scratch& s, const uint32_t* in, int64_t n, int64_t* out,
+                    hipStream_t st);
+
+// Orientation of a symmetric CSR into the context's ooff/otails.
+// Upper bound of |E+| is ne/2 (one direction per undirected pair).
+template <typename T>
+int orient(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne, const int64_t* off, const T* tails, int mode,
+           hipStream_t st);
+
+// Intersection + reduction over the context's oriented CSR.  Adds the global
+// count to ctx counters[0]; per-vertex counts into pv when pv != nullptr.
+// Batch mode: gid (graph of each vertex) and gtotal (per-graph totals) non-null
+// and the intersection kernel drains a device tile queue with a fixed grid.
+int count_oriented(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne_bound, uint64_t* pv,
+                   const int32_t* gid, uint64_t* gtotal, hipStream_t st);
+int alg_stats(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne_bound, hipStream_t st);
+// Pre-sizes every scratch buffer for graphs up to (nv, ne).
+int reserve(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne, bool per_vertex);
+void release(Scratch& s);
+
+size_t scan_tmp_bytes(int64_t n);
+
+// Fill kernels (used instead of hipMemsetAsync so the sequence can be captured).
+int fill_u32(xt_ctx* ctx, uint32_t* p, int64_t n, uint32_t v, hipStream_t st);
+int fill_u64(xt_ctx* ctx, uint64_t* p, int64_t n, uint64_t v, hipStream_t st);
+
+// Validation of a device CSR; returns XT_ERR_INPUT with a message on failure.
+template <typename T>
+int validate(xt_ctx* ctx, int64_t nv, int64_t ne, const int64_t* off, const T* tails,
+             bool symmetry, hipStream_t st);
+
+// Whole single-graph count on the device (orientation + intersection).
+template <typename T>
+int count_device_csr(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne, const int64_t* off, const T* tails,
+                     uint32_t flags, hipStream_t st, bool timed);
+
+// Device generators.
+int gen_rmat(xt_ctx* ctx, int scale, int edgefactor, const uint32_t thr[3], uint64_t seed,
+             int permute, xt_graph* g);
+int gen_er(xt_ctx* ctx, int64_t n, uint64_t seed, uint64_t thr64, xt_graph* g);
+
+// Host mixing function shared by the generators (SplitMix64 output function).
+inline uint64_t mix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+}  // namespace xt
+
+#define XT_HIP(ctx, expr)                                   \
+  do {                                                      \
+    hipError_t _e = (expr);                                 \
+    if (_e != hipSuccess) return ::xt::hip_err(ctx, _e, #expr); \
+  } while (0)
+
+#define XT_TRY(expr)          \
+  do {                        \
+    int _rc = (expr);         \
+    if (_rc != XT_OK) return _rc; \
+  } while (0)
