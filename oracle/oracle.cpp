@@ -177,6 +177,66 @@ extern "C" uint64_t or_count_reference32(int64_t nv, const int64_t* off, const i
   return count_reference<int32_t>(nv, off, tails, per_vertex, nthreads, vstart, vstride);
 }
 
+// Time-bounded sample of the reference count for CPU baselines: vertices are
+// taken in id order (a random order for relabelled R-MAT graphs) by nthreads
+// threads until `seconds` elapse; a vertex still running at the deadline is
+// abandoned and not counted.  Returns the number of vertices completed and
+// their triangles; *edges_done receives the sum of their degrees.
+extern "C" int64_t or_count_reference32_budget(int64_t nv, const int64_t* off, const int32_t* tails,
+                                               int nthreads, double seconds, uint64_t* triangles,
+                                               int64_t* edges_done) {
+  std::vector<int64_t> erange(2 * nv, 0);
+  for (int64_t i = 0; i < nv; i++) {
+    const int64_t e0 = off[i], e1 = off[i + 1];
+    if (e0 + 1 == e1 || e0 == e1) continue;
+    erange[2 * i] = tails[e0];
+    erange[2 * i + 1] = tails[e1 - 1];
+  }
+  if (nthreads < 1) nthreads = 1;
+  const double t0 = omp_get_wtime();
+  int64_t next = 0, done = 0, edges = 0;
+  uint64_t tri = 0;
+#pragma omp parallel num_threads(nthreads) reduction(+ : done, edges, tri)
+  {
+    bool stop = false;
+    while (!stop) {
+      int64_t i;
+#pragma omp atomic capture
+      i = next++;
+      if (i >= nv) break;
+      const int64_t e0 = off[i], e1 = off[i + 1];
+      uint64_t t = 0;
+      int64_t steps = 0;
+      bool aborted = false;
+      if (e0 + 1 != e1) {
+        for (int64_t m = e0; m < e1 - 1 && !aborted; m++) {
+          const int64_t v = tails[m];
+          if (i >= v) continue;
+          const int64_t vmin = erange[2 * v], vmax = erange[2 * v + 1];
+          for (int64_t n = m + 1; n < e1; n++) {
+            const int64_t w = tails[n];
+            if (!(w <= vmax)) break;
+            if (!(w >= vmin)) continue;
+            if (check_edgelist(off, tails, v, w)) t += 1;
+            if ((++steps & 4095) == 0 && omp_get_wtime() - t0 > seconds) {
+              aborted = true;
+              break;
+            }
+          }
+        }
+      }
+      if (aborted) break;
+      tri += t;
+      done += 1;
+      edges += e1 - e0;
+      if ((done & 255) == 0 && omp_get_wtime() - t0 > seconds) stop = true;
+    }
+  }
+  *triangles = tri;
+  *edges_done = edges;
+  return done;
+}
+
 // ---------------------------------------------------------------------------
 // Independent exact counter (not a reference restatement): orientation by
 // (degree, id), sorted-merge intersection of out-lists, credit min(u, v, w).

@@ -56,6 +56,12 @@ uint64_t or_count_reference32(int64_t nv, const int64_t* off, const int32_t* tai
                               uint64_t* per_vertex, int nthreads, int64_t vstart,
                               int64_t vstride);
 
+/* Time-bounded sample of or_count_reference32 for CPU baselines (vertices in
+ * id order, abandoned at the deadline); returns vertices completed. */
+int64_t or_count_reference32_budget(int64_t nv, const int64_t* off, const int32_t* tails,
+                                    int nthreads, double seconds, uint64_t* triangles,
+                                    int64_t* edges_done);
+
 /* Independent exact counter: (degree, id) orientation + sorted merge,
  * per-vertex credit to the smallest id of each triangle. */
 uint64_t or_count_degree_ordered(int64_t nv, const int64_t* off, const int32_t* tails,

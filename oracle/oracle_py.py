@@ -38,6 +38,7 @@ def lib():
             "or_count_reference": (_u64, [_i64, _p, _p, _p, _int, _i64, _i64]),
             "or_count_reference32": (_u64, [_i64, _p, _p, _p, _int, _i64, _i64]),
             "or_count_degree_ordered": (_u64, [_i64, _p, _p, _p, _int]),
+            "or_count_reference32_budget": (_i64, [_i64, _p, _p, _int, _dbl, _p, _p]),
             "or_vite_write": (_int, [ctypes.c_char_p, _i64, _p, _p, _int]),
             "or_vite_read": (_int, [ctypes.c_char_p, _int, _p, _p, _p, _p]),
             "or_mix64": (_u64, [_u64]),
@@ -101,6 +102,20 @@ def count_reference(off, tails, per_vertex=False, nthreads=None, vstart=0, vstri
         tails = np.ascontiguousarray(tails, np.int64)
         tot = lib().or_count_reference(nv, _ptr(off), _ptr(tails), _ptr(pv), nt, vstart, vstride)
     return (int(tot), pv) if per_vertex else int(tot)
+
+
+def count_reference_budget(off, tails, seconds, nthreads=None):
+    """Reference count on vertices 0,1,2,... until `seconds` elapse.
+
+    Returns (vertices_done, triangles_of_those, degree_sum_of_those)."""
+    off = np.ascontiguousarray(off, np.int64)
+    tails = np.ascontiguousarray(tails, np.int32)
+    tri = ctypes.c_uint64()
+    edges = ctypes.c_int64()
+    nt = nthreads or nthreads_default()
+    done = lib().or_count_reference32_budget(len(off) - 1, _ptr(off), _ptr(tails), nt, seconds,
+                                            ctypes.addressof(tri), ctypes.addressof(edges))
+    return int(done), int(tri.value), int(edges.value)
 
 
 def count_degree_ordered(off, tails, per_vertex=False, nthreads=None):

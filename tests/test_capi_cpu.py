@@ -97,6 +97,47 @@ def test_vite_reader_rebases_offsets(tmp_path):
     assert list(o) == [0, 2, 3, 4] and list(t) == [1, 2, 0, 0]
 
 
+TRIC = os.path.join(capi.PKG, "bin", "tric")
+
+
+@pytest.mark.parametrize("layout32", [False, True])
+def test_tric_generates_reference_graph(golden, tmp_path, layout32):
+    g = [x for x in golden["rgg_lcg"] if x["n"] == 4096][0]
+    p = str(tmp_path / "g.bin")
+    args = [TRIC, "-n", "4096", "-l", "-x", "-w", p] + (["-L"] if layout32 else [])
+    out = subprocess.run(args, capture_output=True, text=True, check=True).stdout
+    assert "Number of vertices: 4096" in out and "Number of edges: %d" % g["ne"] in out
+    off, tails = O.vite_read(p, layout32=layout32)
+    assert csr_sha256(off, tails) == g["csr_sha256"]
+
+
+def test_tric_rejects_bad_flags():
+    r = subprocess.run([TRIC], capture_output=True, text=True)
+    assert r.returncode != 0 and "Must specify some options." in r.stderr
+    r = subprocess.run([TRIC, "-l"], capture_output=True, text=True)
+    assert r.returncode != 0
+    r = subprocess.run([TRIC, "-n", "100", "-p", "120"], capture_output=True, text=True)
+    assert r.returncode != 0 and "Invalid random edge percentage" in r.stderr
+
+
+def test_tric_random_extra_edges_are_seeded(tmp_path):
+    outs = []
+    for k in range(2):
+        p = str(tmp_path / ("p%d.bin" % k))
+        subprocess.run([TRIC, "-n", "2000", "-l", "-p", "5", "-x", "-w", p], capture_output=True,
+                       check=True)
+        outs.append(O.vite_read(p))
+    (o1, t1), (o2, t2) = outs
+    assert (o1 == o2).all() and (t1 == t2).all()
+    base, bt = O.rgg_lcg(2000)
+    assert len(t1) > len(bt)                       # edges were added
+    n = len(o1) - 1                                # still simple and symmetric
+    rows = np.repeat(np.arange(n), np.diff(o1))
+    assert (rows != t1).all()
+    fw = set(zip(rows.tolist(), t1.tolist()))
+    assert all((b, a) in fw for a, b in fw)
+
+
 def test_edgelist_converter(tmp_path):
     p = tmp_path / "e.tsv"
     p.write_text("# comment\n1\t2\t1\n2\t3\t1\n3\t1\t1\n3\t4\t1\n2\t1\t1\n")

@@ -1,0 +1,122 @@
+"""The pins that check the oracle must catch plausible mistakes.
+
+A small pure-Python restatement of the reference loop (xxtric.hpp:60-208) and
+of the RGG construction (graph.hpp:789-948, utils.hpp:245-430) takes mutation
+switches.  Unmutated, it agrees with the C oracle; every mutation must be
+caught by at least one pin the oracle is checked against (golden counts of the
+reference binary, brute force, closed forms).  All of this runs on the CPU.
+"""
+from math import comb, log, sqrt
+
+import numpy as np
+import pytest
+
+import oracle_py as O
+from helpers import brute_force, complete, csr_from_edges, random_graph
+
+
+def ref_loop(off, tails, mut=None):
+    n = len(off) - 1
+    er = [(0, 0)] * n
+    for i in range(n):
+        e0, e1 = off[i], off[i + 1]
+        if e1 - e0 >= 2:
+            er[i] = (tails[e0], tails[e1 - 1])
+    pv = [0] * n
+    for i in range(n):
+        e0, e1 = off[i], off[i + 1]
+        if e0 + 1 == e1:
+            continue
+        mend = e1 - 2 if mut == "skip_last_m" else e1 - 1
+        for m in range(e0, mend):
+            v = tails[m]
+            if i >= v:
+                continue
+            lo, hi = (er[v][1], er[v][0]) if mut == "swap_prune" else er[v]
+            nstart = m if mut == "n_from_m" else m + 1
+            row = set(tails[off[v]:off[v + 1]])
+            for k in range(nstart, e1):
+                w = tails[k]
+                if not w <= hi:
+                    break
+                if not w >= lo:
+                    continue
+                if w in row:
+                    who = max(i, v, w) if mut == "credit_max" else i
+                    pv[who] += 1
+    return sum(pv), np.array(pv, np.uint64)
+
+
+def pins_catch(off, tails, mut):
+    tot, pv = ref_loop(off, tails, mut)
+    bt, bpv = brute_force(off, tails)
+    return tot != bt or not (pv == bpv).all()
+
+
+GRAPHS = [complete(12),
+          random_graph(40, 0.3, 1),
+          csr_from_edges(8, [(0, 0), (0, 1), (1, 1), (1, 2), (0, 2), (2, 3), (3, 4), (2, 4), (4, 4)])]
+
+
+def test_unmutated_matches_c_oracle():
+    for off, tails in GRAPHS:
+        tot, pv = ref_loop(off, tails)
+        ct, cpv = O.count_reference(off, tails, per_vertex=True)
+        assert tot == ct and (pv == cpv).all()
+
+
+@pytest.mark.parametrize("mut", ["credit_max", "swap_prune", "n_from_m", "skip_last_m"])
+def test_each_count_mutation_is_caught(mut):
+    assert any(pins_catch(off, tails, mut) for off, tails in GRAPHS), mut
+
+
+def test_credit_max_caught_by_complete_graph_closed_form():
+    off, tails = complete(10)
+    _, pv = ref_loop(off, tails, "credit_max")
+    assert [int(x) for x in pv] != [comb(10 - 1 - i, 2) for i in range(10)]
+
+
+def rgg_edges(n, mut=None):
+    r = [O.reseeder(1)]
+    for _ in range(2 * n - 1):
+        r.append((r[-1] * 16807) % 2147483647)
+    d = np.abs(np.array(r, np.float64)) * (1.0 / 2147483647.0)
+    X = d[:n]
+    Y = d[:n] if mut == "y_from_x_stream" else d[n:]
+    rc = sqrt(log(n) / (3.14159 * n))
+    rt = 0.0 if mut == "drop_rt" else sqrt(2.0736 / n)
+    rn = (rc + rt) / 2.0
+    dx = X[:, None] - X[None, :]
+    dy = Y[:, None] - Y[None, :]
+    ed = np.sqrt(dx * dx + dy * dy)
+    iu = np.triu_indices(n, 1)
+    return int((ed[iu] <= rn).sum()) * 2
+
+
+@pytest.mark.parametrize("mut", [None, "drop_rt", "y_from_x_stream"])
+def test_rgg_mutations_caught_by_reference_golden(golden, mut):
+    g = [x for x in golden["rgg_lcg"] if x["n"] == 1024][0]
+    ne = rgg_edges(1024, mut)
+    if mut is None:
+        assert ne == g["ne"]
+    else:
+        assert ne != g["ne"]
+
+
+def test_degree_ordered_missing_apex_credit_is_caught():
+    # crediting every triangle to min(u, v) instead of min(u, v, w) is a slip the
+    # brute-force per-vertex pin must reject
+    off, tails = random_graph(30, 0.4, 3)
+    bt, bpv = brute_force(off, tails)
+    tot, pv = O.count_degree_ordered(off, tails, per_vertex=True)
+    assert tot == bt and (pv == bpv).all()
+    n = len(off) - 1
+    deg = np.diff(off)
+    key = lambda x: (deg[x], x)  # noqa: E731
+    wrong = np.zeros(n, np.uint64)
+    for u in range(n):
+        Nu = {x for x in tails[off[u]:off[u + 1]] if key(x) > key(u)}
+        for v in Nu:
+            Nv = {x for x in tails[off[v]:off[v + 1]] if key(x) > key(v)}
+            wrong[min(u, v)] += len(Nu & Nv)
+    assert int(wrong.sum()) == bt and not (wrong == bpv).all()
