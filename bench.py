@@ -49,25 +49,31 @@ def rmat_workload(ctx, scale, steps, warmup, seed=1):
     wall = time.perf_counter() - t0
     ms_step = wall * 1e3 / steps
     med = lambda k: float(np.median([e[k] for e in ev]))  # noqa: E731
-    alg_bytes, sum_min, nep = g.alg_bytes()
-    return g, {"triangles": tri, "nv": g.nv, "m": m, "ms_per_step": ms_step,
-               "orient_ms": med("orient_ms"), "count_ms": med("count_ms"),
-               "device_ms": med("total_ms"), "ne_plus": nep, "sum_min": sum_min,
-               "alg_bytes": alg_bytes, "n_deferred": ev[-1]["n_deferred"], "gen_s": gen_s}
+    st = g.stats()
+    out = {"triangles": tri, "nv": g.nv, "m": m, "ms_per_step": ms_step, "gen_s": gen_s,
+           "n_heavy": ev[-1]["n_heavy"], "n_tiles": ev[-1]["n_tiles"]}
+    for k in ("orient_ms", "count_ms", "total_ms", "light_ms", "items_ms", "group_ms"):
+        out[k] = med(k)
+    out.update(st)
+    return g, out
 
 
 def roofline(r):
-    # Dominant phase: the intersection kernels (k_count_small + k_count_wave).
-    # Algorithmic bytes of one count: 8(n+1) + 4|E+| + 4*sum min(d+(u), d+(v)).
-    achieved = r["alg_bytes"] / (r["count_ms"] * 1e-3) / 1e9
+    # Dominant kernel: k_count_groups (heavy edges).  Its algorithmic bytes per
+    # launch: 4 B per id of each heavy edge's shorter out-list plus 4 B per item
+    # id, i.e. 4 * (sum over heavy edges of min(d+(u), d+(v)) + heavy edges).
+    achieved = r["group_bytes"] / (r["group_ms"] * 1e-3) / 1e9
+    whole = r["alg_bytes"] / (r["total_ms"] * 1e-3) / 1e9
     traffic = None
     if os.path.exists(PROFILE_PMC):
         with open(PROFILE_PMC) as f:
-            traffic = json.load(f).get("hbm_bytes_per_count")
+            traffic = json.load(f).get("group_hbm_bytes")
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "kernel": "k_count_small+k_count_wave", "alg_bytes": r["alg_bytes"],
-            "kernel_ms": r["count_ms"]}
+            "kernel": "k_count_groups", "kernel_ms": round(r["group_ms"], 3),
+            "alg_bytes_per_launch": r["group_bytes"],
+            "whole_count": {"alg_bytes": r["alg_bytes"], "ms": round(r["total_ms"], 3),
+                            "achieved": round(whole, 1), "frac": round(whole / HBM_PEAK_GBS, 4)}}
 
 
 def cpu_baseline(g, seconds):
@@ -158,6 +164,25 @@ def serve_workload(ctx, rate, n_req=4000, pool=400, seed=7):
             "batches": st["batches"], "counts_equal_solo": bool(ok)}
 
 
+def copy_calibration(ctx, nbytes=1 << 30, reps=3):
+    """Plain copy kernel of known size: the reference point that scales the
+    FETCH_SIZE / WRITE_SIZE counters of a rocprofv3 --pmc run to HBM bytes."""
+    import xxtric_amd
+    from xxtric_amd import capi
+    src = xxtric_amd.DeviceBuffer(ctx, nbytes)
+    dst = xxtric_amd.DeviceBuffer(ctx, nbytes)
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        ctx._check(ctx._L.xt_memcpy(ctx.h, dst.ptr, src.ptr, nbytes, capi.XT_D2D_KERNEL))
+        ts.append(time.perf_counter() - t0)
+    src.close()
+    dst.close()
+    print(json.dumps({"copy_bytes": nbytes, "reps": reps, "kernel": "k_copy16",
+                      "read_KiB_expected": nbytes // 1024, "write_KiB_expected": nbytes // 1024,
+                      "best_s": min(ts), "GBps_read_plus_write": 2 * nbytes / min(ts) / 1e9}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -167,6 +192,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--serve-rate", type=float, default=20000.0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run the 1 GiB copy kernel used to scale PMC byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the single-device path only "
@@ -174,6 +201,9 @@ def main():
         sys.exit(2)
     import xxtric_amd
     ctx = xxtric_amd.Context(0)
+    if args.copy_calibration:
+        copy_calibration(ctx)
+        return
     g, r = rmat_workload(ctx, args.scale, args.steps, args.warmup)
     value = r["m"] / (r["ms_per_step"] * 1e-3)
     line = {
@@ -185,11 +215,11 @@ def main():
                    "edgefactor": 16, "seed": 1, "nv": r["nv"], "m_undirected": r["m"],
                    "orientation": "degree", "triangles": r["triangles"]},
         "roofline": roofline(r),
-        "phases_ms": {"orient": round(r["orient_ms"], 3), "count": round(r["count_ms"], 3),
-                      "device_total": round(r["device_ms"], 3)},
+        "phases_ms": {k: round(r[k + "_ms"], 3) for k in ("orient", "light", "items", "group", "count", "total")},
         "teps_reference_style": round(2 * r["m"] / (r["ms_per_step"] * 1e-3), 1),
-        "edges_per_s_device": round(r["m"] / (r["device_ms"] * 1e-3), 1),
-        "ne_plus": r["ne_plus"], "sum_min": r["sum_min"], "n_deferred": r["n_deferred"],
+        "edges_per_s_device": round(r["m"] / (r["total_ms"] * 1e-3), 1),
+        "ne_plus": r["ne_plus"], "sum_min": r["sum_min"], "heavy_edges": r["heavy_edges"],
+        "sum_min_heavy": r["sum_min_heavy"], "group_tiles": r["n_tiles"],
     }
     line["cpu_baseline"] = cpu_baseline(g, args.cpu_seconds)
     g.close()

@@ -48,11 +48,14 @@ typedef struct xt_graph xt_graph;
  * the stream the kernels ran on (milliseconds). */
 typedef struct xt_timing {
   double orient_ms;   /* degree keys + out-degree count + scans + oriented fill */
-  double count_ms;    /* intersection + reduction kernels */
+  double count_ms;    /* intersection + reduction kernels (the three phases below) */
   double total_ms;    /* first to last event of the count */
+  double light_ms;    /* light edges counted one per thread, heavy edges classified */
+  double items_ms;    /* item lists of the heavy-edge owners and their tiles */
+  double group_ms;    /* persistent group kernel: staged list x streamed lists */
   int64_t ne_plus;    /* oriented edges |E+| */
-  int64_t n_deferred; /* edges handed to the wave-per-edge kernel */
-  int64_t n_block;    /* edges handed to the block-per-edge kernel */
+  int64_t n_heavy;    /* oriented edges intersected by the group kernel */
+  int64_t n_tiles;    /* group tiles drawn from the device work queue */
 } xt_timing;
 
 const char* xt_version(void);
@@ -74,6 +77,7 @@ int xt_ctx_tune(xt_ctx* ctx, int small_cut, int block_cut, int reserved);
 #define XT_H2D 1
 #define XT_D2H 2
 #define XT_D2D 3
+#define XT_D2D_KERNEL 4 /* device copy by a plain 16-byte-per-lane copy kernel */
 int xt_dev_alloc(xt_ctx* ctx, size_t bytes, void** out);
 int xt_dev_free(xt_ctx* ctx, void* p);
 int xt_memcpy(xt_ctx* ctx, void* dst, const void* src, size_t bytes, int kind);
@@ -135,6 +139,17 @@ int xt_graph_count(xt_ctx* ctx, xt_graph* g, uint32_t flags, uint64_t* total,
  * 8(n+1) + 4|E+| + 4 * sum over oriented (u,v) of min(d+(u), d+(v)). */
 int xt_graph_alg_bytes(xt_ctx* ctx, xt_graph* g, uint32_t flags, uint64_t* bytes,
                        uint64_t* sum_min, int64_t* ne_plus);
+/* The same figures split by kernel: heavy edges are those the group kernel
+ * intersects (an out-list longer than the light cut). */
+typedef struct xt_graph_stats {
+  uint64_t alg_bytes;      /* 8(n+1) + 4|E+| + 4 * sum_min */
+  uint64_t sum_min;        /* sum over oriented edges of min(d+(u), d+(v)) */
+  int64_t ne_plus;         /* |E+| */
+  uint64_t sum_min_heavy;  /* sum of min(d+(u), d+(v)) over heavy edges */
+  int64_t heavy_edges;     /* number of heavy edges */
+  uint64_t group_bytes;    /* 4 * (sum_min_heavy + heavy_edges): streamed ids + item ids */
+} xt_graph_stats;
+int xt_graph_stats_get(xt_ctx* ctx, xt_graph* g, uint32_t flags, xt_graph_stats* out);
 int xt_graph_destroy(xt_ctx* ctx, xt_graph* g);
 
 /* ---- host ingest (C++ in the library; no device work) -------------------- */

@@ -78,7 +78,7 @@ def test_permuted_order_and_split(ctx):
             assert (got == exp[perm]).all()
             b.close()
         finally:
-            ctx.tune(small_cut=64)
+            ctx.tune(small_cut=xxtric_amd.DEFAULT_LIGHT_CUT)
 
 
 def test_solo_equals_in_batch(ctx):

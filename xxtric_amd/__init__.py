@@ -16,6 +16,8 @@ from .capi import (XT_BATCH_HIPGRAPH, XT_BATCH_PER_GRAPH, XT_BATCH_PERSISTENT,  
                    XT_CHECK_SYMMETRY, XT_ORIENT_DEGREE, XT_ORIENT_ID, XT_PER_VERTEX,
                    XT_VALIDATE, XtError)
 
+DEFAULT_LIGHT_CUT = 32  # xt_ctx default: edges with both out-lists <= 32 are "light"
+
 __all__ = ["Context", "DeviceGraph", "TriangleCountingSingle", "rgg_generate", "rgg_radius",
            "vite_read", "vite_write", "edgelist_to_csr", "batch_params", "batch_graph_csr",
            "Batch", "Server", "XtError"]
@@ -205,6 +207,13 @@ class DeviceGraph:
                                                        ctypes.byref(b), ctypes.byref(sm),
                                                        ctypes.byref(nep)))
         return b.value, sm.value, nep.value
+
+    def stats(self, orient="degree"):
+        """Algorithmic byte figures (whole count and the group kernel's share)."""
+        st = capi.XtGraphStats()
+        self.ctx._check(self.ctx._L.xt_graph_stats_get(self.ctx.h, self.h, _flags(orient),
+                                                       ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in st._fields_}
 
     def close(self):
         if self.h:

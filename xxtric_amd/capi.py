@@ -38,6 +38,7 @@ XT_BATCH_PER_GRAPH = 2
 XT_H2D = 1
 XT_D2H = 2
 XT_D2D = 3
+XT_D2D_KERNEL = 4
 
 _i64 = ctypes.c_int64
 _u64 = ctypes.c_uint64
@@ -50,7 +51,13 @@ _pp = ctypes.POINTER(ctypes.c_void_p)
 
 class XtTiming(ctypes.Structure):
     _fields_ = [("orient_ms", _dbl), ("count_ms", _dbl), ("total_ms", _dbl),
-                ("ne_plus", _i64), ("n_deferred", _i64), ("n_block", _i64)]
+                ("light_ms", _dbl), ("items_ms", _dbl), ("group_ms", _dbl),
+                ("ne_plus", _i64), ("n_heavy", _i64), ("n_tiles", _i64)]
+
+
+class XtGraphStats(ctypes.Structure):
+    _fields_ = [("alg_bytes", _u64), ("sum_min", _u64), ("ne_plus", _i64),
+                ("sum_min_heavy", _u64), ("heavy_edges", _i64), ("group_bytes", _u64)]
 
 
 class XtBatchTiming(ctypes.Structure):
@@ -91,6 +98,7 @@ _SIGS = {
     "xt_graph_download": (_int, [_p, _p, _p, _p]),
     "xt_graph_count": (_int, [_p, _p, _u32, _p, _p]),
     "xt_graph_alg_bytes": (_int, [_p, _p, _u32, _p, _p, _p]),
+    "xt_graph_stats_get": (_int, [_p, _p, _u32, ctypes.POINTER(XtGraphStats)]),
     "xt_graph_destroy": (_int, [_p, _p]),
     "xt_rgg_generate": (_int, [_i64, _u32, _dbl, _p, _p, _i64, _p]),
     "xt_rgg_radius": (_dbl, [_i64]),

@@ -14,6 +14,21 @@ int launch_inner(xt_ctx* ctx, int64_t start, int64_t end, int64_t jstart, int64_
                  const int64_t* L, int64_t* out, hipStream_t st);
 int launch_add_total(xt_ctx* ctx, int64_t* out, hipStream_t st);
 
+__global__ __launch_bounds__(256) void k_copy16(int4* __restrict__ dst, const int4* __restrict__ src,
+                                                int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+static int launch_copy16(xt_ctx* ctx, void* dst, const void* src, size_t bytes, hipStream_t st) {
+  const int64_t n = (int64_t)(bytes / 16);
+  if (n == 0) return XT_OK;
+  hipLaunchKernelGGL(k_copy16, dim3((unsigned)(ctx->num_cus * 16)), dim3(256), 0, st,
+                     static_cast<int4*>(dst), static_cast<const int4*>(src), n);
+  XT_HIP(ctx, hipGetLastError());
+  return XT_OK;
+}
+
 int set_err(xt_ctx* ctx, int code, const std::string& msg) {
   if (ctx) ctx->err = msg;
   return code;
@@ -73,7 +88,8 @@ int count_device_csr(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne, const int6
   XT_TRY(orient<T>(ctx, s, nv, ne, off, tails, mode, st));
   if (timed) XT_HIP(ctx, hipEventRecord(ctx->ev[kEvOriented], st));
   if (pv) XT_TRY(fill_u64(ctx, pv, nv, 0, st));
-  XT_TRY(count_oriented(ctx, s, nv, ne, pv, nullptr, nullptr, st));
+  XT_TRY(count_oriented(ctx, s, nv, ne, pv, nullptr, nullptr, st,
+                        timed ? &ctx->ev[kEvLight] : nullptr));
   if (timed) XT_HIP(ctx, hipEventRecord(ctx->ev[kEvCounted], st));
   return XT_OK;
 }
@@ -94,12 +110,19 @@ static int finish_count(xt_ctx* ctx, int64_t nv, uint32_t flags, hipStream_t st,
   XT_HIP(ctx, hipStreamSynchronize(st));
   if (h[kBadIds]) return set_err(ctx, XT_ERR_INPUT, "neighbour id out of range [0, nv)");
   if (total) *total = h[kTotal];
-  int64_t nep = 0;
+  int64_t nep = 0, nheavy = 0, ntiles = 0;
+  const bool worked = nv > 0 && ctx->s.itoff.p != nullptr && ctx->s.ooff.p != nullptr;
   if (nv > 0)
     XT_HIP(ctx, hipMemcpy(&nep, ctx->s.ooff.as<int64_t>() + nv, sizeof(int64_t), hipMemcpyDeviceToHost));
+  const bool split = worked && nep > 0;  // the light/items/group phases ran
+  if (split) {
+    XT_HIP(ctx, hipMemcpy(&nheavy, ctx->s.itoff.as<int64_t>() + nv, sizeof(int64_t), hipMemcpyDeviceToHost));
+    XT_HIP(ctx, hipMemcpy(&ntiles, ctx->s.tileoff.as<int64_t>() + nv, sizeof(int64_t), hipMemcpyDeviceToHost));
+  }
+  ctx->timing = xt_timing{};
   ctx->timing.ne_plus = nep;
-  ctx->timing.n_deferred = (int64_t)h[kDeferred];
-  ctx->timing.n_block = (int64_t)h[kBlockEdges];
+  ctx->timing.n_heavy = nheavy;
+  ctx->timing.n_tiles = ntiles;
   if (timed) {
     float a = 0, b = 0, c = 0;
     XT_HIP(ctx, hipEventElapsedTime(&a, ctx->ev[kEvStart], ctx->ev[kEvOriented]));
@@ -108,6 +131,15 @@ static int finish_count(xt_ctx* ctx, int64_t nv, uint32_t flags, hipStream_t st,
     ctx->timing.orient_ms = a;
     ctx->timing.count_ms = b;
     ctx->timing.total_ms = c;
+    if (split) {
+      float l = 0, i = 0, g = 0;
+      XT_HIP(ctx, hipEventElapsedTime(&l, ctx->ev[kEvOriented], ctx->ev[kEvLight]));
+      XT_HIP(ctx, hipEventElapsedTime(&i, ctx->ev[kEvLight], ctx->ev[kEvItems]));
+      XT_HIP(ctx, hipEventElapsedTime(&g, ctx->ev[kEvItems], ctx->ev[kEvCounted]));
+      ctx->timing.light_ms = l;
+      ctx->timing.items_ms = i;
+      ctx->timing.group_ms = g;
+    }
   }
   return XT_OK;
 }
@@ -226,11 +258,18 @@ int xt_memcpy(xt_ctx* ctx, void* dst, const void* src, size_t bytes, int kind) {
   if (!ctx || (bytes > 0 && (!dst || !src))) return XT_ERR_ARG;
   std::lock_guard<std::recursive_mutex> lk(ctx->mu);
   XT_HIP(ctx, hipSetDevice(ctx->device));
+  if (kind == XT_D2D_KERNEL) {
+    if (bytes % 16 != 0 || ((uintptr_t)dst | (uintptr_t)src) % 16 != 0)
+      return set_err(ctx, XT_ERR_ARG, "kernel copy needs 16-byte aligned sizes and pointers");
+    XT_TRY(launch_copy16(ctx, dst, src, bytes, ctx->stream));
+    XT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return XT_OK;
+  }
   hipMemcpyKind k = kind == XT_H2D ? hipMemcpyHostToDevice
                     : kind == XT_D2H ? hipMemcpyDeviceToHost
                     : kind == XT_D2D ? hipMemcpyDeviceToDevice
                                      : hipMemcpyDefault;
-  if (kind < XT_H2D || kind > XT_D2D) return set_err(ctx, XT_ERR_ARG, "bad copy kind");
+  if (kind < XT_H2D || kind > XT_D2D_KERNEL) return set_err(ctx, XT_ERR_ARG, "bad copy kind");
   if (bytes == 0) return XT_OK;
   XT_HIP(ctx, hipDeviceSynchronize());
   XT_HIP(ctx, hipMemcpy(dst, src, bytes, k));
@@ -432,6 +471,21 @@ int xt_graph_alg_bytes(xt_ctx* ctx, xt_graph* g, uint32_t flags, uint64_t* bytes
   if (sum_min) *sum_min = sm;
   if (ne_plus) *ne_plus = nep;
   if (bytes) *bytes = 8ULL * (uint64_t)(g->nv + 1) + 4ULL * (uint64_t)nep + 4ULL * sm;
+  return XT_OK;
+}
+
+int xt_graph_stats_get(xt_ctx* ctx, xt_graph* g, uint32_t flags, xt_graph_stats* out) {
+  if (!ctx || !g || !out) return XT_ERR_ARG;
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  xt_graph_stats st = {};
+  XT_TRY(xt_graph_alg_bytes(ctx, g, flags, &st.alg_bytes, &st.sum_min, &st.ne_plus));
+  uint64_t h[2] = {0, 0};
+  XT_HIP(ctx, hipMemcpy(h, ctx->s.counters.as<uint64_t>() + kStatMinHeavy, sizeof(h),
+                        hipMemcpyDeviceToHost));
+  st.sum_min_heavy = h[0];
+  st.heavy_edges = (int64_t)h[1];
+  st.group_bytes = 4ULL * (st.sum_min_heavy + (uint64_t)st.heavy_edges);
+  *out = st;
   return XT_OK;
 }
 

@@ -1,34 +1,41 @@
 // count.hip -- intersection and reduction kernels over the oriented CSR (gfx950).
 //
 // Every oriented edge u->v closes one triangle per common out-neighbour w of u
-// and v.  The shorter of the two out-lists is walked and each of its ids is
-// searched in the longer one (lower_bound restarted from the previous hit, so
-// a walk over a sorted list never searches backwards).
+// and v, so the count is the sum over oriented edges of |N+(u) & N+(v)|.
+// Out-lists hold ascending ids.  d+(x) = |N+(x)|.
 //
-//  k_count_small: one thread per oriented edge (chunks of 2048 edges per
-//                 256-thread workgroup, strided so a wave reads consecutive
-//                 edges).  Edges whose shorter list exceeds `small_cut` are
-//                 appended to a device list instead.
-//  k_count_wave:  one wave64 per listed edge, lanes striding the shorter list.
-//  Reduction:     per-thread uint64 -> wave64 shuffle tree -> LDS -> one
-//                 64-bit atomic per workgroup.
+// Edges are binned by the longer of their two out-lists:
+//  * light edges (both lists <= light_cut): one thread per edge walks the
+//    shorter list and searches each id in the longer one (lower_bound restarted
+//    after the previous hit) -- k_count_light, which also classifies the rest;
+//  * heavy edges: the edge belongs to its endpoint x with the longer list (ties
+//    to the source).  Each such x gets an item list -- the other endpoints y of
+//    its heavy edges (k_fill_items).  k_count_groups, a persistent kernel,
+//    draws (x, up to 256 items) tiles from a device work queue, stages N+(x)
+//    once per tile in an LDS hash table, and streams the items' shorter lists
+//    N+(y) through it with coalesced loads: each oriented edge reads only its
+//    shorter list, the longer one is read once per tile.
+// Reductions: per-thread uint64 -> wave64 shuffle tree -> LDS -> one 64-bit
+// atomic per workgroup (per tile for the group kernel).
 //
-// Per-vertex credit (XT_PER_VERTEX): the triangle {u, v, w} belongs to
-// min(u, v, w) (xxtric.hpp:165).  Hits with w > min(u, v) are summed in a
-// register and credited with one atomic per edge; hits with w < min(u, v)
-// are credited to w directly.
+// Per-vertex credit (XT_PER_VERTEX): the triangle {a, b, w} belongs to
+// min(a, b, w) (xxtric.hpp:165); hits credited to the edge's endpoints are
+// summed in registers, hits credited to w use one atomic each.
 #include "xt_device.hpp"
 #include "xt_internal.hpp"
 
 namespace xt {
 
+constexpr int kGraphSlots = 64;        // per-tile LDS slots for per-graph batch totals
+constexpr int kItemsPerTile = 256;     // items of one group tile (one per thread)
+constexpr int kHashSlots = 8192;       // LDS hash capacity: staged lists up to 4096 ids
+constexpr int kGroupBlocksPerCU = 4;   // LDS use of k_count_groups allows four per CU
+
 struct EdgeLists {
   int64_t s0, s1, l0, l1;  // shorter [s0,s1), longer [l0,l1) in otails
 };
 
-__device__ __forceinline__ EdgeLists edge_lists(const int64_t* __restrict__ ooff, int64_t a0,
-                                                int64_t a1, int64_t v) {
-  const int64_t b0 = ooff[v], b1 = ooff[v + 1];
+__device__ __forceinline__ EdgeLists edge_lists(int64_t a0, int64_t a1, int64_t b0, int64_t b1) {
   EdgeLists L;
   if (a1 - a0 <= b1 - b0) {
     L.s0 = a0; L.s1 = a1; L.l0 = b0; L.l1 = b1;
@@ -38,148 +45,87 @@ __device__ __forceinline__ EdgeLists edge_lists(const int64_t* __restrict__ ooff
   return L;
 }
 
-// Per-graph totals of a batch: a tile touches few graphs, so hits are summed
-// in LDS slots relative to the tile's first graph and flushed once per tile.
-constexpr int kGraphSlots = 64;
+// Rows of the oriented edges [eb, ee) staged in LDS when they fit.
+struct TileRows {
+  int64_t ulo, uhi, span;
+  bool in_lds;
+};
 
+__device__ __forceinline__ TileRows tile_rows(const int64_t* __restrict__ ooff, int64_t nv,
+                                              int64_t eb, int64_t ee, int64_t* s_off,
+                                              int64_t* s_lohi) {
+  if (threadIdx.x == 0) {
+    const int64_t lo = row_of(ooff, 0, nv - 1, eb);
+    s_lohi[0] = lo;
+    s_lohi[1] = row_of(ooff, lo, nv - 1, ee - 1);
+  }
+  __syncthreads();
+  TileRows r;
+  r.ulo = s_lohi[0];
+  r.uhi = s_lohi[1];
+  r.span = r.uhi - r.ulo + 1;
+  r.in_lds = r.span <= kOffCap;
+  if (r.in_lds)
+    for (int64_t k = threadIdx.x; k <= r.span; k += blockDim.x) s_off[k] = ooff[r.ulo + k];
+  __syncthreads();
+  return r;
+}
+
+__device__ __forceinline__ int64_t tile_row_of(const TileRows& r, const int64_t* s_off,
+                                               const int64_t* __restrict__ ooff, int64_t e) {
+  return r.in_lds ? r.ulo + lds_row_of(s_off, (int)r.span, e) : row_of(ooff, r.ulo, r.uhi, e);
+}
+
+__device__ __forceinline__ int64_t tile_off(const TileRows& r, const int64_t* s_off,
+                                            const int64_t* __restrict__ ooff, int64_t u) {
+  return r.in_lds ? s_off[u - r.ulo] : ooff[u];
+}
+
+// ---------------------------------------------------------------------------
+// Light edges counted inline; heavy edges counted per owning endpoint.
 template <bool PV, bool BATCH>
-__global__ __launch_bounds__(kBlock) void k_count_small(
-    int64_t nv, const int64_t* __restrict__ ooff, const int32_t* __restrict__ ot, int small_cut,
-    int64_t defer_cap, uint64_t* __restrict__ counters, int64_t* __restrict__ defer,
+__global__ __launch_bounds__(kBlock) void k_count_light(
+    int64_t nv, const int64_t* __restrict__ ooff, const int32_t* __restrict__ ot, int light_cut,
+    uint64_t* __restrict__ counters, uint32_t* __restrict__ itcnt,
     unsigned long long* __restrict__ pv, const int32_t* __restrict__ gid,
     unsigned long long* __restrict__ gtotal) {
   __shared__ int64_t s_off[kOffCap + 1];
   __shared__ int64_t s_lohi[2];
   __shared__ uint64_t s_red[kBlock / kWave];
   __shared__ unsigned long long s_g[BATCH ? kGraphSlots : 1];
-  __shared__ int64_t s_tile;
   const int64_t nep = ooff[nv];
-  const int64_t ntiles = (nep + kChunk - 1) / kChunk;
-  for (;;) {
-    // Tiles come from blockIdx (one tile per workgroup) or, for batches, from
-    // a device work queue drained by a fixed grid of persistent workgroups.
-    int64_t tile;
-    if constexpr (BATCH) {
-      if (threadIdx.x == 0) s_tile = (int64_t)atomicAdd((unsigned long long*)&counters[kTileQueue], 1ull);
-      if (threadIdx.x < kGraphSlots) s_g[threadIdx.x] = 0;
-      __syncthreads();
-      tile = s_tile;
-    } else {
-      tile = blockIdx.x;
-    }
-    if (tile >= ntiles) break;  // uniform across the workgroup
-    const int64_t eb = tile * kChunk;
-    const int64_t ee = min(nep, eb + (int64_t)kChunk);
-    if (threadIdx.x == 0) {
-      const int64_t lo = row_of(ooff, 0, nv - 1, eb);
-      s_lohi[0] = lo;
-      s_lohi[1] = row_of(ooff, lo, nv - 1, ee - 1);
-    }
-    __syncthreads();
-    const int64_t ulo = s_lohi[0], uhi = s_lohi[1], span = uhi - ulo + 1;
-    const bool in_lds = span <= kOffCap;
-    if (in_lds)
-      for (int64_t k = threadIdx.x; k <= span; k += kBlock) s_off[k] = ooff[ulo + k];
-    const int32_t g0 = BATCH ? gid[ulo] : 0;
-    __syncthreads();
+  const int64_t eb = (int64_t)blockIdx.x * kChunk;
+  if (eb >= nep) return;  // whole workgroup leaves together
+  const int64_t ee = min(nep, eb + (int64_t)kChunk);
+  if (BATCH && threadIdx.x < kGraphSlots) s_g[threadIdx.x] = 0;
+  const TileRows r = tile_rows(ooff, nv, eb, ee, s_off, s_lohi);
+  const int32_t g0 = BATCH ? gid[r.ulo] : 0;
 
-    uint64_t hits_all = 0;
-#pragma unroll 1
-    for (int j = 0; j < kPerThread; j++) {
-      const int64_t e = eb + (int64_t)j * kBlock + threadIdx.x;
-      if (e >= ee) break;
-      const int64_t u = in_lds ? ulo + lds_row_of(s_off, (int)span, e) : row_of(ooff, ulo, uhi, e);
-      const int64_t a0 = in_lds ? s_off[u - ulo] : ooff[u];
-      const int64_t a1 = in_lds ? s_off[u - ulo + 1] : ooff[u + 1];
-      const int64_t v = ot[e];
-      const EdgeLists L = edge_lists(ooff, a0, a1, v);
-      const int64_t ns = L.s1 - L.s0;
-      if (ns == 0) continue;
-      if (ns > small_cut) {
-        const unsigned long long idx = atomicAdd((unsigned long long*)&counters[kDeferred], 1ull);
-        if ((int64_t)idx < defer_cap) {
-          defer[2 * idx] = e;
-          defer[2 * idx + 1] = u;
-          continue;
-        }
-        // list full: fall through and count the edge here
-      }
-      const int64_t mmin = u < v ? u : v;
-      uint64_t hits = 0, at_min = 0;
-      int64_t pos = L.l0;
-      const int32_t lmax = ot[L.l1 - 1];
-      for (int64_t k = L.s0; k < L.s1; k++) {
-        const int32_t w = ot[k];
-        if (w > lmax) break;
-        pos = lower_bound_dev<int32_t>(ot, pos, L.l1, w);
-        if (ot[pos] == w) {
-          hits++;
-          if constexpr (PV) {
-            if ((int64_t)w < mmin)
-              atomicAdd(&pv[w], 1ull);
-            else
-              at_min++;
-          }
-          pos++;
-          if (pos >= L.l1) break;
-        }
-      }
-      if constexpr (PV) {
-        if (at_min) atomicAdd(&pv[mmin], (unsigned long long)at_min);
-      }
-      if constexpr (BATCH) {
-        if (hits) {
-          const int32_t slot = gid[u] - g0;
-          if (slot < kGraphSlots)
-            atomicAdd(&s_g[slot], (unsigned long long)hits);
-          else
-            atomicAdd(&gtotal[gid[u]], (unsigned long long)hits);
-        }
-      } else {
-        hits_all += hits;
-      }
-    }
-    if constexpr (BATCH) {
-      __syncthreads();
-      if (threadIdx.x < kGraphSlots && s_g[threadIdx.x])
-        atomicAdd(&gtotal[g0 + threadIdx.x], s_g[threadIdx.x]);
-      __syncthreads();  // s_g and s_tile are reused by the next tile
-    } else {
-      const uint64_t bsum = block_sum_u64<kBlock>(hits_all, s_red);
-      if (threadIdx.x == 0 && bsum)
-        atomicAdd((unsigned long long*)&counters[kTotal], (unsigned long long)bsum);
-      break;
-    }
-  }
-}
-
-template <bool PV, bool BATCH>
-__global__ __launch_bounds__(256) void k_count_wave(const int64_t* __restrict__ ooff,
-                                                    const int32_t* __restrict__ ot,
-                                                    int64_t defer_cap,
-                                                    uint64_t* __restrict__ counters,
-                                                    const int64_t* __restrict__ defer,
-                                                    unsigned long long* __restrict__ pv,
-                                                    const int32_t* __restrict__ gid,
-                                                    unsigned long long* __restrict__ gtotal) {
-  __shared__ uint64_t s_red[256 / kWave];
-  int64_t nd = (int64_t)counters[kDeferred];
-  if (nd > defer_cap) nd = defer_cap;
-  const int lane = threadIdx.x & (kWave - 1);
-  const int64_t wid = ((int64_t)blockIdx.x * 256 + threadIdx.x) / kWave;
-  const int64_t nw = (int64_t)gridDim.x * (256 / kWave);
   uint64_t hits_all = 0;
-  for (int64_t d = wid; d < nd; d += nw) {  // uniform per wave
-    const int64_t e = defer[2 * d];
-    const int64_t u = defer[2 * d + 1];
+#pragma unroll 1
+  for (int j = 0; j < kPerThread; j++) {
+    const int64_t e = eb + (int64_t)j * kBlock + threadIdx.x;
+    if (e >= ee) break;
+    const int64_t u = tile_row_of(r, s_off, ooff, e);
+    const int64_t a0 = tile_off(r, s_off, ooff, u), a1 = tile_off(r, s_off, ooff, u + 1);
     const int64_t v = ot[e];
-    const EdgeLists L = edge_lists(ooff, ooff[u], ooff[u + 1], v);
+    const int64_t b0 = ooff[v], b1 = ooff[v + 1];
+    const int64_t du = a1 - a0, dv = b1 - b0;
+    if (du > light_cut || dv > light_cut) {
+      atomicAdd(&itcnt[du >= dv ? u : v], 1u);  // heavy: item of the longer list's owner
+      continue;
+    }
+    const EdgeLists L = edge_lists(a0, a1, b0, b1);
+    if (L.s1 == L.s0) continue;
     const int64_t mmin = u < v ? u : v;
     uint64_t hits = 0, at_min = 0;
-    for (int64_t k = L.s0 + lane; k < L.s1; k += kWave) {
+    int64_t pos = L.l0;
+    const int32_t lmax = ot[L.l1 - 1];
+    for (int64_t k = L.s0; k < L.s1; k++) {
       const int32_t w = ot[k];
-      const int64_t pos = lower_bound_dev<int32_t>(ot, L.l0, L.l1, w);
-      if (pos < L.l1 && ot[pos] == w) {
+      if (w > lmax) break;
+      pos = lower_bound_dev<int32_t>(ot, pos, L.l1, w);
+      if (ot[pos] == w) {
         hits++;
         if constexpr (PV) {
           if ((int64_t)w < mmin)
@@ -187,95 +133,285 @@ __global__ __launch_bounds__(256) void k_count_wave(const int64_t* __restrict__ 
           else
             at_min++;
         }
+        pos++;
+        if (pos >= L.l1) break;
       }
     }
     if constexpr (PV) {
-      at_min = wave_sum_u64(at_min);
-      if (lane == 0 && at_min) atomicAdd(&pv[mmin], (unsigned long long)at_min);
+      if (at_min) atomicAdd(&pv[mmin], (unsigned long long)at_min);
     }
     if constexpr (BATCH) {
-      hits = wave_sum_u64(hits);
-      if (lane == 0 && hits) atomicAdd(&gtotal[gid[u]], (unsigned long long)hits);
+      if (hits) {
+        const int32_t slot = gid[u] - g0;
+        if (slot < kGraphSlots)
+          atomicAdd(&s_g[slot], (unsigned long long)hits);
+        else
+          atomicAdd(&gtotal[gid[u]], (unsigned long long)hits);
+      }
     } else {
       hits_all += hits;
     }
   }
-  if constexpr (!BATCH) {
-    const uint64_t bsum = block_sum_u64<256>(hits_all, s_red);
+  if constexpr (BATCH) {
+    __syncthreads();
+    if (threadIdx.x < kGraphSlots && s_g[threadIdx.x])
+      atomicAdd(&gtotal[g0 + threadIdx.x], s_g[threadIdx.x]);
+  } else {
+    const uint64_t bsum = block_sum_u64<kBlock>(hits_all, s_red);
     if (threadIdx.x == 0 && bsum) atomicAdd((unsigned long long*)&counters[kTotal], (unsigned long long)bsum);
   }
 }
 
-// Sum over oriented edges of min(d+(u), d+(v)) -- the algorithmic probe count.
+// Writes the item lists: y goes to items[itoff[x] ..] of its owner x.  itcnt
+// counts down to zero (slot order within a list is irrelevant).
+__global__ __launch_bounds__(kBlock) void k_fill_items(int64_t nv, const int64_t* __restrict__ ooff,
+                                                       const int32_t* __restrict__ ot, int light_cut,
+                                                       const int64_t* __restrict__ itoff,
+                                                       uint32_t* __restrict__ itcnt,
+                                                       int32_t* __restrict__ items) {
+  __shared__ int64_t s_off[kOffCap + 1];
+  __shared__ int64_t s_lohi[2];
+  const int64_t nep = ooff[nv];
+  const int64_t eb = (int64_t)blockIdx.x * kChunk;
+  if (eb >= nep) return;
+  const int64_t ee = min(nep, eb + (int64_t)kChunk);
+  const TileRows r = tile_rows(ooff, nv, eb, ee, s_off, s_lohi);
+#pragma unroll 1
+  for (int j = 0; j < kPerThread; j++) {
+    const int64_t e = eb + (int64_t)j * kBlock + threadIdx.x;
+    if (e >= ee) break;
+    const int64_t u = tile_row_of(r, s_off, ooff, e);
+    const int64_t du = tile_off(r, s_off, ooff, u + 1) - tile_off(r, s_off, ooff, u);
+    const int64_t v = ot[e];
+    const int64_t dv = ooff[v + 1] - ooff[v];
+    if (du > light_cut || dv > light_cut) {
+      const int64_t x = du >= dv ? u : v, y = du >= dv ? v : u;
+      const uint32_t k = atomicSub(&itcnt[x], 1u) - 1u;
+      items[itoff[x] + k] = (int32_t)y;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_tile_counts(int64_t nv, const int64_t* __restrict__ itoff,
+                                                     uint32_t* __restrict__ tcnt) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < nv; x += stride)
+    tcnt[x] = (uint32_t)((itoff[x + 1] - itoff[x] + kItemsPerTile - 1) / kItemsPerTile);
+}
+
+__device__ __forceinline__ uint32_t hash_slot(int32_t w, int bits) {
+  return ((uint32_t)w * 2654435761u) >> (32 - bits);
+}
+
+template <bool PV, bool BATCH>
+__global__ __launch_bounds__(kBlock) void k_count_groups(
+    int64_t nv, const int64_t* __restrict__ ooff, const int32_t* __restrict__ ot,
+    const int64_t* __restrict__ itoff, const int32_t* __restrict__ items,
+    const int64_t* __restrict__ tileoff, uint64_t* __restrict__ counters,
+    unsigned long long* __restrict__ pv, const int32_t* __restrict__ gid,
+    unsigned long long* __restrict__ gtotal) {
+  __shared__ int32_t s_hash[kHashSlots];
+  __shared__ int64_t s_start[kItemsPerTile + 1];
+  __shared__ int64_t s_base[kItemsPerTile];
+  __shared__ int32_t s_y[kItemsPerTile];
+  __shared__ int64_t s_scan[kBlock / kWave];
+  __shared__ uint64_t s_red[kBlock / kWave];
+  __shared__ int64_t s_tile, s_x;
+  const int64_t ntiles = tileoff[nv];
+  for (;;) {
+    if (threadIdx.x == 0) {
+      const int64_t t = (int64_t)atomicAdd((unsigned long long*)&counters[kTileQueue], 1ull);
+      s_tile = t;
+      s_x = t < ntiles ? row_of(tileoff, 0, nv - 1, t) : 0;
+    }
+    __syncthreads();
+    const int64_t tile = s_tile;
+    if (tile >= ntiles) break;  // uniform across the workgroup
+    const int64_t x = s_x;
+    const int64_t a0 = ooff[x], a1 = ooff[x + 1], dx = a1 - a0;
+    const int64_t i0 = itoff[x] + (tile - tileoff[x]) * kItemsPerTile;
+    const int64_t i1 = min(i0 + (int64_t)kItemsPerTile, itoff[x + 1]);
+    const int nitems = (int)(i1 - i0);
+    const bool hashed = dx <= kHashSlots / 2;
+    int bits = 6;
+    while ((1ll << bits) < 2 * dx) bits++;
+    const uint32_t mask = (1u << bits) - 1u;
+    if (hashed) {
+      for (int k = threadIdx.x; k <= (int)mask; k += kBlock) s_hash[k] = -1;
+    }
+    // items of this tile: lengths and bases of the streamed lists
+    int64_t len = 0, base = 0;
+    int32_t y = 0;
+    if ((int)threadIdx.x < nitems) {
+      y = items[i0 + threadIdx.x];
+      base = ooff[y];
+      len = ooff[y + 1] - base;
+    }
+    int64_t W;
+    const int64_t start = block_excl_scan_i64<kBlock>(len, s_scan, &W);
+    s_start[threadIdx.x] = start;
+    s_base[threadIdx.x] = base;
+    s_y[threadIdx.x] = y;
+    if (threadIdx.x == 0) s_start[nitems] = W;
+    __syncthreads();
+    if (hashed) {
+      for (int64_t k = threadIdx.x; k < dx; k += kBlock) {
+        const int32_t key = ot[a0 + k];
+        uint32_t h = hash_slot(key, bits);
+        while (atomicCAS(&s_hash[h], -1, key) != -1) h = (h + 1) & mask;
+      }
+    }
+    __syncthreads();
+    uint64_t hits = 0, at_x = 0;
+    int lo_item = 0;
+    for (int64_t j = threadIdx.x; j < W; j += kBlock) {
+      // item of element j: last k with s_start[k] <= j (searched from the previous one)
+      int lo = lo_item, hi = nitems - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (s_start[mid] <= j)
+          lo = mid;
+        else
+          hi = mid - 1;
+      }
+      lo_item = lo;
+      const int32_t w = ot[s_base[lo] + (j - s_start[lo])];
+      bool found;
+      if (hashed) {
+        uint32_t h = hash_slot(w, bits);
+        int32_t sl;
+        while ((sl = s_hash[h]) != w && sl != -1) h = (h + 1) & mask;
+        found = sl == w;
+      } else {
+        const int64_t p = lower_bound_dev<int32_t>(ot, a0, a1, w);
+        found = p < a1 && ot[p] == w;
+      }
+      if (found) {
+        hits++;
+        if constexpr (PV) {
+          const int64_t yy = s_y[lo];
+          int64_t m = x < yy ? x : yy;
+          if ((int64_t)w < m) m = w;
+          if (m == x)
+            at_x++;
+          else
+            atomicAdd(&pv[m], 1ull);
+        }
+      }
+    }
+    const uint64_t bh = block_sum_u64<kBlock>(hits, s_red);
+    if constexpr (PV) {
+      __syncthreads();
+      const uint64_t bx = block_sum_u64<kBlock>(at_x, s_red);
+      if (threadIdx.x == 0 && bx) atomicAdd(&pv[x], (unsigned long long)bx);
+    }
+    if (threadIdx.x == 0 && bh) {
+      if constexpr (BATCH)
+        atomicAdd(&gtotal[gid[x]], (unsigned long long)bh);
+      else
+        atomicAdd((unsigned long long*)&counters[kTotal], (unsigned long long)bh);
+    }
+    __syncthreads();  // LDS is reused by the next tile
+  }
+}
+
+// Sums over oriented edges of min(d+(u), d+(v)) (all edges, and the heavy ones)
+// and the number of heavy edges -- the algorithmic byte figures.
 __global__ __launch_bounds__(kBlock) void k_alg_stats(int64_t nv, const int64_t* __restrict__ ooff,
-                                                      const int32_t* __restrict__ ot,
+                                                      const int32_t* __restrict__ ot, int light_cut,
                                                       uint64_t* __restrict__ counters) {
+  __shared__ int64_t s_off[kOffCap + 1];
   __shared__ int64_t s_lohi[2];
   __shared__ uint64_t s_red[kBlock / kWave];
   const int64_t nep = ooff[nv];
   const int64_t eb = (int64_t)blockIdx.x * kChunk;
   if (eb >= nep) return;
   const int64_t ee = min(nep, eb + (int64_t)kChunk);
-  if (threadIdx.x == 0) {
-    const int64_t lo = row_of(ooff, 0, nv - 1, eb);
-    s_lohi[0] = lo;
-    s_lohi[1] = row_of(ooff, lo, nv - 1, ee - 1);
-  }
-  __syncthreads();
-  uint64_t acc = 0;
+  const TileRows r = tile_rows(ooff, nv, eb, ee, s_off, s_lohi);
+  uint64_t acc = 0, heavy = 0, nheavy = 0;
   for (int j = 0; j < kPerThread; j++) {
     const int64_t e = eb + (int64_t)j * kBlock + threadIdx.x;
     if (e >= ee) break;
-    const int64_t u = row_of(ooff, s_lohi[0], s_lohi[1], e);
+    const int64_t u = tile_row_of(r, s_off, ooff, e);
     const int64_t v = ot[e];
-    const int64_t du = ooff[u + 1] - ooff[u], dv = ooff[v + 1] - ooff[v];
-    acc += (uint64_t)(du < dv ? du : dv);
+    const int64_t du = tile_off(r, s_off, ooff, u + 1) - tile_off(r, s_off, ooff, u);
+    const int64_t dv = ooff[v + 1] - ooff[v];
+    const uint64_t m = (uint64_t)(du < dv ? du : dv);
+    acc += m;
+    if (du > light_cut || dv > light_cut) {
+      heavy += m;
+      nheavy++;
+    }
   }
-  const uint64_t bsum = block_sum_u64<kBlock>(acc, s_red);
-  if (threadIdx.x == 0 && bsum) atomicAdd((unsigned long long*)&counters[kStatMin], (unsigned long long)bsum);
-}
-
-static int64_t defer_capacity(int64_t ne_bound) {
-  return ne_bound < (int64_t)(1 << 26) ? (ne_bound > 0 ? ne_bound : 1) : (int64_t)(1 << 26);
+  uint64_t b = block_sum_u64<kBlock>(acc, s_red);
+  if (threadIdx.x == 0 && b) atomicAdd((unsigned long long*)&counters[kStatMin], (unsigned long long)b);
+  __syncthreads();
+  b = block_sum_u64<kBlock>(heavy, s_red);
+  if (threadIdx.x == 0 && b) atomicAdd((unsigned long long*)&counters[kStatMinHeavy], (unsigned long long)b);
+  __syncthreads();
+  b = block_sum_u64<kBlock>(nheavy, s_red);
+  if (threadIdx.x == 0 && b) atomicAdd((unsigned long long*)&counters[kStatHeavyEdges], (unsigned long long)b);
 }
 
 int count_oriented(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne_bound, uint64_t* pv,
-                   const int32_t* gid, uint64_t* gtotal, hipStream_t st) {
+                   const int32_t* gid, uint64_t* gtotal, hipStream_t st, hipEvent_t* evs) {
   if (nv <= 0 || ne_bound <= 0) return XT_OK;
   const bool batch = gid != nullptr;
   const int64_t nchunks = (ne_bound + kChunk - 1) / kChunk;
-  // Deferred-edge list: 2 int64 per edge, capped; overflow is counted inline.
-  const int64_t defer_cap = defer_capacity(ne_bound);
-  XT_TRY(ensure(ctx, s.defer, (size_t)defer_cap * 2 * sizeof(int64_t)));
+  XT_TRY(ensure(ctx, s.itcnt, (size_t)nv * sizeof(uint32_t)));
+  XT_TRY(ensure(ctx, s.itoff, (size_t)(nv + 1) * sizeof(int64_t)));
+  XT_TRY(ensure(ctx, s.items, (size_t)ne_bound * sizeof(int32_t)));
+  XT_TRY(ensure(ctx, s.tileoff, (size_t)(nv + 1) * sizeof(int64_t)));
+  XT_TRY(ensure(ctx, s.deg, (size_t)nv * sizeof(uint32_t)));  // reused as per-owner tile counts
   uint64_t* counters = s.counters.as<uint64_t>();
-  XT_TRY(fill_u64(ctx, counters + kDeferred, 2, 0, st));
-  if (batch) XT_TRY(fill_u64(ctx, counters + kTileQueue, 1, 0, st));
+  XT_TRY(fill_u64(ctx, counters + kTileQueue, 1, 0, st));
+  XT_TRY(fill_u32(ctx, s.itcnt.as<uint32_t>(), nv, 0u, st));
   const int64_t* ooff = s.ooff.as<int64_t>();
   const int32_t* ot = s.otails.as<int32_t>();
-  int64_t* defer = s.defer.as<int64_t>();
   unsigned long long* pvp = (unsigned long long*)pv;
   unsigned long long* gt = (unsigned long long*)gtotal;
-  const unsigned wave_grid = (unsigned)(ctx->num_cus * 8);
-  int64_t persist = (int64_t)ctx->num_cus * ctx->persist_blocks;
-  const unsigned small_grid = (unsigned)(batch ? (nchunks < persist ? nchunks : persist) : nchunks);
-#define XT_COUNT_LAUNCH(P, B)                                                                         \
-  do {                                                                                                \
-    hipLaunchKernelGGL((k_count_small<P, B>), dim3(small_grid), dim3(kBlock), 0, st, nv, ooff, ot,    \
-                       ctx->small_cut, defer_cap, counters, defer, pvp, gid, gt);                     \
-    XT_HIP(ctx, hipGetLastError());                                                                   \
-    hipLaunchKernelGGL((k_count_wave<P, B>), dim3(wave_grid), dim3(256), 0, st, ooff, ot, defer_cap, \
-                       counters, (const int64_t*)defer, pvp, gid, gt);                                \
-    XT_HIP(ctx, hipGetLastError());                                                                   \
-  } while (0)
+  const int cut = ctx->small_cut;
+#define XT_LIGHT(P, B)                                                                             \
+  hipLaunchKernelGGL((k_count_light<P, B>), dim3((unsigned)nchunks), dim3(kBlock), 0, st, nv, ooff, \
+                     ot, cut, counters, s.itcnt.as<uint32_t>(), pvp, gid, gt)
   if (pv && batch)
-    XT_COUNT_LAUNCH(true, true);
+    XT_LIGHT(true, true);
   else if (pv)
-    XT_COUNT_LAUNCH(true, false);
+    XT_LIGHT(true, false);
   else if (batch)
-    XT_COUNT_LAUNCH(false, true);
+    XT_LIGHT(false, true);
   else
-    XT_COUNT_LAUNCH(false, false);
-#undef XT_COUNT_LAUNCH
+    XT_LIGHT(false, false);
+#undef XT_LIGHT
+  XT_HIP(ctx, hipGetLastError());
+  if (evs) XT_HIP(ctx, hipEventRecord(evs[0], st));
+  XT_TRY(scan_u32_to_i64(ctx, s, s.itcnt.as<uint32_t>(), nv, s.itoff.as<int64_t>(), st));
+  hipLaunchKernelGGL(k_fill_items, dim3((unsigned)nchunks), dim3(kBlock), 0, st, nv, ooff, ot, cut,
+                     (const int64_t*)s.itoff.as<int64_t>(), s.itcnt.as<uint32_t>(),
+                     s.items.as<int32_t>());
+  XT_HIP(ctx, hipGetLastError());
+  int64_t g = (nv + 255) / 256;
+  if (g > 65536) g = 65536;
+  hipLaunchKernelGGL(k_tile_counts, dim3((unsigned)g), dim3(256), 0, st, nv,
+                     (const int64_t*)s.itoff.as<int64_t>(), s.deg.as<uint32_t>());
+  XT_HIP(ctx, hipGetLastError());
+  XT_TRY(scan_u32_to_i64(ctx, s, s.deg.as<uint32_t>(), nv, s.tileoff.as<int64_t>(), st));
+  if (evs) XT_HIP(ctx, hipEventRecord(evs[1], st));
+  const unsigned groups = (unsigned)(ctx->num_cus * kGroupBlocksPerCU);
+#define XT_GROUPS(P, B)                                                                             \
+  hipLaunchKernelGGL((k_count_groups<P, B>), dim3(groups), dim3(kBlock), 0, st, nv, ooff, ot,       \
+                     (const int64_t*)s.itoff.as<int64_t>(), (const int32_t*)s.items.as<int32_t>(), \
+                     (const int64_t*)s.tileoff.as<int64_t>(), counters, pvp, gid, gt)
+  if (pv && batch)
+    XT_GROUPS(true, true);
+  else if (pv)
+    XT_GROUPS(true, false);
+  else if (batch)
+    XT_GROUPS(false, true);
+  else
+    XT_GROUPS(false, false);
+#undef XT_GROUPS
+  XT_HIP(ctx, hipGetLastError());
   return XT_OK;
 }
 
@@ -284,9 +420,10 @@ int alg_stats(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne_bound, hipStream_t
   const int64_t nchunks = (ne_bound + kChunk - 1) / kChunk;
   uint64_t* counters = s.counters.as<uint64_t>();
   XT_TRY(fill_u64(ctx, counters + kStatMin, 1, 0, st));
+  XT_TRY(fill_u64(ctx, counters + kStatMinHeavy, 2, 0, st));
   hipLaunchKernelGGL(k_alg_stats, dim3((unsigned)nchunks), dim3(kBlock), 0, st, nv,
                      (const int64_t*)s.ooff.as<int64_t>(), (const int32_t*)s.otails.as<int32_t>(),
-                     counters);
+                     ctx->small_cut, counters);
   XT_HIP(ctx, hipGetLastError());
   return XT_OK;
 }
@@ -301,7 +438,10 @@ int reserve(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne, bool per_vertex) {
   XT_TRY(ensure(ctx, s.ooff, (size_t)(nvv + 1) * sizeof(int64_t)));
   XT_TRY(ensure(ctx, s.otails, (size_t)nee * sizeof(int32_t)));
   XT_TRY(ensure(ctx, s.counters, kNumSlots * sizeof(uint64_t)));
-  XT_TRY(ensure(ctx, s.defer, (size_t)defer_capacity(nee) * 2 * sizeof(int64_t)));
+  XT_TRY(ensure(ctx, s.itcnt, (size_t)nvv * sizeof(uint32_t)));
+  XT_TRY(ensure(ctx, s.itoff, (size_t)(nvv + 1) * sizeof(int64_t)));
+  XT_TRY(ensure(ctx, s.items, (size_t)nee * sizeof(int32_t)));
+  XT_TRY(ensure(ctx, s.tileoff, (size_t)(nvv + 1) * sizeof(int64_t)));
   const size_t st = scan_tmp_bytes(nvv) > scan_tmp_bytes(nchunks) ? scan_tmp_bytes(nvv) : scan_tmp_bytes(nchunks);
   XT_TRY(ensure(ctx, s.scan_tmp, st));
   if (per_vertex) XT_TRY(ensure(ctx, s.pv, (size_t)nvv * sizeof(uint64_t)));
@@ -310,13 +450,15 @@ int reserve(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne, bool per_vertex) {
 
 void release(Scratch& s) {
   DevBuf* bufs[] = {&s.deg, &s.dplus, &s.tailcnt, &s.qscan, &s.ooff, &s.otails, &s.counters,
-                    &s.defer, &s.pv, &s.scan_tmp, &s.in_off, &s.in_tails};
+                    &s.itcnt, &s.itoff, &s.items, &s.tileoff, &s.pv, &s.scan_tmp, &s.in_off,
+                    &s.in_tails};
   for (DevBuf* b : bufs) {
     if (b->p) (void)hipFree(b->p);
     b->p = nullptr;
     b->bytes = 0;
   }
 }
+
 // ---------------------------------------------------------------------------
 // Reference lower-FFI shapes.
 

@@ -21,15 +21,15 @@ constexpr int kOffCap = 2048;
 
 // Counter slots in Scratch::counters (uint64).
 enum Slot : int {
-  kTotal = 0,      // global triangle count
-  kDeferred = 1,   // edges appended to the wave-kernel list
-  kBlockEdges = 2, // edges appended to the block-kernel list
-  kBadIds = 3,     // neighbour ids outside [0, nv)
-  kBadRows = 4,    // rows not strictly ascending
-  kAsym = 5,       // entries without their reverse entry
-  kBadOff = 6,     // offsets not monotone / wrong ends
-  kStatMin = 7,    // sum of min(d+(u), d+(v)) (statistics kernel)
-  kTileQueue = 8,  // next tile of the persistent batch intersection kernel
+  kTotal = 0,          // global triangle count
+  kBadIds = 3,         // neighbour ids outside [0, nv)
+  kBadRows = 4,        // rows not strictly ascending
+  kAsym = 5,           // entries without their reverse entry
+  kBadOff = 6,         // offsets not monotone / wrong ends
+  kStatMin = 7,        // sum of min(d+(u), d+(v)) over oriented edges (statistics kernel)
+  kTileQueue = 8,      // next tile of the persistent group kernel
+  kStatMinHeavy = 9,   // same sum over heavy edges only
+  kStatHeavyEdges = 10,// number of heavy edges
   kNumSlots = 16
 };
 
@@ -45,17 +45,20 @@ struct DevBuf {
 };
 
 // Phase events recorded on the context stream.
-enum Ev : int { kEvStart = 0, kEvOriented, kEvCounted, kEvEnd, kNumEv };
+enum Ev : int { kEvStart = 0, kEvOriented, kEvLight, kEvItems, kEvCounted, kNumEv };
 
 struct Scratch {
-  DevBuf deg;       // uint32[nv]          degree of each vertex
+  DevBuf deg;       // uint32[nv]          degree of each vertex; later tiles per group owner
   DevBuf dplus;     // uint32[nv]          out-degree of each vertex
   DevBuf tailcnt;   // uint32[nchunks]     flagged entries of each chunk's last vertex
   DevBuf qscan;     // int64[nchunks+1]    exclusive scan of tailcnt
   DevBuf ooff;      // int64[nv+1]         oriented offsets
-  DevBuf otails;    // int32[ne/2]         oriented out-lists, ascending ids
-  DevBuf counters;  // uint64[16]          total, deferred count, block count, flags
-  DevBuf defer;     // int64[..]           oriented edges deferred to the wave kernel
+  DevBuf otails;    // int32[ne]           oriented out-lists, ascending ids (|E+| <= ne)
+  DevBuf counters;  // uint64[16]          total, tile queue, input-error flags, statistics
+  DevBuf itcnt;     // uint32[nv]          heavy edges owned by each vertex
+  DevBuf itoff;     // int64[nv+1]         exclusive scan of itcnt
+  DevBuf items;     // int32[ne]           other endpoints of each owner's heavy edges
+  DevBuf tileoff;   // int64[nv+1]         exclusive scan of tiles per owner
   DevBuf pv;        // uint64[nv]          per-vertex counts
   DevBuf scan_tmp;  // int64[..]           block sums of the scans
   DevBuf in_off;    // int64[nv+1]         staged input offsets (xt_count_csr)
@@ -73,9 +76,8 @@ struct xt_ctx {
   xt::Scratch s;
   hipEvent_t ev[xt::kNumEv] = {};
   xt_timing timing = {};
-  int small_cut = 64;     // |shorter list| above which an edge goes to the wave kernel
-  int block_cut = 4096;   // |shorter list| above which an edge goes to the block kernel
-  int persist_blocks = 8; // persistent workgroups per CU (batch tile queue)
+  int small_cut = 32;     // light edges: both out-lists at most this long
+  int block_cut = 4096;   // reserved tuning knob
 };
 
 struct xt_graph {
@@ -107,8 +109,10 @@ int orient(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne, const int64_t* off, 
 // count to ctx counters[0]; per-vertex counts into pv when pv != nullptr.
 // Batch mode: gid (graph of each vertex) and gtotal (per-graph totals) non-null
 // and the intersection kernel drains a device tile queue with a fixed grid.
+// evs (nullable): two events recorded after the light-edge kernel and after
+// the item lists are built.
 int count_oriented(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne_bound, uint64_t* pv,
-                   const int32_t* gid, uint64_t* gtotal, hipStream_t st);
+                   const int32_t* gid, uint64_t* gtotal, hipStream_t st, hipEvent_t* evs);
 int alg_stats(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne_bound, hipStream_t st);
 // Pre-sizes every scratch buffer for graphs up to (nv, ne).
 int reserve(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne, bool per_vertex);
