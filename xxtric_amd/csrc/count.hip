@@ -81,6 +81,20 @@ __device__ __forceinline__ int64_t tile_off(const TileRows& r, const int64_t* s_
   return r.in_lds ? s_off[u - r.ulo] : ooff[u];
 }
 
+// Ids of a thread's run of kPerThread oriented edges (two 16-byte loads when whole).
+__device__ __forceinline__ void load_ids(const int32_t* __restrict__ ot, int64_t e0, int64_t ee,
+                                         int32_t (&v)[kPerThread]) {
+  if (e0 + kPerThread <= ee) {
+    const int4* p = reinterpret_cast<const int4*>(ot + e0);
+    const int4 a = p[0], b = p[1];
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+    v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+  } else {
+#pragma unroll
+    for (int j = 0; j < kPerThread; j++) v[j] = e0 + j < ee ? ot[e0 + j] : 0;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Light edges counted inline; heavy edges counted per owning endpoint.
 template <bool PV, bool BATCH>
@@ -101,18 +115,34 @@ __global__ __launch_bounds__(kBlock) void k_count_light(
   const TileRows r = tile_rows(ooff, nv, eb, ee, s_off, s_lohi);
   const int32_t g0 = BATCH ? gid[r.ulo] : 0;
 
+  // Eight contiguous edges per thread: one row search, then a walk.
   uint64_t hits_all = 0;
+  const int64_t e0 = eb + (int64_t)threadIdx.x * kPerThread;
+  int32_t ids[kPerThread];
+  if (e0 < ee) load_ids(ot, e0, ee, ids);
+  int64_t u = e0 < ee ? tile_row_of(r, s_off, ooff, e0) : 0;
+  int64_t a0 = e0 < ee ? tile_off(r, s_off, ooff, u) : 0;
+  int64_t a1 = e0 < ee ? tile_off(r, s_off, ooff, u + 1) : 0;
+  uint32_t own_run = 0;  // heavy edges of row u that u owns (source-owned)
 #pragma unroll 1
   for (int j = 0; j < kPerThread; j++) {
-    const int64_t e = eb + (int64_t)j * kBlock + threadIdx.x;
+    const int64_t e = e0 + j;
     if (e >= ee) break;
-    const int64_t u = tile_row_of(r, s_off, ooff, e);
-    const int64_t a0 = tile_off(r, s_off, ooff, u), a1 = tile_off(r, s_off, ooff, u + 1);
-    const int64_t v = ot[e];
+    while (e >= a1) {
+      if (own_run) atomicAdd(&itcnt[u], own_run);
+      own_run = 0;
+      ++u;
+      a0 = a1;
+      a1 = tile_off(r, s_off, ooff, u + 1);
+    }
+    const int64_t v = ids[j];
     const int64_t b0 = ooff[v], b1 = ooff[v + 1];
     const int64_t du = a1 - a0, dv = b1 - b0;
-    if (du > light_cut || dv > light_cut) {
-      atomicAdd(&itcnt[du >= dv ? u : v], 1u);  // heavy: item of the longer list's owner
+    if (du > light_cut || dv > light_cut) {  // heavy: item of the longer list's owner
+      if (du >= dv)
+        own_run++;
+      else
+        atomicAdd(&itcnt[v], 1u);
       continue;
     }
     const EdgeLists L = edge_lists(a0, a1, b0, b1);
@@ -152,6 +182,7 @@ __global__ __launch_bounds__(kBlock) void k_count_light(
       hits_all += hits;
     }
   }
+  if (own_run) atomicAdd(&itcnt[u], own_run);
   if constexpr (BATCH) {
     __syncthreads();
     if (threadIdx.x < kGraphSlots && s_g[threadIdx.x])
@@ -176,20 +207,47 @@ __global__ __launch_bounds__(kBlock) void k_fill_items(int64_t nv, const int64_t
   if (eb >= nep) return;
   const int64_t ee = min(nep, eb + (int64_t)kChunk);
   const TileRows r = tile_rows(ooff, nv, eb, ee, s_off, s_lohi);
-#pragma unroll 1
+  const int64_t e0 = eb + (int64_t)threadIdx.x * kPerThread;
+  if (e0 >= ee) return;
+  int32_t ids[kPerThread];
+  load_ids(ot, e0, ee, ids);
+  int64_t u = tile_row_of(r, s_off, ooff, e0);
+  int64_t a0 = tile_off(r, s_off, ooff, u), a1 = tile_off(r, s_off, ooff, u + 1);
+  uint32_t own = 0;  // positions of this row's source-owned heavy edges
+  // One atomic reserves a contiguous range for a run of source-owned items.
+  auto flush = [&]() {
+    if (!own) return;
+    const uint32_t cnt = (uint32_t)__popc(own);
+    const int64_t base = itoff[u] + (int64_t)(atomicSub(&itcnt[u], cnt) - cnt);
+    uint32_t rank = 0;
+#pragma unroll
+    for (int jj = 0; jj < kPerThread; jj++)
+      if ((own >> jj) & 1u) items[base + rank++] = ids[jj];
+    own = 0;
+  };
+#pragma unroll
   for (int j = 0; j < kPerThread; j++) {
-    const int64_t e = eb + (int64_t)j * kBlock + threadIdx.x;
-    if (e >= ee) break;
-    const int64_t u = tile_row_of(r, s_off, ooff, e);
-    const int64_t du = tile_off(r, s_off, ooff, u + 1) - tile_off(r, s_off, ooff, u);
-    const int64_t v = ot[e];
-    const int64_t dv = ooff[v + 1] - ooff[v];
-    if (du > light_cut || dv > light_cut) {
-      const int64_t x = du >= dv ? u : v, y = du >= dv ? v : u;
-      const uint32_t k = atomicSub(&itcnt[x], 1u) - 1u;
-      items[itoff[x] + k] = (int32_t)y;
+    const int64_t e = e0 + j;
+    if (e < ee) {
+      while (e >= a1) {
+        flush();
+        ++u;
+        a0 = a1;
+        a1 = tile_off(r, s_off, ooff, u + 1);
+      }
+      const int64_t v = ids[j];
+      const int64_t du = a1 - a0, dv = ooff[v + 1] - ooff[v];
+      if (du > light_cut || dv > light_cut) {
+        if (du >= dv) {
+          own |= 1u << j;
+        } else {
+          const uint32_t k = atomicSub(&itcnt[v], 1u) - 1u;
+          items[itoff[v] + k] = (int32_t)u;
+        }
+      }
     }
   }
+  flush();
 }
 
 __global__ __launch_bounds__(256) void k_tile_counts(int64_t nv, const int64_t* __restrict__ itoff,
@@ -211,18 +269,20 @@ __global__ __launch_bounds__(kBlock) void k_count_groups(
     unsigned long long* __restrict__ pv, const int32_t* __restrict__ gid,
     unsigned long long* __restrict__ gtotal) {
   __shared__ int32_t s_hash[kHashSlots];
-  __shared__ int64_t s_start[kItemsPerTile + 1];
   __shared__ int64_t s_base[kItemsPerTile];
+  __shared__ int32_t s_len[kItemsPerTile];
   __shared__ int32_t s_y[kItemsPerTile];
-  __shared__ int64_t s_scan[kBlock / kWave];
   __shared__ uint64_t s_red[kBlock / kWave];
   __shared__ int64_t s_tile, s_x;
+  __shared__ int s_next;
+  const int lane = threadIdx.x & (kWave - 1);
   const int64_t ntiles = tileoff[nv];
   for (;;) {
     if (threadIdx.x == 0) {
       const int64_t t = (int64_t)atomicAdd((unsigned long long*)&counters[kTileQueue], 1ull);
       s_tile = t;
       s_x = t < ntiles ? row_of(tileoff, 0, nv - 1, t) : 0;
+      s_next = 0;
     }
     __syncthreads();
     const int64_t tile = s_tile;
@@ -233,28 +293,23 @@ __global__ __launch_bounds__(kBlock) void k_count_groups(
     const int64_t i1 = min(i0 + (int64_t)kItemsPerTile, itoff[x + 1]);
     const int nitems = (int)(i1 - i0);
     const bool hashed = dx <= kHashSlots / 2;
+    // load factor <= 1/4 when the table fits (most probes miss: short probe runs)
     int bits = 6;
-    while ((1ll << bits) < 2 * dx) bits++;
+    while ((1ll << bits) < 4 * dx && (1 << bits) < kHashSlots) bits++;
     const uint32_t mask = (1u << bits) - 1u;
     if (hashed) {
       for (int k = threadIdx.x; k <= (int)mask; k += kBlock) s_hash[k] = -1;
     }
-    // items of this tile: lengths and bases of the streamed lists
-    int64_t len = 0, base = 0;
-    int32_t y = 0;
+    // items of this tile: bases and lengths of the streamed lists
     if ((int)threadIdx.x < nitems) {
-      y = items[i0 + threadIdx.x];
-      base = ooff[y];
-      len = ooff[y + 1] - base;
+      const int32_t y = items[i0 + threadIdx.x];
+      const int64_t b = ooff[y];
+      s_y[threadIdx.x] = y;
+      s_base[threadIdx.x] = b;
+      s_len[threadIdx.x] = (int32_t)(ooff[y + 1] - b);
     }
-    int64_t W;
-    const int64_t start = block_excl_scan_i64<kBlock>(len, s_scan, &W);
-    s_start[threadIdx.x] = start;
-    s_base[threadIdx.x] = base;
-    s_y[threadIdx.x] = y;
-    if (threadIdx.x == 0) s_start[nitems] = W;
-    __syncthreads();
     if (hashed) {
+      __syncthreads();  // table cleared before inserts
       for (int64_t k = threadIdx.x; k < dx; k += kBlock) {
         const int32_t key = ot[a0 + k];
         uint32_t h = hash_slot(key, bits);
@@ -262,20 +317,13 @@ __global__ __launch_bounds__(kBlock) void k_count_groups(
       }
     }
     __syncthreads();
+    // Each wave takes whole items from an LDS counter and streams the item's
+    // list with its 64 lanes: 16-byte loads (four ids per lane), four in flight
+    // per lane, plus scalar loads for the unaligned head and the tail.
     uint64_t hits = 0, at_x = 0;
-    int lo_item = 0;
-    for (int64_t j = threadIdx.x; j < W; j += kBlock) {
-      // item of element j: last k with s_start[k] <= j (searched from the previous one)
-      int lo = lo_item, hi = nitems - 1;
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (s_start[mid] <= j)
-          lo = mid;
-        else
-          hi = mid - 1;
-      }
-      lo_item = lo;
-      const int32_t w = ot[s_base[lo] + (j - s_start[lo])];
+    int64_t yy = 0;
+    auto probe = [&](int32_t w) {
+      if (w < 0) return;
       bool found;
       if (hashed) {
         uint32_t h = hash_slot(w, bits);
@@ -289,7 +337,6 @@ __global__ __launch_bounds__(kBlock) void k_count_groups(
       if (found) {
         hits++;
         if constexpr (PV) {
-          const int64_t yy = s_y[lo];
           int64_t m = x < yy ? x : yy;
           if ((int64_t)w < m) m = w;
           if (m == x)
@@ -298,7 +345,38 @@ __global__ __launch_bounds__(kBlock) void k_count_groups(
             atomicAdd(&pv[m], 1ull);
         }
       }
+    };
+    for (;;) {
+      int k = 0;
+      if (lane == 0) k = atomicAdd(&s_next, 1);
+      k = __shfl(k, 0, kWave);
+      if (k >= nitems) break;  // uniform per wave
+      const int64_t base = s_base[k];
+      const int len = s_len[k];
+      yy = s_y[k];
+      const int32_t* __restrict__ lst = ot + base;
+      int head = (int)((4 - (base & 3)) & 3);  // ids before the first 16-byte boundary
+      if (head > len) head = len;
+      if (lane < head) probe(lst[lane]);
+      const int4* __restrict__ body = reinterpret_cast<const int4*>(lst + head);
+      const int nbody = (len - head) >> 2;
+      for (int i = lane; i < nbody; i += 4 * kWave) {
+        int4 v[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+          v[q] = i + q * kWave < nbody ? body[i + q * kWave] : make_int4(-1, -1, -1, -1);
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          probe(v[q].x);
+          probe(v[q].y);
+          probe(v[q].z);
+          probe(v[q].w);
+        }
+      }
+      const int tail0 = head + 4 * nbody;
+      if (lane < len - tail0) probe(lst[tail0 + lane]);
     }
+    __syncthreads();
     const uint64_t bh = block_sum_u64<kBlock>(hits, s_red);
     if constexpr (PV) {
       __syncthreads();
@@ -438,6 +516,7 @@ int reserve(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne, bool per_vertex) {
   XT_TRY(ensure(ctx, s.ooff, (size_t)(nvv + 1) * sizeof(int64_t)));
   XT_TRY(ensure(ctx, s.otails, (size_t)nee * sizeof(int32_t)));
   XT_TRY(ensure(ctx, s.counters, kNumSlots * sizeof(uint64_t)));
+  XT_TRY(ensure(ctx, s.oflags, (size_t)(nchunks * kBlock)));
   XT_TRY(ensure(ctx, s.itcnt, (size_t)nvv * sizeof(uint32_t)));
   XT_TRY(ensure(ctx, s.itoff, (size_t)(nvv + 1) * sizeof(int64_t)));
   XT_TRY(ensure(ctx, s.items, (size_t)nee * sizeof(int32_t)));
@@ -450,7 +529,7 @@ int reserve(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne, bool per_vertex) {
 
 void release(Scratch& s) {
   DevBuf* bufs[] = {&s.deg, &s.dplus, &s.tailcnt, &s.qscan, &s.ooff, &s.otails, &s.counters,
-                    &s.itcnt, &s.itoff, &s.items, &s.tileoff, &s.pv, &s.scan_tmp, &s.in_off,
+                    &s.oflags, &s.itcnt, &s.itoff, &s.items, &s.tileoff, &s.pv, &s.scan_tmp, &s.in_off,
                     &s.in_tails};
   for (DevBuf* b : bufs) {
     if (b->p) (void)hipFree(b->p);

@@ -26,11 +26,16 @@ namespace xt {
 
 
 // ---------------------------------------------------------------------------
+// Degrees saturated at 65535: the key (min(deg, 65535), id) is still a strict
+// total order (ties broken by id), so the orientation stays a DAG and the
+// count is unchanged; 16-bit keys halve the footprint of the neighbour gathers.
 __global__ __launch_bounds__(256) void k_degree(int64_t nv, const int64_t* __restrict__ off,
-                                                uint32_t* __restrict__ deg) {
+                                                uint16_t* __restrict__ deg) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < nv; u += stride)
-    deg[u] = (uint32_t)(off[u + 1] - off[u]);
+  for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < nv; u += stride) {
+    const int64_t d = off[u + 1] - off[u];
+    deg[u] = (uint16_t)(d < 65535 ? d : 65535);
+  }
 }
 
 template <typename T>
@@ -68,7 +73,7 @@ __device__ __forceinline__ void load_run(const T* __restrict__ tails, int64_t e0
 }
 
 template <int MODE>
-__device__ __forceinline__ uint64_t orient_key(const uint32_t* __restrict__ deg, int64_t x) {
+__device__ __forceinline__ uint64_t orient_key(const uint16_t* __restrict__ deg, int64_t x) {
   if constexpr (MODE == kOrientDegree)
     return ((uint64_t)deg[x] << 32) | (uint64_t)x;
   else
@@ -115,8 +120,9 @@ __device__ __forceinline__ int64_t chunk_off(const ChunkRows& r, const int64_t* 
 template <typename T, int MODE>
 __global__ __launch_bounds__(kBlock) void k_orient_count(
     int64_t nv, int64_t ne, const int64_t* __restrict__ off, const T* __restrict__ tails,
-    const uint32_t* __restrict__ deg, uint32_t* __restrict__ dplus,
-    uint32_t* __restrict__ tailcnt, uint64_t* __restrict__ counters) {
+    const uint16_t* __restrict__ deg, uint32_t* __restrict__ dplus,
+    uint32_t* __restrict__ tailcnt, uint64_t* __restrict__ counters,
+    uint8_t* __restrict__ oflags) {
   __shared__ int64_t s_off[kOffCap + 1];
   __shared__ uint32_t s_cnt[kOffCap];
   __shared__ int64_t s_lohi[2];
@@ -138,7 +144,7 @@ __global__ __launch_bounds__(kBlock) void k_orient_count(
     int64_t u = chunk_row_of(r, s_off, off, e0);
     int64_t nxt = chunk_off(r, s_off, off, u + 1);
     uint64_t ku = orient_key<MODE>(deg, u);
-    uint32_t run = 0;
+    uint32_t run = 0, fl = 0;
     auto flush = [&](int64_t uu, uint32_t cnt) {
       if (cnt == 0) return;
       if (r.in_lds)
@@ -162,12 +168,14 @@ __global__ __launch_bounds__(kBlock) void k_orient_count(
         const int64_t v = vals[j];
         if (v < 0 || v >= nv) {
           bad++;
-        } else {
-          run += orient_key<MODE>(deg, v) > ku ? 1u : 0u;
+        } else if (orient_key<MODE>(deg, v) > ku) {
+          run++;
+          fl |= 1u << j;
         }
       }
     }
     flush(u, run);
+    oflags[e0 / kPerThread] = (uint8_t)fl;  // kept entries of this run, for the fill pass
   }
   if (bad) atomicAdd((unsigned long long*)&counters[kBadIds], (unsigned long long)bad);
   __syncthreads();
@@ -182,10 +190,10 @@ __global__ __launch_bounds__(kBlock) void k_orient_count(
   }
 }
 
-template <typename T, int MODE>
+template <typename T>
 __global__ __launch_bounds__(kBlock) void k_orient_fill(
     int64_t nv, int64_t ne, const int64_t* __restrict__ off, const T* __restrict__ tails,
-    const uint32_t* __restrict__ deg, const int64_t* __restrict__ ooff,
+    const uint8_t* __restrict__ oflags, const int64_t* __restrict__ ooff,
     const int64_t* __restrict__ q, int32_t* __restrict__ otails) {
   __shared__ int64_t s_off[kOffCap + 1];
   __shared__ uint16_t s_x[kChunk];
@@ -201,10 +209,10 @@ __global__ __launch_bounds__(kBlock) void k_orient_fill(
   int64_t rows[kPerThread];
   uint32_t fl = 0;
   if (e0 < ee) {
-    load_run<T>(tails, e0, ee, vals);
+    fl = oflags[e0 / kPerThread];
+    if (fl) load_run<T>(tails, e0, ee, vals);
     int64_t u = chunk_row_of(r, s_off, off, e0);
     int64_t nxt = chunk_off(r, s_off, off, u + 1);
-    uint64_t ku = orient_key<MODE>(deg, u);
 #pragma unroll
     for (int j = 0; j < kPerThread; j++) {
       const int64_t e = e0 + j;
@@ -213,11 +221,8 @@ __global__ __launch_bounds__(kBlock) void k_orient_fill(
         while (e >= nxt) {
           ++u;
           nxt = chunk_off(r, s_off, off, u + 1);
-          ku = orient_key<MODE>(deg, u);
         }
         rows[j] = u;
-        const int64_t v = vals[j];
-        if (v >= 0 && v < nv && orient_key<MODE>(deg, v) > ku) fl |= 1u << j;
       }
     }
   }
@@ -381,36 +386,31 @@ int orient(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne, const int64_t* off, 
   const int64_t nchunks = (ne + kChunk - 1) / kChunk;
   XT_TRY(ensure(ctx, s.tailcnt, (size_t)nchunks * sizeof(uint32_t)));
   XT_TRY(ensure(ctx, s.qscan, (size_t)(nchunks + 1) * sizeof(int64_t)));
-  uint32_t* deg = nullptr;
+  XT_TRY(ensure(ctx, s.oflags, (size_t)(nchunks * kBlock)));
+  uint16_t* deg = nullptr;
   if (mode == kOrientDegree) {
     XT_TRY(ensure(ctx, s.deg, (size_t)nv * sizeof(uint32_t)));
-    deg = s.deg.as<uint32_t>();
+    deg = s.deg.as<uint16_t>();
     hipLaunchKernelGGL(k_degree, dim3(grid_for(nv, 256)), dim3(256), 0, st, nv, off, deg);
     XT_HIP(ctx, hipGetLastError());
   }
   XT_TRY(fill_u32(ctx, s.dplus.as<uint32_t>(), nv, 0u, st));
   uint64_t* counters = s.counters.as<uint64_t>();
+  uint8_t* oflags = s.oflags.as<uint8_t>();
   if (mode == kOrientDegree)
     hipLaunchKernelGGL((k_orient_count<T, kOrientDegree>), dim3((unsigned)nchunks), dim3(kBlock), 0,
-                       st, nv, ne, off, tails, (const uint32_t*)deg, s.dplus.as<uint32_t>(),
-                       s.tailcnt.as<uint32_t>(), counters);
+                       st, nv, ne, off, tails, (const uint16_t*)deg, s.dplus.as<uint32_t>(),
+                       s.tailcnt.as<uint32_t>(), counters, oflags);
   else
     hipLaunchKernelGGL((k_orient_count<T, kOrientId>), dim3((unsigned)nchunks), dim3(kBlock), 0, st,
-                       nv, ne, off, tails, (const uint32_t*)deg, s.dplus.as<uint32_t>(),
-                       s.tailcnt.as<uint32_t>(), counters);
+                       nv, ne, off, tails, (const uint16_t*)deg, s.dplus.as<uint32_t>(),
+                       s.tailcnt.as<uint32_t>(), counters, oflags);
   XT_HIP(ctx, hipGetLastError());
   XT_TRY(scan_u32_to_i64(ctx, s, s.dplus.as<uint32_t>(), nv, s.ooff.as<int64_t>(), st));
   XT_TRY(scan_u32_to_i64(ctx, s, s.tailcnt.as<uint32_t>(), nchunks, s.qscan.as<int64_t>(), st));
-  if (mode == kOrientDegree)
-    hipLaunchKernelGGL((k_orient_fill<T, kOrientDegree>), dim3((unsigned)nchunks), dim3(kBlock), 0,
-                       st, nv, ne, off, tails, (const uint32_t*)deg,
-                       (const int64_t*)s.ooff.as<int64_t>(), (const int64_t*)s.qscan.as<int64_t>(),
-                       s.otails.as<int32_t>());
-  else
-    hipLaunchKernelGGL((k_orient_fill<T, kOrientId>), dim3((unsigned)nchunks), dim3(kBlock), 0, st,
-                       nv, ne, off, tails, (const uint32_t*)deg,
-                       (const int64_t*)s.ooff.as<int64_t>(), (const int64_t*)s.qscan.as<int64_t>(),
-                       s.otails.as<int32_t>());
+  hipLaunchKernelGGL((k_orient_fill<T>), dim3((unsigned)nchunks), dim3(kBlock), 0, st, nv, ne, off,
+                     tails, (const uint8_t*)oflags, (const int64_t*)s.ooff.as<int64_t>(),
+                     (const int64_t*)s.qscan.as<int64_t>(), s.otails.as<int32_t>());
   XT_HIP(ctx, hipGetLastError());
   return XT_OK;
 }

@@ -55,6 +55,7 @@ struct Scratch {
   DevBuf ooff;      // int64[nv+1]         oriented offsets
   DevBuf otails;    // int32[ne]           oriented out-lists, ascending ids (|E+| <= ne)
   DevBuf counters;  // uint64[16]          total, tile queue, input-error flags, statistics
+  DevBuf oflags;    // uint8[ne/8]         kept-entry mask per 8 CSR entries (orientation)
   DevBuf itcnt;     // uint32[nv]          heavy edges owned by each vertex
   DevBuf itoff;     // int64[nv+1]         exclusive scan of itcnt
   DevBuf items;     // int32[ne]           other endpoints of each owner's heavy edges
