@@ -101,7 +101,7 @@ def cpu_baseline(g, seconds):
 def batch_workload(ctx, n_graphs=4096, runs=5):
     import xxtric_amd
     from xxtric_amd import XT_BATCH_HIPGRAPH, XT_BATCH_PER_GRAPH, XT_BATCH_PERSISTENT
-    b = xxtric_amd.Batch(ctx, 40_000_000, 400_000_000, n_graphs)
+    b = xxtric_amd.Batch(ctx, 40_000_000, 700_000_000, n_graphs)
     t0 = time.time()
     b.add_generated(2026, 0, n_graphs)
     add_s = time.time() - t0
@@ -194,6 +194,10 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=20000.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the 1 GiB copy kernel used to scale PMC byte counters")
+    ap.add_argument("--serve-sweep", action="store_true",
+                    help="only run the serving workload at 1k..50k graphs/s and print one line per rate")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="only run the batched 4096-graph workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the single-device path only "
@@ -203,6 +207,14 @@ def main():
     ctx = xxtric_amd.Context(0)
     if args.copy_calibration:
         copy_calibration(ctx)
+        return
+    if args.serve_sweep:
+        for rate in (1000, 2000, 5000, 10000, 20000, 50000):
+            print(json.dumps(serve_workload(ctx, float(rate), n_req=max(2000, min(20000, rate)))),
+                  flush=True)
+        return
+    if args.batch_only:
+        print(json.dumps(batch_workload(ctx)), flush=True)
         return
     g, r = rmat_workload(ctx, args.scale, args.steps, args.warmup)
     value = r["m"] / (r["ms_per_step"] * 1e-3)

@@ -65,9 +65,12 @@ int xt_ctx_create(int device, xt_ctx** out);
 int xt_ctx_destroy(xt_ctx* ctx);
 const char* xt_last_error(const xt_ctx* ctx);
 int xt_last_timing(const xt_ctx* ctx, xt_timing* out);
-/* Tuning knobs (normally set by the autotuner): small-edge cutoff and chunk
- * sizes; 0 keeps the current value. */
-int xt_ctx_tune(xt_ctx* ctx, int small_cut, int block_cut, int reserved);
+/* Tuning knobs (set by xxtric_amd.autotune from measurements; 0 keeps the
+ * current value): light_cut -- edges whose two out-lists are both at most this
+ * long are intersected one per thread, the rest by the group kernel (default
+ * 32); group_blocks -- persistent group-kernel workgroups per CU (1..4,
+ * default 4); reserved must be 0. */
+int xt_ctx_tune(xt_ctx* ctx, int light_cut, int group_blocks, int reserved);
 
 /* ---- device memory plumbing ------------------------------------------------
  * The CudaMalloc / CudaMemcpyHtoD / CudaMemcpyDtoH / CudaFree /

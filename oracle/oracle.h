@@ -56,6 +56,12 @@ uint64_t or_count_reference32(int64_t nv, const int64_t* off, const int32_t* tai
                               uint64_t* per_vertex, int nthreads, int64_t vstart,
                               int64_t vstride);
 
+/* or_count_reference32 restricted to a vertex list: per_vertex[k] = tcount of
+ * vertices[k] (xxtric.hpp:160-206). */
+void or_count_reference32_list(int64_t nv, const int64_t* off, const int32_t* tails,
+                               const int64_t* vertices, int64_t nlist, uint64_t* per_vertex,
+                               int nthreads);
+
 /* Time-bounded sample of or_count_reference32 for CPU baselines (vertices in
  * id order, abandoned at the deadline); returns vertices completed. */
 int64_t or_count_reference32_budget(int64_t nv, const int64_t* off, const int32_t* tails,

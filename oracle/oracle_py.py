@@ -39,6 +39,7 @@ def lib():
             "or_count_reference32": (_u64, [_i64, _p, _p, _p, _int, _i64, _i64]),
             "or_count_degree_ordered": (_u64, [_i64, _p, _p, _p, _int]),
             "or_count_reference32_budget": (_i64, [_i64, _p, _p, _int, _dbl, _p, _p]),
+            "or_count_reference32_list": (None, [_i64, _p, _p, _p, _i64, _p, _int]),
             "or_vite_write": (_int, [ctypes.c_char_p, _i64, _p, _p, _int]),
             "or_vite_read": (_int, [ctypes.c_char_p, _int, _p, _p, _p, _p]),
             "or_mix64": (_u64, [_u64]),
@@ -102,6 +103,18 @@ def count_reference(off, tails, per_vertex=False, nthreads=None, vstart=0, vstri
         tails = np.ascontiguousarray(tails, np.int64)
         tot = lib().or_count_reference(nv, _ptr(off), _ptr(tails), _ptr(pv), nt, vstart, vstride)
     return (int(tot), pv) if per_vertex else int(tot)
+
+
+def count_reference_list(off, tails, vertices, nthreads=None):
+    """Reference per-vertex counts (smallest-id credit) of the listed vertices only."""
+    off = np.ascontiguousarray(off, np.int64)
+    tails = np.ascontiguousarray(tails, np.int32)
+    vertices = np.ascontiguousarray(vertices, np.int64)
+    pv = np.zeros(len(vertices), np.uint64)
+    nt = nthreads or nthreads_default()
+    lib().or_count_reference32_list(len(off) - 1, _ptr(off), _ptr(tails), _ptr(vertices),
+                                    len(vertices), _ptr(pv), nt)
+    return pv
 
 
 def count_reference_budget(off, tails, seconds, nthreads=None):

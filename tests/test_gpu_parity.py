@@ -137,8 +137,11 @@ def test_kernel_split_does_not_change_counts(ctx, small_cut):
         t = ctx.timing()
         if small_cut == 1:
             assert t["n_heavy"] > 0 and t["n_tiles"] > 0
+        for gb in (1, 3):  # persistent grid size of the group kernel
+            ctx.tune(group_blocks=gb)
+            assert ctx.count_csr(off, tails.astype(np.int64)) == exp
     finally:
-        ctx.tune(small_cut=xxtric_amd.DEFAULT_LIGHT_CUT)
+        ctx.tune(small_cut=xxtric_amd.DEFAULT_LIGHT_CUT, group_blocks=4)
 
 
 def test_invalid_inputs_are_rejected(ctx):
@@ -220,6 +223,38 @@ def test_reference_ffi_shapes(ctx):
     assert int(d_cnt.download(np.int64, 1)[0]) == exp2
     for b in (d_ind, d_el, d_cnt, d_mask, d_el2):
         b.close()
+
+
+def test_tric_driver(golden, tmp_path):
+    """The reference-compatible driver: same flags, same output lines."""
+    import os
+    import re
+    import subprocess
+    tric = os.path.join(os.path.dirname(xxtric_amd.__file__), "bin", "tric")
+    g = [x for x in golden["rgg_lcg"] if x["n"] == 100000][0]
+    _say("tric -n 100000 -l")
+    out = subprocess.run([tric, "-n", "100000", "-l"], capture_output=True, text=True, check=True,
+                         timeout=120).stdout
+    assert "Number of edges: %d" % g["ne"] in out
+    assert "Number of triangles: %d" % g["triangles"] in out
+    assert re.search(r"TEPS: [0-9.e+]+", out)
+    # vite input with per-vertex output
+    off, tails = O.rmat_csr(12, 16, 1, True)
+    p = str(tmp_path / "g.bin")
+    O.vite_write(p, off, tails)
+    vf = str(tmp_path / "pv.txt")
+    _say("tric -f vite -v")
+    out = subprocess.run([tric, "-f", p, "-v", vf, "-m", "id"], capture_output=True, text=True,
+                         check=True, timeout=120).stdout
+    exp, exp_pv = O.count_reference(off, tails, per_vertex=True)
+    assert "Number of triangles: %d" % exp in out
+    assert np.array_equal(np.loadtxt(vf, dtype=np.uint64), exp_pv)
+    # device R-MAT
+    r = [x for x in golden["rmat"] if x["scale"] == 16 and x["permute"] == 1][0]
+    _say("tric -R 16")
+    out = subprocess.run([tric, "-R", "16", "-S", "1", "-j"], capture_output=True, text=True,
+                         check=True, timeout=120).stdout
+    assert "Number of triangles: %d" % r["triangles"] in out
 
 
 def test_resident_graph_and_alg_bytes(ctx):

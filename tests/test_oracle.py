@@ -147,6 +147,13 @@ def test_sampled_reference_count_adds_up():
     assert sum(parts) == full
 
 
+def test_listed_reference_count_matches_full():
+    off, tails = O.rmat_csr(12, 16, 5, True)
+    full, pv = O.count_reference(off, tails, per_vertex=True)
+    verts = np.array([0, 5, 17, 4095, 1000, 2], np.int64)
+    assert (O.count_reference_list(off, tails, verts) == pv[verts]).all()
+
+
 def test_openmp_threads_agree():
     off, tails = O.rmat_csr(12, 16, 1, True)
     assert O.count_reference(off, tails, nthreads=1) == O.count_reference(off, tails, nthreads=8)

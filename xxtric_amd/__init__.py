@@ -70,8 +70,9 @@ class Context:
         except Exception:
             pass
 
-    def tune(self, small_cut=0, block_cut=0):
-        self._check(self._L.xt_ctx_tune(self.h, small_cut, block_cut, 0))
+    def tune(self, small_cut=0, group_blocks=0):
+        """Set the light-edge cut and/or the group kernel's workgroups per CU (0 = keep)."""
+        self._check(self._L.xt_ctx_tune(self.h, small_cut, group_blocks, 0))
 
     def timing(self):
         t = capi.XtTiming()

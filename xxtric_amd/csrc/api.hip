@@ -225,12 +225,11 @@ int xt_last_timing(const xt_ctx* ctx, xt_timing* out) {
   return XT_OK;
 }
 
-int xt_ctx_tune(xt_ctx* ctx, int small_cut, int block_cut, int reserved) {
-  (void)reserved;
-  if (!ctx) return XT_ERR_ARG;
+int xt_ctx_tune(xt_ctx* ctx, int light_cut, int group_blocks, int reserved) {
+  if (!ctx || reserved != 0 || group_blocks > 4 || light_cut < 0 || group_blocks < 0) return XT_ERR_ARG;
   std::lock_guard<std::recursive_mutex> lk(ctx->mu);
-  if (small_cut > 0) ctx->small_cut = small_cut;
-  if (block_cut > 0) ctx->block_cut = block_cut;
+  if (light_cut > 0) ctx->small_cut = light_cut;
+  if (group_blocks > 0) ctx->group_blocks = group_blocks;
   return XT_OK;
 }
 

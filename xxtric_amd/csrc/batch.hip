@@ -69,6 +69,7 @@ struct xt_batch {
   int64_t exec_V = -1, exec_E = -1, exec_G = -1;
   uint32_t exec_flags = 0;
   int exec_small_cut = -1;
+  int exec_group_blocks = -1;
   int last_mode = -1;
   bool running = false;
   xt_batch_timing timing = {};
@@ -141,7 +142,8 @@ int batch_launch(xt_batch* b, int mode, uint32_t flags) {
     XT_HIP(ctx, hipEventRecord(b->ev1, st));
   } else if (mode == XT_BATCH_HIPGRAPH) {
     if (!b->exec || b->exec_V != b->V || b->exec_E != b->E || b->exec_G != G ||
-        b->exec_flags != flags || b->exec_small_cut != ctx->small_cut) {
+        b->exec_flags != flags || b->exec_small_cut != ctx->small_cut ||
+        b->exec_group_blocks != ctx->group_blocks) {
       if (b->exec) {
         XT_HIP(ctx, hipGraphExecDestroy(b->exec));
         b->exec = nullptr;
@@ -166,6 +168,7 @@ int batch_launch(xt_batch* b, int mode, uint32_t flags) {
       b->exec_G = G;
       b->exec_flags = flags;
       b->exec_small_cut = ctx->small_cut;
+      b->exec_group_blocks = ctx->group_blocks;
     }
     XT_HIP(ctx, hipEventRecord(b->ev0, st));
     XT_HIP(ctx, hipGraphLaunch(b->exec, st));

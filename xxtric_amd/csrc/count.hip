@@ -29,7 +29,7 @@ namespace xt {
 constexpr int kGraphSlots = 64;        // per-tile LDS slots for per-graph batch totals
 constexpr int kItemsPerTile = 256;     // items of one group tile (one per thread)
 constexpr int kHashSlots = 8192;       // LDS hash capacity: staged lists up to 4096 ids
-constexpr int kGroupBlocksPerCU = 4;   // LDS use of k_count_groups allows four per CU
+constexpr int kGroupThreads = 512;     // eight waves share one staged hash table
 
 struct EdgeLists {
   int64_t s0, s1, l0, l1;  // shorter [s0,s1), longer [l0,l1) in otails
@@ -262,7 +262,7 @@ __device__ __forceinline__ uint32_t hash_slot(int32_t w, int bits) {
 }
 
 template <bool PV, bool BATCH>
-__global__ __launch_bounds__(kBlock) void k_count_groups(
+__global__ __launch_bounds__(kGroupThreads) void k_count_groups(
     int64_t nv, const int64_t* __restrict__ ooff, const int32_t* __restrict__ ot,
     const int64_t* __restrict__ itoff, const int32_t* __restrict__ items,
     const int64_t* __restrict__ tileoff, uint64_t* __restrict__ counters,
@@ -272,7 +272,7 @@ __global__ __launch_bounds__(kBlock) void k_count_groups(
   __shared__ int64_t s_base[kItemsPerTile];
   __shared__ int32_t s_len[kItemsPerTile];
   __shared__ int32_t s_y[kItemsPerTile];
-  __shared__ uint64_t s_red[kBlock / kWave];
+  __shared__ uint64_t s_red[kGroupThreads / kWave];
   __shared__ int64_t s_tile, s_x;
   __shared__ int s_next;
   const int lane = threadIdx.x & (kWave - 1);
@@ -298,7 +298,7 @@ __global__ __launch_bounds__(kBlock) void k_count_groups(
     while ((1ll << bits) < 4 * dx && (1 << bits) < kHashSlots) bits++;
     const uint32_t mask = (1u << bits) - 1u;
     if (hashed) {
-      for (int k = threadIdx.x; k <= (int)mask; k += kBlock) s_hash[k] = -1;
+      for (int k = threadIdx.x; k <= (int)mask; k += kGroupThreads) s_hash[k] = -1;
     }
     // items of this tile: bases and lengths of the streamed lists
     if ((int)threadIdx.x < nitems) {
@@ -310,7 +310,7 @@ __global__ __launch_bounds__(kBlock) void k_count_groups(
     }
     if (hashed) {
       __syncthreads();  // table cleared before inserts
-      for (int64_t k = threadIdx.x; k < dx; k += kBlock) {
+      for (int64_t k = threadIdx.x; k < dx; k += kGroupThreads) {
         const int32_t key = ot[a0 + k];
         uint32_t h = hash_slot(key, bits);
         while (atomicCAS(&s_hash[h], -1, key) != -1) h = (h + 1) & mask;
@@ -377,10 +377,10 @@ __global__ __launch_bounds__(kBlock) void k_count_groups(
       if (lane < len - tail0) probe(lst[tail0 + lane]);
     }
     __syncthreads();
-    const uint64_t bh = block_sum_u64<kBlock>(hits, s_red);
+    const uint64_t bh = block_sum_u64<kGroupThreads>(hits, s_red);
     if constexpr (PV) {
       __syncthreads();
-      const uint64_t bx = block_sum_u64<kBlock>(at_x, s_red);
+      const uint64_t bx = block_sum_u64<kGroupThreads>(at_x, s_red);
       if (threadIdx.x == 0 && bx) atomicAdd(&pv[x], (unsigned long long)bx);
     }
     if (threadIdx.x == 0 && bh) {
@@ -475,9 +475,9 @@ int count_oriented(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne_bound, uint64
   XT_HIP(ctx, hipGetLastError());
   XT_TRY(scan_u32_to_i64(ctx, s, s.deg.as<uint32_t>(), nv, s.tileoff.as<int64_t>(), st));
   if (evs) XT_HIP(ctx, hipEventRecord(evs[1], st));
-  const unsigned groups = (unsigned)(ctx->num_cus * kGroupBlocksPerCU);
+  const unsigned groups = (unsigned)(ctx->num_cus * ctx->group_blocks);
 #define XT_GROUPS(P, B)                                                                             \
-  hipLaunchKernelGGL((k_count_groups<P, B>), dim3(groups), dim3(kBlock), 0, st, nv, ooff, ot,       \
+  hipLaunchKernelGGL((k_count_groups<P, B>), dim3(groups), dim3(kGroupThreads), 0, st, nv, ooff, ot,       \
                      (const int64_t*)s.itoff.as<int64_t>(), (const int32_t*)s.items.as<int32_t>(), \
                      (const int64_t*)s.tileoff.as<int64_t>(), counters, pvp, gid, gt)
   if (pv && batch)

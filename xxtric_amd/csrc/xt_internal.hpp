@@ -78,7 +78,7 @@ struct xt_ctx {
   hipEvent_t ev[xt::kNumEv] = {};
   xt_timing timing = {};
   int small_cut = 32;     // light edges: both out-lists at most this long
-  int block_cut = 4096;   // reserved tuning knob
+  int group_blocks = 4;   // persistent group-kernel workgroups per CU
 };
 
 struct xt_graph {
