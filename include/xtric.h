@@ -69,8 +69,10 @@ int xt_last_timing(const xt_ctx* ctx, xt_timing* out);
  * current value): light_cut -- edges whose two out-lists are both at most this
  * long are intersected one per thread, the rest by the group kernel (default
  * 32); group_blocks -- persistent group-kernel workgroups per CU (1..4,
- * default 4); reserved must be 0. */
-int xt_ctx_tune(xt_ctx* ctx, int light_cut, int group_blocks, int reserved);
+ * default 4); group_round -- 16-byte list chunks each group-kernel thread has
+ * in flight per streaming round (1, 2 or 4, default 2).  None of them changes
+ * a result. */
+int xt_ctx_tune(xt_ctx* ctx, int light_cut, int group_blocks, int group_round);
 
 /* ---- device memory plumbing ------------------------------------------------
  * The CudaMalloc / CudaMemcpyHtoD / CudaMemcpyDtoH / CudaFree /

@@ -125,3 +125,26 @@ def test_capacity_errors(ctx):
     b.run(XT_BATCH_PERSISTENT)
     assert list(b.results()) == [1, 1]
     b.close()
+
+
+def test_staged_host_adds_survive_reuse(ctx):
+    """Host adds are staged in pinned memory and flushed at launch: the caller's
+    arrays may change right after add, and a cleared batch refilled with other
+    graphs gives their counts in every launch mode."""
+    b = xxtric_amd.Batch(ctx, 2_000_000, 20_000_000, 256)
+    for rnd, (lo, hi) in enumerate([(0, 40), (40, 60)]):
+        _say("staged round", rnd)
+        gs = list(range(lo, hi))
+        exp = _expected(gs)
+        for g in gs:
+            off, tails = xxtric_amd.batch_graph_csr(SEED, g, 200, 20000)
+            off = off.astype(np.int64)                  # the exact buffers handed over
+            tails = tails.astype(np.int32)
+            b.add(off, tails)
+            tails[:] = 0                                # caller reuses its buffers
+            off[:] = 0
+        for mode in (XT_BATCH_PERSISTENT, XT_BATCH_HIPGRAPH, XT_BATCH_PER_GRAPH):
+            b.run(mode)
+            assert (b.results() == exp).all()
+        b.clear()
+    b.close()

@@ -140,8 +140,30 @@ def test_kernel_split_does_not_change_counts(ctx, small_cut):
         for gb in (1, 3):  # persistent grid size of the group kernel
             ctx.tune(group_blocks=gb)
             assert ctx.count_csr(off, tails.astype(np.int64)) == exp
+        for gr in (1, 2):  # chunks in flight per group-kernel thread
+            ctx.tune(group_blocks=4, group_round=gr)
+            tot, pv = ctx.count_csr(off, tails.astype(np.int64), per_vertex=True)
+            assert tot == exp and (pv == exp_pv).all()
     finally:
-        ctx.tune(small_cut=xxtric_amd.DEFAULT_LIGHT_CUT, group_blocks=4)
+        ctx.tune(small_cut=xxtric_amd.DEFAULT_LIGHT_CUT, group_blocks=4, group_round=2)
+
+
+@pytest.mark.parametrize("n", [1500, 4098])
+def test_complete_graph_fills_the_lds_hash(ctx, n):
+    """K_n: out-lists of up to n-1 ids, so the group kernel's LDS hash runs at
+    up to two ids per bucket (spilled buckets, continued lookups); at n = 4098
+    vertex 0's list of 4097 ids is past the hash and is searched in global
+    memory.  Closed forms: C(n,3) triangles, vertex i credited C(n-1-i, 2)."""
+    _say("complete", n)
+    off = np.arange(n + 1, dtype=np.int64) * (n - 1)
+    cols = np.tile(np.arange(n, dtype=np.int64), n).reshape(n, n)
+    tails = cols[~np.eye(n, dtype=bool)].reshape(-1)
+    tot, pv = ctx.count_csr(off, tails, per_vertex=True)
+    i = np.arange(n, dtype=np.int64)
+    assert tot == n * (n - 1) * (n - 2) // 6
+    assert (pv == (n - 1 - i) * (n - 2 - i) // 2).all()
+    assert ctx.timing()["n_heavy"] > 0
+    assert ctx.count_csr(off, tails) == tot
 
 
 def test_invalid_inputs_are_rejected(ctx):

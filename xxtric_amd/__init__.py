@@ -70,9 +70,10 @@ class Context:
         except Exception:
             pass
 
-    def tune(self, small_cut=0, group_blocks=0):
-        """Set the light-edge cut and/or the group kernel's workgroups per CU (0 = keep)."""
-        self._check(self._L.xt_ctx_tune(self.h, small_cut, group_blocks, 0))
+    def tune(self, small_cut=0, group_blocks=0, group_round=0):
+        """Set the light-edge cut, the group kernel's workgroups per CU and/or its
+        chunks in flight per thread (0 = keep).  No setting changes a result."""
+        self._check(self._L.xt_ctx_tune(self.h, small_cut, group_blocks, group_round))
 
     def timing(self):
         t = capi.XtTiming()

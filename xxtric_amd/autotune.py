@@ -1,11 +1,14 @@
 """Autotuner for the counting pipeline's launch knobs.
 
-Two knobs decide how work is split (xt_ctx_tune):
+Three knobs decide how work is split (xt_ctx_tune):
   * light_cut    -- edges whose out-lists are both at most this long are
                     intersected one per thread; longer ones go to the group
                     kernel (LDS-staged list, streamed partner lists);
   * group_blocks -- persistent group-kernel workgroups per CU (each holds a
-                    32 KB LDS hash table, so at most four fit a CU).
+                    32 KB LDS hash table, so at most four fit a CU);
+  * group_round  -- 16-byte list chunks each group-kernel thread keeps in
+                    flight per round (1, 2 or 4: more loads in flight against
+                    more registers, hence fewer resident workgroups).
 The tuner times every combination on a sample workload with the library's own
 HIP-event timers (median of `reps` counts), keeps the fastest, and caches the
 choice per device and workload in a JSON file so later contexts apply it
@@ -25,6 +28,7 @@ import numpy as np
 DEFAULT_CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
 LIGHT_CUTS = (8, 16, 32, 64, 128)
 GROUP_BLOCKS = (2, 3, 4)
+GROUP_ROUNDS = (1, 2, 4)
 
 
 def _time_graph(ctx, g, reps):
@@ -57,7 +61,7 @@ def _time_batch(b, reps):
 
 
 def tune(ctx, workload="rmat", scale=22, graphs=1024, reps=3, cuts=LIGHT_CUTS,
-         blocks=GROUP_BLOCKS, cache=DEFAULT_CACHE, log=print):
+         blocks=GROUP_BLOCKS, rounds=GROUP_ROUNDS, cache=DEFAULT_CACHE, log=print):
     """Sweep the knobs on a sample workload; apply and cache the fastest."""
     import xxtric_amd
     trials = []
@@ -75,17 +79,21 @@ def tune(ctx, workload="rmat", scale=22, graphs=1024, reps=3, cuts=LIGHT_CUTS,
     ref = None
     for c in cuts:
         for gb in blocks:
-            ctx.tune(small_cut=c, group_blocks=gb)
-            ms, res = runner()
-            if ref is None:
-                ref = res
-            same = bool(np.all(np.asarray(res) == np.asarray(ref)))
-            if not same:
-                raise AssertionError("results differ at light_cut=%d group_blocks=%d" % (c, gb))
-            trials.append({"light_cut": c, "group_blocks": gb, "ms": round(ms, 3)})
-            log("autotune %s light_cut=%d group_blocks=%d: %.3f ms" % (key, c, gb, ms))
+            for gr in rounds:
+                ctx.tune(small_cut=c, group_blocks=gb, group_round=gr)
+                ms, res = runner()
+                if ref is None:
+                    ref = res
+                same = bool(np.all(np.asarray(res) == np.asarray(ref)))
+                if not same:
+                    raise AssertionError("results differ at light_cut=%d group_blocks=%d group_round=%d"
+                                         % (c, gb, gr))
+                trials.append({"light_cut": c, "group_blocks": gb, "group_round": gr,
+                               "ms": round(ms, 3)})
+                log("autotune %s light_cut=%d group_blocks=%d group_round=%d: %.3f ms"
+                    % (key, c, gb, gr, ms))
     best = min(trials, key=lambda t: t["ms"])
-    ctx.tune(small_cut=best["light_cut"], group_blocks=best["group_blocks"])
+    _apply(ctx, best)
     entry = {"best": best, "trials": trials, "time": time.strftime("%Y-%m-%d %H:%M:%S")}
     if cache:
         data = {}
@@ -108,8 +116,13 @@ def apply_cached(ctx, key, cache=DEFAULT_CACHE):
     if not entry:
         return None
     b = entry["best"]
-    ctx.tune(small_cut=b["light_cut"], group_blocks=b["group_blocks"])
+    _apply(ctx, b)
     return b
+
+
+def _apply(ctx, b):
+    ctx.tune(small_cut=b["light_cut"], group_blocks=b["group_blocks"],
+             group_round=b.get("group_round", 0))
 
 
 def main():

@@ -13,6 +13,11 @@
 //     that launches the single-graph pipeline once per graph.
 // All scratch is sized at creation so the pipeline can be captured once as a
 // hipGraph and replayed (no allocation, no memset node inside the capture).
+// Graphs added from host memory are staged: xt_batch_add copies the local CSR
+// into a pinned host mirror of the local arena and returns; the next launch
+// (or device-side generation) flushes all staged graphs with two bulk
+// asynchronous copies and one rebase kernel over all of them.
+#include <cstring>
 #include <vector>
 
 #include "xt_device.hpp"
@@ -34,6 +39,45 @@ __global__ __launch_bounds__(256) void k_rebase(int64_t nv, int64_t ne, int64_t 
     if (i <= nv) goff[vbase + i] = ebase + loff[i];
     if (i < nv) gid[vbase + i] = g;
     if (i < ne) gtails[ebase + i] = (int32_t)(vbase + ltails[i]);
+  }
+}
+
+// Rebase of many staged graphs in one launch.  rec holds kRecWords words per
+// graph {nv, ne, vbase, ebase, lobase, work start}; graph j spans work items
+// [start_j, start_j + max(nv_j + 1, ne_j)).  Each thread takes a run of
+// kRebaseRun consecutive items: one search, then a forward walk.
+constexpr int kRecWords = 6;
+constexpr int kRebaseRun = 8;
+
+__global__ __launch_bounds__(256) void k_rebase_many(int64_t ng, int64_t total, int32_t g0,
+                                                     const int64_t* __restrict__ rec,
+                                                     const int64_t* __restrict__ loff,
+                                                     const int32_t* __restrict__ ltails,
+                                                     int64_t* __restrict__ goff,
+                                                     int32_t* __restrict__ gtails,
+                                                     int32_t* __restrict__ gid) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x * kRebaseRun;
+  for (int64_t i0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * kRebaseRun; i0 < total;
+       i0 += stride) {
+    int64_t lo = 0, hi = ng - 1;  // last graph with start <= i0
+    while (lo < hi) {
+      const int64_t mid = lo + ((hi - lo + 1) >> 1);
+      if (rec[mid * kRecWords + 5] <= i0)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    int64_t j = lo;
+    const int64_t i1 = i0 + kRebaseRun < total ? i0 + kRebaseRun : total;
+    for (int64_t i = i0; i < i1; i++) {
+      while (j + 1 < ng && rec[(j + 1) * kRecWords + 5] <= i) j++;
+      const int64_t* r = rec + j * kRecWords;
+      const int64_t nv = r[0], ne = r[1], vbase = r[2], ebase = r[3], lobase = r[4];
+      const int64_t k = i - r[5];
+      if (k <= nv) goff[vbase + k] = ebase + loff[lobase + k];
+      if (k < nv) gid[vbase + k] = g0 + (int32_t)j;
+      if (k < ne) gtails[ebase + k] = (int32_t)(vbase + ltails[ebase + k]);
+    }
   }
 }
 
@@ -70,6 +114,15 @@ struct xt_batch {
   uint32_t exec_flags = 0;
   int exec_small_cut = -1;
   int exec_group_blocks = -1;
+  int exec_group_round = -1;
+  // pinned host mirror of the local arena for staged host adds
+  int64_t* h_loff = nullptr;
+  int32_t* h_ltails = nullptr;
+  int64_t* h_rec = nullptr;  // kRecWords per staged graph
+  int64_t* d_rec = nullptr;
+  int64_t pend_g = 0;         // graphs [pend_g, graphs.size()) are staged, not yet on the device
+  hipEvent_t ev_stage = nullptr;
+  bool stage_inflight = false;
   int last_mode = -1;
   bool running = false;
   xt_batch_timing timing = {};
@@ -81,9 +134,13 @@ static void batch_free(xt_batch* b) {
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
-  void* ps[] = {b->goff, b->gtails, b->gid, b->loff, b->ltails, b->gtotal, b->ptotal};
+  void* ps[] = {b->goff, b->gtails, b->gid, b->loff, b->ltails, b->gtotal, b->ptotal, b->d_rec};
   for (void* p : ps)
     if (p) (void)hipFree(p);
+  void* hs[] = {b->h_loff, b->h_ltails, b->h_rec};
+  for (void* p : hs)
+    if (p) (void)hipHostFree(p);
+  if (b->ev_stage) (void)hipEventDestroy(b->ev_stage);
   release(b->s);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);
@@ -114,6 +171,58 @@ static int batch_commit(xt_batch* b, int64_t nv, int64_t ne) {
   b->V += nv;
   b->E += ne;
   b->LO += nv + 1;
+  b->pend_g = (int64_t)b->graphs.size();
+  return XT_OK;
+}
+
+// Waits until the last flush has finished reading the pinned staging buffers.
+static int batch_stage_idle(xt_batch* b) {
+  if (b->stage_inflight) {
+    XT_HIP(b->ctx, hipEventSynchronize(b->ev_stage));
+    b->stage_inflight = false;
+  }
+  return XT_OK;
+}
+
+// Moves every staged graph to the device: two bulk copies of the local CSR
+// ranges (the staged graphs form a suffix of the local arena) and one rebase
+// launch for all of them.  Stream-ordered; nothing waits on the host.
+static int batch_flush(xt_batch* b) {
+  const int64_t G = (int64_t)b->graphs.size();
+  if (b->pend_g >= G) return XT_OK;
+  xt_ctx* ctx = b->ctx;
+  XT_TRY(batch_stage_idle(b));
+  const BatchGraph& first = b->graphs[b->pend_g];
+  const int64_t lo0 = first.lobase, e0 = first.ebase;
+  XT_HIP(ctx, hipMemcpyAsync(b->loff + lo0, b->h_loff + lo0, (size_t)(b->LO - lo0) * sizeof(int64_t),
+                             hipMemcpyHostToDevice, b->stream));
+  if (b->E > e0)
+    XT_HIP(ctx, hipMemcpyAsync(b->ltails + e0, b->h_ltails + e0, (size_t)(b->E - e0) * sizeof(int32_t),
+                               hipMemcpyHostToDevice, b->stream));
+  const int64_t ng = G - b->pend_g;
+  int64_t work = 0;
+  for (int64_t j = 0; j < ng; j++) {
+    const BatchGraph& r = b->graphs[b->pend_g + j];
+    int64_t* w = b->h_rec + j * kRecWords;
+    w[0] = r.nv;
+    w[1] = r.ne;
+    w[2] = r.vbase;
+    w[3] = r.ebase;
+    w[4] = r.lobase;
+    w[5] = work;
+    work += r.nv + 1 > r.ne ? r.nv + 1 : r.ne;
+  }
+  XT_HIP(ctx, hipMemcpyAsync(b->d_rec, b->h_rec, (size_t)ng * kRecWords * sizeof(int64_t),
+                             hipMemcpyHostToDevice, b->stream));
+  int64_t grid = (work + 256 * kRebaseRun - 1) / (256 * kRebaseRun);
+  if (grid > 8192) grid = 8192;
+  hipLaunchKernelGGL(k_rebase_many, dim3((unsigned)grid), dim3(256), 0, b->stream, ng, work,
+                     (int32_t)b->pend_g, (const int64_t*)b->d_rec, (const int64_t*)b->loff,
+                     (const int32_t*)b->ltails, b->goff, b->gtails, b->gid);
+  XT_HIP(ctx, hipGetLastError());
+  XT_HIP(ctx, hipEventRecord(b->ev_stage, b->stream));
+  b->stage_inflight = true;
+  b->pend_g = G;
   return XT_OK;
 }
 
@@ -136,6 +245,7 @@ int batch_launch(xt_batch* b, int mode, uint32_t flags) {
   const int64_t G = (int64_t)b->graphs.size();
   b->last_mode = mode;
   if (G == 0) return XT_OK;
+  XT_TRY(batch_flush(b));
   if (mode == XT_BATCH_PERSISTENT) {
     XT_HIP(ctx, hipEventRecord(b->ev0, st));
     XT_TRY(batch_pipeline(b, flags));
@@ -143,7 +253,7 @@ int batch_launch(xt_batch* b, int mode, uint32_t flags) {
   } else if (mode == XT_BATCH_HIPGRAPH) {
     if (!b->exec || b->exec_V != b->V || b->exec_E != b->E || b->exec_G != G ||
         b->exec_flags != flags || b->exec_small_cut != ctx->small_cut ||
-        b->exec_group_blocks != ctx->group_blocks) {
+        b->exec_group_blocks != ctx->group_blocks || b->exec_group_round != ctx->group_round) {
       if (b->exec) {
         XT_HIP(ctx, hipGraphExecDestroy(b->exec));
         b->exec = nullptr;
@@ -169,6 +279,7 @@ int batch_launch(xt_batch* b, int mode, uint32_t flags) {
       b->exec_flags = flags;
       b->exec_small_cut = ctx->small_cut;
       b->exec_group_blocks = ctx->group_blocks;
+      b->exec_group_round = ctx->group_round;
     }
     XT_HIP(ctx, hipEventRecord(b->ev0, st));
     XT_HIP(ctx, hipGraphLaunch(b->exec, st));
@@ -229,14 +340,21 @@ int batch_add_host(xt_batch* b, int64_t nv, int64_t ne, const int64_t* off, cons
   for (int64_t e = 0; e < ne; e++)
     if (tails[e] < 0 || tails[e] >= nv) return set_err(ctx, XT_ERR_INPUT, "neighbour id out of range [0, nv)");
   XT_TRY(batch_check_room(b, nv, ne));
-  XT_HIP(ctx, hipMemcpyAsync(b->loff + b->LO, off, (size_t)(nv + 1) * sizeof(int64_t),
-                             hipMemcpyHostToDevice, b->stream));
-  if (ne > 0)
-    XT_HIP(ctx, hipMemcpyAsync(b->ltails + b->E, tails, (size_t)ne * sizeof(int32_t),
-                               hipMemcpyHostToDevice, b->stream));
-  XT_TRY(batch_commit(b, nv, ne));
-  // The caller's arrays may be released on return.
-  XT_HIP(ctx, hipStreamSynchronize(b->stream));
+  if (!b->h_loff) {  // pinned staging, allocated on the first host add
+    hipError_t e = hipHostMalloc((void**)&b->h_loff, (size_t)(b->max_v + b->max_g) * sizeof(int64_t));
+    if (e == hipSuccess) e = hipHostMalloc((void**)&b->h_ltails, (size_t)b->max_e * sizeof(int32_t));
+    if (e == hipSuccess) e = hipHostMalloc((void**)&b->h_rec, (size_t)b->max_g * kRecWords * sizeof(int64_t));
+    if (e == hipSuccess) e = hipMalloc((void**)&b->d_rec, (size_t)b->max_g * kRecWords * sizeof(int64_t));
+    if (e == hipSuccess && !b->ev_stage) e = hipEventCreateWithFlags(&b->ev_stage, hipEventDisableTiming);
+    if (e != hipSuccess) return hip_err(ctx, e, "batch staging buffers");
+  }
+  XT_TRY(batch_stage_idle(b));
+  std::memcpy(b->h_loff + b->LO, off, (size_t)(nv + 1) * sizeof(int64_t));
+  if (ne > 0) std::memcpy(b->h_ltails + b->E, tails, (size_t)ne * sizeof(int32_t));
+  b->graphs.push_back(BatchGraph{nv, ne, b->V, b->E, b->LO});
+  b->V += nv;
+  b->E += ne;
+  b->LO += nv + 1;
   return XT_OK;
 }
 
@@ -320,7 +438,9 @@ int xt_batch_add_generated(xt_batch* b, uint64_t base_seed, int64_t first, int64
       std::vector<int32_t> t32(t64.begin(), t64.begin() + m);
       XT_TRY(batch_add_host(b, n, m, off.data(), t32.data()));
     } else {
-      // Erdos-Renyi: generated on the device straight into the local arena
+      // Erdos-Renyi: generated on the device straight into the local arena,
+      // behind any staged host graphs
+      XT_TRY(batch_flush(b));
       XT_TRY(batch_check_room(b, n, 0));
       int64_t m = 0;
       XT_TRY(gen_er_rows(ctx, b->s, n, gseed + 3, thr, b->loff + b->LO, nullptr, 0, &m, b->stream));
@@ -330,6 +450,7 @@ int xt_batch_add_generated(xt_batch* b, uint64_t base_seed, int64_t first, int64
       XT_TRY(batch_commit(b, n, m));
     }
   }
+  XT_TRY(batch_flush(b));
   XT_HIP(ctx, hipStreamSynchronize(b->stream));
   return XT_OK;
 }
@@ -366,8 +487,10 @@ int xt_batch_clear(xt_batch* b) {
   if (!b) return XT_ERR_ARG;
   std::lock_guard<std::recursive_mutex> lk(b->ctx->mu);
   XT_TRY(batch_wait(b));
+  XT_TRY(batch_stage_idle(b));
   b->graphs.clear();
   b->V = b->E = b->LO = 0;
+  b->pend_g = 0;
   return XT_OK;
 }
 

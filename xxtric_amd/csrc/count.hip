@@ -9,12 +9,16 @@
 //    shorter list and searches each id in the longer one (lower_bound restarted
 //    after the previous hit) -- k_count_light, which also classifies the rest;
 //  * heavy edges: the edge belongs to its endpoint x with the longer list (ties
-//    to the source).  Each such x gets an item list -- the other endpoints y of
-//    its heavy edges (k_fill_items).  k_count_groups, a persistent kernel,
-//    draws (x, up to 256 items) tiles from a device work queue, stages N+(x)
-//    once per tile in an LDS hash table, and streams the items' shorter lists
-//    N+(y) through it with coalesced loads: each oriented edge reads only its
-//    shorter list, the longer one is read once per tile.
+//    to the source).  Each such x gets an item list -- one 16-byte record
+//    {offset, length, id} per other endpoint y of its heavy edges
+//    (k_fill_items).  The item lists are cut into tiles of up to 256 items with
+//    one descriptor each (k_tile_desc).  k_count_groups, a persistent kernel,
+//    draws tiles from a device work queue, stages N+(x) once per tile in an LDS
+//    hash table of 4-slot buckets, and streams the items' shorter lists N+(y)
+//    through it: the tile's lists are concatenated into one run of 16-byte
+//    chunks that all 512 threads read with coalesced loads, the next round of
+//    loads in flight while the current one is probed.  Each oriented edge reads
+//    only its shorter list; the longer one is read once per tile.
 // Reductions: per-thread uint64 -> wave64 shuffle tree -> LDS -> one 64-bit
 // atomic per workgroup (per tile for the group kernel).
 //
@@ -27,9 +31,18 @@
 namespace xt {
 
 constexpr int kGraphSlots = 64;        // per-tile LDS slots for per-graph batch totals
-constexpr int kItemsPerTile = 256;     // items of one group tile (one per thread)
-constexpr int kHashSlots = 8192;       // LDS hash capacity: staged lists up to 4096 ids
+constexpr int kItemsPerTile = 256;     // items of one group tile
+constexpr int kTabLog = 11;            // LDS hash: 2048 buckets of four ids (32 KB)
+constexpr int kTabBuckets = 1 << kTabLog;
 constexpr int kGroupThreads = 512;     // eight waves share one staged hash table
+constexpr int64_t kMaxCountEntries = (1ll << 33) - (1ll << 16);
+
+// One heavy edge seen from its owner x: the other endpoint y and its out-list.
+struct alignas(16) Item {
+  int64_t base;  // ooff[y]
+  int32_t len;   // d+(y)
+  int32_t y;
+};
 
 struct EdgeLists {
   int64_t s0, s1, l0, l1;  // shorter [s0,s1), longer [l0,l1) in otails
@@ -193,13 +206,13 @@ __global__ __launch_bounds__(kBlock) void k_count_light(
   }
 }
 
-// Writes the item lists: y goes to items[itoff[x] ..] of its owner x.  itcnt
-// counts down to zero (slot order within a list is irrelevant).
+// Writes the item lists: y's record goes to items[itoff[x] ..] of its owner x.
+// itcnt counts down to zero (slot order within a list is irrelevant).
 __global__ __launch_bounds__(kBlock) void k_fill_items(int64_t nv, const int64_t* __restrict__ ooff,
                                                        const int32_t* __restrict__ ot, int light_cut,
                                                        const int64_t* __restrict__ itoff,
                                                        uint32_t* __restrict__ itcnt,
-                                                       int32_t* __restrict__ items) {
+                                                       Item* __restrict__ items) {
   __shared__ int64_t s_off[kOffCap + 1];
   __shared__ int64_t s_lohi[2];
   const int64_t nep = ooff[nv];
@@ -214,6 +227,8 @@ __global__ __launch_bounds__(kBlock) void k_fill_items(int64_t nv, const int64_t
   int64_t u = tile_row_of(r, s_off, ooff, e0);
   int64_t a0 = tile_off(r, s_off, ooff, u), a1 = tile_off(r, s_off, ooff, u + 1);
   uint32_t own = 0;  // positions of this row's source-owned heavy edges
+  int64_t vb[kPerThread];
+  int32_t vl[kPerThread];
   // One atomic reserves a contiguous range for a run of source-owned items.
   auto flush = [&]() {
     if (!own) return;
@@ -222,7 +237,7 @@ __global__ __launch_bounds__(kBlock) void k_fill_items(int64_t nv, const int64_t
     uint32_t rank = 0;
 #pragma unroll
     for (int jj = 0; jj < kPerThread; jj++)
-      if ((own >> jj) & 1u) items[base + rank++] = ids[jj];
+      if ((own >> jj) & 1u) items[base + rank++] = Item{vb[jj], vl[jj], ids[jj]};
     own = 0;
   };
 #pragma unroll
@@ -236,13 +251,16 @@ __global__ __launch_bounds__(kBlock) void k_fill_items(int64_t nv, const int64_t
         a1 = tile_off(r, s_off, ooff, u + 1);
       }
       const int64_t v = ids[j];
-      const int64_t du = a1 - a0, dv = ooff[v + 1] - ooff[v];
+      const int64_t b0 = ooff[v];
+      const int64_t du = a1 - a0, dv = ooff[v + 1] - b0;
       if (du > light_cut || dv > light_cut) {
         if (du >= dv) {
           own |= 1u << j;
+          vb[j] = b0;
+          vl[j] = (int32_t)dv;
         } else {
           const uint32_t k = atomicSub(&itcnt[v], 1u) - 1u;
-          items[itoff[v] + k] = (int32_t)u;
+          items[itoff[v] + k] = Item{a0, (int32_t)du, (int32_t)u};
         }
       }
     }
@@ -257,139 +275,268 @@ __global__ __launch_bounds__(256) void k_tile_counts(int64_t nv, const int64_t* 
     tcnt[x] = (uint32_t)((itoff[x + 1] - itoff[x] + kItemsPerTile - 1) / kItemsPerTile);
 }
 
-__device__ __forceinline__ uint32_t hash_slot(int32_t w, int bits) {
-  return ((uint32_t)w * 2654435761u) >> (32 - bits);
+// One descriptor per tile: {owner x, item count, first item (low, high word)}.
+__global__ __launch_bounds__(256) void k_tile_desc(int64_t nv, const int64_t* __restrict__ itoff,
+                                                   const int64_t* __restrict__ tileoff,
+                                                   int4* __restrict__ desc) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < nv; x += stride) {
+    const int64_t t0 = tileoff[x], t1 = tileoff[x + 1];
+    if (t0 == t1) continue;
+    const int64_t ib = itoff[x], ie = itoff[x + 1];
+    for (int64_t t = t0; t < t1; t++) {
+      const int64_t i0 = ib + (t - t0) * kItemsPerTile;
+      const int64_t n = ie - i0 < kItemsPerTile ? ie - i0 : kItemsPerTile;
+      desc[t] = make_int4((int32_t)x, (int32_t)n, (int32_t)(uint32_t)(uint64_t)i0,
+                          (int32_t)((uint64_t)i0 >> 32));
+    }
+  }
 }
 
-template <bool PV, bool BATCH>
-__global__ __launch_bounds__(kGroupThreads) void k_count_groups(
-    int64_t nv, const int64_t* __restrict__ ooff, const int32_t* __restrict__ ot,
-    const int64_t* __restrict__ itoff, const int32_t* __restrict__ items,
-    const int64_t* __restrict__ tileoff, uint64_t* __restrict__ counters,
-    unsigned long long* __restrict__ pv, const int32_t* __restrict__ gid,
-    unsigned long long* __restrict__ gtotal) {
-  __shared__ int32_t s_hash[kHashSlots];
-  __shared__ int64_t s_base[kItemsPerTile];
-  __shared__ int32_t s_len[kItemsPerTile];
-  __shared__ int32_t s_y[kItemsPerTile];
-  __shared__ uint64_t s_red[kGroupThreads / kWave];
-  __shared__ int64_t s_tile, s_x;
-  __shared__ int s_next;
-  const int lane = threadIdx.x & (kWave - 1);
-  const int64_t ntiles = tileoff[nv];
+// LDS hash of N+(x): buckets of four ids, empty slots hold -1.  A bucket that
+// was full when a later key hashed to it has the top bit of its last slot set
+// ("spilled"); that key went to the next bucket (linear probing by bucket).  A
+// lookup reads its home bucket with one 16-byte LDS read and stops at the first
+// bucket that holds the id or has not spilled.  Ids are < 2^31 - 1, so neither
+// -1 nor a marked slot equals a key.
+__device__ __forceinline__ uint32_t bucket_of(int32_t w, int bits) {
+  return ((uint32_t)w * 0x9E3779B1u) >> (32 - bits);
+}
+
+__device__ __forceinline__ bool bucket_has(const int4 t, int32_t w) {
+  return (t.x == w) | (t.y == w) | (t.z == w) | ((t.w & 0x7fffffff) == w);
+}
+
+__device__ __forceinline__ bool bucket_spilled(const int4 t) { return t.w < -1; }
+
+__device__ __forceinline__ void tab_insert(int4* tab, uint32_t bmask, int bits, int32_t key) {
+  uint32_t b = bucket_of(key, bits);
   for (;;) {
-    if (threadIdx.x == 0) {
-      const int64_t t = (int64_t)atomicAdd((unsigned long long*)&counters[kTileQueue], 1ull);
-      s_tile = t;
-      s_x = t < ntiles ? row_of(tileoff, 0, nv - 1, t) : 0;
-      s_next = 0;
-    }
-    __syncthreads();
-    const int64_t tile = s_tile;
+    int* s = reinterpret_cast<int*>(tab + b);
+    if (atomicCAS(s, -1, key) == -1) return;
+    if (atomicCAS(s + 1, -1, key) == -1) return;
+    if (atomicCAS(s + 2, -1, key) == -1) return;
+    if (atomicCAS(s + 3, -1, key) == -1) return;
+    atomicOr(s + 3, (int)0x80000000);
+    b = (b + 1) & bmask;
+  }
+}
+
+// Continues a lookup past a spilled home bucket b.  Ends: at most half the
+// slots are used, so some bucket is not full and therefore not spilled.
+__device__ __noinline__ bool tab_continue(const int4* tab, uint32_t b, uint32_t bmask, int32_t w) {
+  for (;;) {
+    b = (b + 1) & bmask;
+    const int4 t = tab[b];
+    if (bucket_has(t, w)) return true;
+    if (!bucket_spilled(t)) return false;
+  }
+}
+
+template <bool PV, bool BATCH, int RC, int WPE>
+__global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(WPE, 8))) void k_count_groups(
+    const int64_t* __restrict__ ooff, const int32_t* __restrict__ ot, const Item* __restrict__ items,
+    const int4* __restrict__ tdesc, const int64_t* __restrict__ ntiles_p,
+    uint64_t* __restrict__ counters, unsigned long long* __restrict__ pv,
+    const int32_t* __restrict__ gid, unsigned long long* __restrict__ gtotal) {
+  __shared__ int4 s_tab[kTabBuckets];
+  __shared__ int64_t s_body[kItemsPerTile];      // first 16-byte aligned id of each item list
+  __shared__ int32_t s_pre[kItemsPerTile + 1];   // exclusive scan of the items' chunk counts
+  __shared__ int32_t s_y[PV ? kItemsPerTile : 1];
+  __shared__ int64_t s_scan[kGroupThreads / kWave];
+  __shared__ uint64_t s_red[kGroupThreads / kWave];
+  __shared__ int4 s_desc;
+  __shared__ uint32_t s_tile;
+  constexpr int kT = kGroupThreads;
+  const int tid = threadIdx.x;
+  const int lane = tid & (kWave - 1);
+  const int wave = tid / kWave;
+  // Tile indices fit 32 bits: there are fewer than nv + ne/256 + 1 < 2^32 tiles.
+  const uint32_t ntiles = (uint32_t)*ntiles_p;
+  unsigned int* queue = reinterpret_cast<unsigned int*>(&counters[kTileQueue]);
+
+  // Work queue, fetched ahead by thread 0: while tile T runs, the descriptor of
+  // T+1 is loaded and the index of T+2 is drawn.
+  uint32_t t_nx = 0, t_nx2 = 0;
+  int4 d_nx = make_int4(0, 0, 0, 0);
+  if (tid == 0) {
+    const uint32_t t = atomicAdd(queue, 1u);
+    s_tile = t;
+    s_desc = t < ntiles ? tdesc[t] : make_int4(0, 0, 0, 0);
+    t_nx = atomicAdd(queue, 1u);
+  }
+  __syncthreads();
+  for (;;) {
+    const uint32_t tile = s_tile;
     if (tile >= ntiles) break;  // uniform across the workgroup
-    const int64_t x = s_x;
-    const int64_t a0 = ooff[x], a1 = ooff[x + 1], dx = a1 - a0;
-    const int64_t i0 = itoff[x] + (tile - tileoff[x]) * kItemsPerTile;
-    const int64_t i1 = min(i0 + (int64_t)kItemsPerTile, itoff[x + 1]);
-    const int nitems = (int)(i1 - i0);
-    const bool hashed = dx <= kHashSlots / 2;
-    // load factor <= 1/4 when the table fits (most probes miss: short probe runs)
-    int bits = 6;
-    while ((1ll << bits) < 4 * dx && (1 << bits) < kHashSlots) bits++;
-    const uint32_t mask = (1u << bits) - 1u;
-    if (hashed) {
-      for (int k = threadIdx.x; k <= (int)mask; k += kGroupThreads) s_hash[k] = -1;
-    }
-    // items of this tile: bases and lengths of the streamed lists
-    if ((int)threadIdx.x < nitems) {
-      const int32_t y = items[i0 + threadIdx.x];
-      const int64_t b = ooff[y];
-      s_y[threadIdx.x] = y;
-      s_base[threadIdx.x] = b;
-      s_len[threadIdx.x] = (int32_t)(ooff[y + 1] - b);
-    }
-    if (hashed) {
-      __syncthreads();  // table cleared before inserts
-      for (int64_t k = threadIdx.x; k < dx; k += kGroupThreads) {
-        const int32_t key = ot[a0 + k];
-        uint32_t h = hash_slot(key, bits);
-        while (atomicCAS(&s_hash[h], -1, key) != -1) h = (h + 1) & mask;
+    const int4 d = s_desc;
+    if (tid == 0) {
+      t_nx2 = t_nx;
+      if (t_nx < ntiles) {
+        d_nx = tdesc[t_nx];
+        t_nx2 = atomicAdd(queue, 1u);
       }
     }
-    __syncthreads();
-    // Each wave takes whole items from an LDS counter and streams the item's
-    // list with its 64 lanes: 16-byte loads (four ids per lane), four in flight
-    // per lane, plus scalar loads for the unaligned head and the tail.
-    uint64_t hits = 0, at_x = 0;
-    int64_t yy = 0;
-    auto probe = [&](int32_t w) {
-      if (w < 0) return;
-      bool found;
+    const int64_t x = d.x;
+    const int nitems = d.y;
+    const int64_t i0 = (int64_t)(((uint64_t)(uint32_t)d.w << 32) | (uint64_t)(uint32_t)d.z);
+    const int64_t a0 = ooff[x], dx = ooff[x + 1] - a0;
+    const bool hashed = dx <= 2 * kTabBuckets;  // at most two ids per bucket on average
+    int bits = 4;
+    while ((1ll << bits) < 2 * dx && bits < kTabLog) bits++;
+    const uint32_t bmask = (1u << bits) - 1u;
+
+    // This thread's item: aligned body of 16-byte chunks plus at most three
+    // leading and three trailing ids, loaded now and probed once the table is up.
+    int32_t nbody = 0, nhead = 0, ntail = 0;
+    int32_t sc[6];
+    if (tid < nitems) {
+      const Item it = items[i0 + tid];
+      int head = (int)((4 - (it.base & 3)) & 3);
+      if (head > it.len) head = it.len;
+      nbody = (it.len - head) >> 2;
+      nhead = head;
+      ntail = it.len - head - 4 * nbody;
+      const int64_t body = it.base + head;
+      s_body[tid] = body;
+      if constexpr (PV) s_y[tid] = it.y;
+      const int64_t tl = body + 4 * (int64_t)nbody;
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        sc[i] = i < nhead ? ot[it.base + i] : -1;
+        sc[3 + i] = i < ntail ? ot[tl + i] : -1;
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < 6; i++) sc[i] = -1;
+    }
+    if (hashed)
+      for (int b = tid; b <= (int)bmask; b += kT) s_tab[b] = make_int4(-1, -1, -1, -1);
+    int64_t nb_all = 0;
+    const int64_t pre = block_excl_scan_i64<kT>(nbody, s_scan, &nb_all);
+    if (tid < nitems) s_pre[tid] = (int32_t)pre;
+    if (tid == 0) s_pre[nitems] = (int32_t)nb_all;
+    __syncthreads();  // table cleared, chunk scan published
+    if (hashed)
+      for (int64_t k = tid; k < dx; k += kT) tab_insert(s_tab, bmask, bits, ot[a0 + k]);
+    __syncthreads();  // table complete
+
+    uint32_t hits = 0, at_x = 0;  // below 2^32: a tile streams fewer than 2^33 ids over 512 threads
+    auto credit = [&](int32_t w, int k) {
+      hits++;
+      if constexpr (PV) {
+        const int64_t yy = s_y[k];
+        int64_t m = x < yy ? x : yy;
+        if ((int64_t)w < m) m = w;
+        if (m == x)
+          at_x++;
+        else
+          atomicAdd(&pv[m], 1ull);
+      }
+    };
+    // Membership of four ids of item k (negative ids are padding).
+    auto probe4 = [&](const int32_t (&w)[4], int k) {
       if (hashed) {
-        uint32_t h = hash_slot(w, bits);
-        int32_t sl;
-        while ((sl = s_hash[h]) != w && sl != -1) h = (h + 1) & mask;
-        found = sl == w;
+        uint32_t b[4];
+        int4 t[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          b[j] = bucket_of(w[j], bits);
+          t[j] = s_tab[b[j]];
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          if (w[j] < 0) continue;
+          bool h = bucket_has(t[j], w[j]);
+          if (!h && bucket_spilled(t[j])) h = tab_continue(s_tab, b[j], bmask, w[j]);
+          if (h) credit(w[j], k);
+        }
       } else {
-        const int64_t p = lower_bound_dev<int32_t>(ot, a0, a1, w);
-        found = p < a1 && ot[p] == w;
-      }
-      if (found) {
-        hits++;
-        if constexpr (PV) {
-          int64_t m = x < yy ? x : yy;
-          if ((int64_t)w < m) m = w;
-          if (m == x)
-            at_x++;
-          else
-            atomicAdd(&pv[m], 1ull);
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          if (w[j] < 0) continue;
+          const int64_t p = lower_bound_dev<int32_t>(ot, a0, a0 + dx, w[j]);
+          if (p < a0 + dx && ot[p] == w[j]) credit(w[j], k);
         }
       }
     };
-    for (;;) {
-      int k = 0;
-      if (lane == 0) k = atomicAdd(&s_next, 1);
-      k = __shfl(k, 0, kWave);
-      if (k >= nitems) break;  // uniform per wave
-      const int64_t base = s_base[k];
-      const int len = s_len[k];
-      yy = s_y[k];
-      const int32_t* __restrict__ lst = ot + base;
-      int head = (int)((4 - (base & 3)) & 3);  // ids before the first 16-byte boundary
-      if (head > len) head = len;
-      if (lane < head) probe(lst[lane]);
-      const int4* __restrict__ body = reinterpret_cast<const int4*>(lst + head);
-      const int nbody = (len - head) >> 2;
-      for (int i = lane; i < nbody; i += 4 * kWave) {
-        int4 v[4];
+    {
+      const int k = tid < nitems ? tid : 0;
+      const int32_t h4[4] = {sc[0], sc[1], sc[2], -1};
+      const int32_t t4[4] = {sc[3], sc[4], sc[5], -1};
+      probe4(h4, k);
+      probe4(t4, k);
+    }
+
+    // Body chunks: round r covers chunks [r*kT*RC, ...); wave w takes
+    // a contiguous window of 64*RC of them, lane l the chunks
+    // window + q*64 + l.  A chunk's item is found by binary search in s_pre for
+    // q = 0 and by stepping forward for the next q.
+    const int nchunk = (int)nb_all;
+    auto locate = [&](int c0, int (&kq)[RC], int4 (&vq)[RC]) {
+      int c = c0 + wave * (kWave * RC) + lane;
+      int kk = 0;
+      if (c < nchunk) {
 #pragma unroll
-        for (int q = 0; q < 4; q++)
-          v[q] = i + q * kWave < nbody ? body[i + q * kWave] : make_int4(-1, -1, -1, -1);
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-          probe(v[q].x);
-          probe(v[q].y);
-          probe(v[q].z);
-          probe(v[q].w);
+        for (int step = kItemsPerTile / 2; step > 0; step >>= 1) {
+          const int j = kk + step;
+          if (j < nitems && s_pre[j] <= c) kk = j;
         }
       }
-      const int tail0 = head + 4 * nbody;
-      if (lane < len - tail0) probe(lst[tail0 + lane]);
+#pragma unroll
+      for (int q = 0; q < RC; q++) {
+        if (c < nchunk) {
+          while (s_pre[kk + 1] <= c) kk++;
+          kq[q] = kk;
+          vq[q] = *reinterpret_cast<const int4*>(ot + s_body[kk] + 4 * (int64_t)(c - s_pre[kk]));
+        } else {
+          kq[q] = 0;
+          vq[q] = make_int4(-1, -1, -1, -1);
+        }
+        c += kWave;
+      }
+    };
+    constexpr int kRound = kT * RC;
+    int kc[RC];
+    int4 vc[RC];
+    if (nchunk > 0) locate(0, kc, vc);
+    for (int c0 = 0; c0 < nchunk; c0 += kRound) {
+      int kn[RC];
+      int4 vn[RC];
+      const bool more = c0 + kRound < nchunk;  // uniform
+      if (more) locate(c0 + kRound, kn, vn);
+#pragma unroll
+      for (int q = 0; q < RC; q++) {
+        const int32_t w4[4] = {vc[q].x, vc[q].y, vc[q].z, vc[q].w};
+        probe4(w4, kc[q]);
+      }
+      if (more) {
+#pragma unroll
+        for (int q = 0; q < RC; q++) {
+          kc[q] = kn[q];
+          vc[q] = vn[q];
+        }
+      }
     }
-    __syncthreads();
-    const uint64_t bh = block_sum_u64<kGroupThreads>(hits, s_red);
+
+    const uint64_t bh = block_sum_u64<kT>((uint64_t)hits, s_red);
     if constexpr (PV) {
       __syncthreads();
-      const uint64_t bx = block_sum_u64<kGroupThreads>(at_x, s_red);
-      if (threadIdx.x == 0 && bx) atomicAdd(&pv[x], (unsigned long long)bx);
+      const uint64_t bx = block_sum_u64<kT>((uint64_t)at_x, s_red);
+      if (tid == 0 && bx) atomicAdd(&pv[x], (unsigned long long)bx);
     }
-    if (threadIdx.x == 0 && bh) {
-      if constexpr (BATCH)
-        atomicAdd(&gtotal[gid[x]], (unsigned long long)bh);
-      else
-        atomicAdd((unsigned long long*)&counters[kTotal], (unsigned long long)bh);
+    if (tid == 0) {
+      if (bh) {
+        if constexpr (BATCH)
+          atomicAdd(&gtotal[gid[x]], (unsigned long long)bh);
+        else
+          atomicAdd((unsigned long long*)&counters[kTotal], (unsigned long long)bh);
+      }
+      s_tile = t_nx;
+      s_desc = d_nx;
+      t_nx = t_nx2;
     }
-    __syncthreads();  // LDS is reused by the next tile
+    __syncthreads();  // next tile published; LDS free for reuse
   }
 }
 
@@ -431,15 +578,28 @@ __global__ __launch_bounds__(kBlock) void k_alg_stats(int64_t nv, const int64_t*
   if (threadIdx.x == 0 && b) atomicAdd((unsigned long long*)&counters[kStatHeavyEdges], (unsigned long long)b);
 }
 
+// Tiles of all owners: at most one partial tile per owner plus the full ones,
+// and never more than the heavy edges.
+static int64_t tiles_bound(int64_t nv, int64_t ne) {
+  const int64_t b = nv + ne / kItemsPerTile + 1;
+  return b < ne + 1 ? b : ne + 1;
+}
+
 int count_oriented(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne_bound, uint64_t* pv,
                    const int32_t* gid, uint64_t* gtotal, hipStream_t st, hipEvent_t* evs) {
   if (nv <= 0 || ne_bound <= 0) return XT_OK;
+  // A tile's chunk indices are 32-bit: its lists hold at most ne_bound ids.
+  if (ne_bound >= kMaxCountEntries) {
+    return set_err(ctx, XT_ERR_CAPACITY,
+                   "graph too large: the counting kernels take fewer than 2^33 - 2^16 entries");
+  }
   const bool batch = gid != nullptr;
   const int64_t nchunks = (ne_bound + kChunk - 1) / kChunk;
   XT_TRY(ensure(ctx, s.itcnt, (size_t)nv * sizeof(uint32_t)));
   XT_TRY(ensure(ctx, s.itoff, (size_t)(nv + 1) * sizeof(int64_t)));
-  XT_TRY(ensure(ctx, s.items, (size_t)ne_bound * sizeof(int32_t)));
+  XT_TRY(ensure(ctx, s.items, (size_t)ne_bound * sizeof(Item)));
   XT_TRY(ensure(ctx, s.tileoff, (size_t)(nv + 1) * sizeof(int64_t)));
+  XT_TRY(ensure(ctx, s.tiledesc, (size_t)tiles_bound(nv, ne_bound) * sizeof(int4)));
   XT_TRY(ensure(ctx, s.deg, (size_t)nv * sizeof(uint32_t)));  // reused as per-owner tile counts
   uint64_t* counters = s.counters.as<uint64_t>();
   XT_TRY(fill_u64(ctx, counters + kTileQueue, 1, 0, st));
@@ -466,7 +626,7 @@ int count_oriented(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne_bound, uint64
   XT_TRY(scan_u32_to_i64(ctx, s, s.itcnt.as<uint32_t>(), nv, s.itoff.as<int64_t>(), st));
   hipLaunchKernelGGL(k_fill_items, dim3((unsigned)nchunks), dim3(kBlock), 0, st, nv, ooff, ot, cut,
                      (const int64_t*)s.itoff.as<int64_t>(), s.itcnt.as<uint32_t>(),
-                     s.items.as<int32_t>());
+                     s.items.as<Item>());
   XT_HIP(ctx, hipGetLastError());
   int64_t g = (nv + 255) / 256;
   if (g > 65536) g = 65536;
@@ -474,20 +634,33 @@ int count_oriented(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne_bound, uint64
                      (const int64_t*)s.itoff.as<int64_t>(), s.deg.as<uint32_t>());
   XT_HIP(ctx, hipGetLastError());
   XT_TRY(scan_u32_to_i64(ctx, s, s.deg.as<uint32_t>(), nv, s.tileoff.as<int64_t>(), st));
+  hipLaunchKernelGGL(k_tile_desc, dim3((unsigned)g), dim3(256), 0, st, nv,
+                     (const int64_t*)s.itoff.as<int64_t>(), (const int64_t*)s.tileoff.as<int64_t>(),
+                     s.tiledesc.as<int4>());
+  XT_HIP(ctx, hipGetLastError());
   if (evs) XT_HIP(ctx, hipEventRecord(evs[1], st));
   const unsigned groups = (unsigned)(ctx->num_cus * ctx->group_blocks);
-#define XT_GROUPS(P, B)                                                                             \
-  hipLaunchKernelGGL((k_count_groups<P, B>), dim3(groups), dim3(kGroupThreads), 0, st, nv, ooff, ot,       \
-                     (const int64_t*)s.itoff.as<int64_t>(), (const int32_t*)s.items.as<int32_t>(), \
-                     (const int64_t*)s.tileoff.as<int64_t>(), counters, pvp, gid, gt)
-  if (pv && batch)
-    XT_GROUPS(true, true);
-  else if (pv)
-    XT_GROUPS(true, false);
-  else if (batch)
-    XT_GROUPS(false, true);
-  else
-    XT_GROUPS(false, false);
+#define XT_GROUPS(P, B, R, W)                                                                      \
+  hipLaunchKernelGGL((k_count_groups<P, B, R, W>), dim3(groups), dim3(kGroupThreads), 0, st, ooff,   \
+                     ot, (const Item*)s.items.as<Item>(), (const int4*)s.tiledesc.as<int4>(),       \
+                     (const int64_t*)s.tileoff.as<int64_t>() + nv, counters, pvp, gid, gt)
+#define XT_GROUPS_R(P, B)          \
+  if (ctx->group_round == 1)       \
+    XT_GROUPS(P, B, 1, 8);         \
+  else if (ctx->group_round == 2)  \
+    XT_GROUPS(P, B, 2, 6);         \
+  else                             \
+    XT_GROUPS(P, B, 4, 4);
+  if (pv && batch) {
+    XT_GROUPS_R(true, true);
+  } else if (pv) {
+    XT_GROUPS_R(true, false);
+  } else if (batch) {
+    XT_GROUPS_R(false, true);
+  } else {
+    XT_GROUPS_R(false, false);
+  }
+#undef XT_GROUPS_R
 #undef XT_GROUPS
   XT_HIP(ctx, hipGetLastError());
   return XT_OK;
@@ -519,8 +692,9 @@ int reserve(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne, bool per_vertex) {
   XT_TRY(ensure(ctx, s.oflags, (size_t)(nchunks * kBlock)));
   XT_TRY(ensure(ctx, s.itcnt, (size_t)nvv * sizeof(uint32_t)));
   XT_TRY(ensure(ctx, s.itoff, (size_t)(nvv + 1) * sizeof(int64_t)));
-  XT_TRY(ensure(ctx, s.items, (size_t)nee * sizeof(int32_t)));
+  XT_TRY(ensure(ctx, s.items, (size_t)nee * sizeof(Item)));
   XT_TRY(ensure(ctx, s.tileoff, (size_t)(nvv + 1) * sizeof(int64_t)));
+  XT_TRY(ensure(ctx, s.tiledesc, (size_t)tiles_bound(nvv, nee) * sizeof(int4)));
   const size_t st = scan_tmp_bytes(nvv) > scan_tmp_bytes(nchunks) ? scan_tmp_bytes(nvv) : scan_tmp_bytes(nchunks);
   XT_TRY(ensure(ctx, s.scan_tmp, st));
   if (per_vertex) XT_TRY(ensure(ctx, s.pv, (size_t)nvv * sizeof(uint64_t)));
@@ -529,7 +703,7 @@ int reserve(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne, bool per_vertex) {
 
 void release(Scratch& s) {
   DevBuf* bufs[] = {&s.deg, &s.dplus, &s.tailcnt, &s.qscan, &s.ooff, &s.otails, &s.counters,
-                    &s.oflags, &s.itcnt, &s.itoff, &s.items, &s.tileoff, &s.pv, &s.scan_tmp, &s.in_off,
+                    &s.oflags, &s.itcnt, &s.itoff, &s.items, &s.tileoff, &s.tiledesc, &s.pv, &s.scan_tmp, &s.in_off,
                     &s.in_tails};
   for (DevBuf* b : bufs) {
     if (b->p) (void)hipFree(b->p);

@@ -58,8 +58,9 @@ struct Scratch {
   DevBuf oflags;    // uint8[ne/8]         kept-entry mask per 8 CSR entries (orientation)
   DevBuf itcnt;     // uint32[nv]          heavy edges owned by each vertex
   DevBuf itoff;     // int64[nv+1]         exclusive scan of itcnt
-  DevBuf items;     // int32[ne]           other endpoints of each owner's heavy edges
+  DevBuf items;     // 16 B[ne]            {offset, length, id} of the other endpoint of each owner's heavy edges
   DevBuf tileoff;   // int64[nv+1]         exclusive scan of tiles per owner
+  DevBuf tiledesc;  // int4[nv+ne/256+1]   {owner, items, first item} of each tile
   DevBuf pv;        // uint64[nv]          per-vertex counts
   DevBuf scan_tmp;  // int64[..]           block sums of the scans
   DevBuf in_off;    // int64[nv+1]         staged input offsets (xt_count_csr)
@@ -79,6 +80,7 @@ struct xt_ctx {
   xt_timing timing = {};
   int small_cut = 32;     // light edges: both out-lists at most this long
   int group_blocks = 4;   // persistent group-kernel workgroups per CU
+  int group_round = 2;    // 16-byte chunks per thread per group-kernel streaming round (1, 2, 4)
 };
 
 struct xt_graph {
