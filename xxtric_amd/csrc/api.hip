@@ -83,12 +83,12 @@ int count_device_csr(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne, const int6
   // Grow every scratch buffer before the first event so no allocation (and
   // its implicit synchronisation) falls inside the timed window.
   XT_TRY(ensure(ctx, s.ooff, (size_t)(nv + 1) * sizeof(int64_t)));
-  XT_TRY(ensure(ctx, s.otails, (size_t)(ne > 0 ? ne : 1) * sizeof(int32_t)));
+  XT_TRY(ensure(ctx, s.otails, (size_t)(ne + kOtailsPad) * sizeof(int32_t)));
   if (timed) XT_HIP(ctx, hipEventRecord(ctx->ev[kEvStart], st));
   XT_TRY(orient<T>(ctx, s, nv, ne, off, tails, mode, st));
   if (timed) XT_HIP(ctx, hipEventRecord(ctx->ev[kEvOriented], st));
   if (pv) XT_TRY(fill_u64(ctx, pv, nv, 0, st));
-  XT_TRY(count_oriented(ctx, s, nv, ne, pv, nullptr, nullptr, st,
+  XT_TRY(count_oriented<T>(ctx, s, nv, ne, off, tails, pv, nullptr, nullptr, st,
                         timed ? &ctx->ev[kEvLight] : nullptr));
   if (timed) XT_HIP(ctx, hipEventRecord(ctx->ev[kEvCounted], st));
   return XT_OK;
@@ -111,12 +111,12 @@ static int finish_count(xt_ctx* ctx, int64_t nv, uint32_t flags, hipStream_t st,
   if (h[kBadIds]) return set_err(ctx, XT_ERR_INPUT, "neighbour id out of range [0, nv)");
   if (total) *total = h[kTotal];
   int64_t nep = 0, nheavy = 0, ntiles = 0;
-  const bool worked = nv > 0 && ctx->s.itoff.p != nullptr && ctx->s.ooff.p != nullptr;
+  const bool worked = nv > 0 && ctx->s.itcnt.p != nullptr && ctx->s.ooff.p != nullptr;
   if (nv > 0)
     XT_HIP(ctx, hipMemcpy(&nep, ctx->s.ooff.as<int64_t>() + nv, sizeof(int64_t), hipMemcpyDeviceToHost));
   const bool split = worked && nep > 0;  // the light/items/group phases ran
   if (split) {
-    XT_HIP(ctx, hipMemcpy(&nheavy, ctx->s.itoff.as<int64_t>() + nv, sizeof(int64_t), hipMemcpyDeviceToHost));
+    nheavy = (int64_t)h[kHeavyItems];
     XT_HIP(ctx, hipMemcpy(&ntiles, ctx->s.tileoff.as<int64_t>() + nv, sizeof(int64_t), hipMemcpyDeviceToHost));
   }
   ctx->timing = xt_timing{};

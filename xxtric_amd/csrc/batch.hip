@@ -235,7 +235,8 @@ static int batch_pipeline(xt_batch* b, uint32_t flags) {
   XT_TRY(fill_u64(ctx, b->s.counters.as<uint64_t>(), kNumSlots, 0, st));
   XT_TRY(fill_u64(ctx, b->gtotal, G, 0, st));
   XT_TRY(orient<int32_t>(ctx, b->s, b->V, b->E, b->goff, b->gtails, mode, st));
-  XT_TRY(count_oriented(ctx, b->s, b->V, b->E, nullptr, b->gid, b->gtotal, st, nullptr));
+  XT_TRY(count_oriented<int32_t>(ctx, b->s, b->V, b->E, b->goff, b->gtails, nullptr, b->gid, b->gtotal,
+                                 st, nullptr));
   return XT_OK;
 }
 

@@ -31,10 +31,14 @@
 namespace xt {
 
 constexpr int kGraphSlots = 64;        // per-tile LDS slots for per-graph batch totals
-constexpr int kItemsPerTile = 256;     // items of one group tile
+#ifndef XT_ITEMS_PER_TILE
+#define XT_ITEMS_PER_TILE 256
+#endif
+constexpr int kItemsPerTile = XT_ITEMS_PER_TILE;  // items of one group tile
 constexpr int kTabLog = 11;            // LDS hash: 2048 buckets of four ids (32 KB)
 constexpr int kTabBuckets = 1 << kTabLog;
 constexpr int kGroupThreads = 512;     // eight waves share one staged hash table
+constexpr int kLutBlocks = 1024;       // chunk->item lookup for tiles of up to 64K chunks
 constexpr int64_t kMaxCountEntries = (1ll << 33) - (1ll << 16);
 
 // One heavy edge seen from its owner x: the other endpoint y and its out-list.
@@ -42,6 +46,22 @@ struct alignas(16) Item {
   int64_t base;  // ooff[y]
   int32_t len;   // d+(y)
   int32_t y;
+};
+
+// Four ids at a 4-byte aligned address (one 16-byte load on gfx950).
+typedef int v4i_a4 __attribute__((ext_vector_type(4), aligned(4)));
+// Two consecutive offsets at an 8-byte aligned address (one 16-byte load).
+typedef long long v2i64_a8 __attribute__((ext_vector_type(2), aligned(8)));
+
+// One group tile: up to kItemsPerTile items of owner x and x's own out-list
+// (32 bytes: the group kernel reads it as two int4 values in this field order).
+struct alignas(16) TileDesc {
+  int64_t i0;      // first item
+  int64_t a0;      // ooff[x]
+  int32_t x;
+  int32_t nitems;
+  int32_t dx;      // d+(x)
+  int32_t pad;
 };
 
 struct EdgeLists {
@@ -109,93 +129,136 @@ __device__ __forceinline__ void load_ids(const int32_t* __restrict__ ot, int64_t
 }
 
 // ---------------------------------------------------------------------------
-// Light edges counted inline; heavy edges counted per owning endpoint.
-template <bool PV, bool BATCH>
-__global__ __launch_bounds__(kBlock) void k_count_light(
-    int64_t nv, const int64_t* __restrict__ ooff, const int32_t* __restrict__ ot, int light_cut,
-    uint64_t* __restrict__ counters, uint32_t* __restrict__ itcnt,
-    unsigned long long* __restrict__ pv, const int32_t* __restrict__ gid,
-    unsigned long long* __restrict__ gtotal) {
-  __shared__ int64_t s_off[kOffCap + 1];
+// One pass over the symmetric input rows (2048 entries per workgroup, eight
+// contiguous entries per thread).  Entry (x, w) of row x is the out-edge x->w
+// when orientation kept it and the in-edge w->x otherwise (the input is
+// symmetric).  With dx = d+(x), dw = d+(w):
+//  * light edge (dx, dw <= light_cut), out-edge: intersected here -- the
+//    thread walks the shorter list and searches each id in the longer one
+//    (lower_bound restarted after the previous hit);
+//  * heavy edge owned by x (out-edge with dx >= dw, or in-edge with dx > dw)
+//    and dw > 0 (an empty list closes nothing): appended to x's item list,
+//    which lives in items[off[x] ..] (row x has at most deg(x) items).  One
+//    atomic per thread and row reserves the slots.
+// Every oriented edge is therefore either counted here (once, from its source
+// row) or listed once under its owner; in-edges and edges owned by the other
+// endpoint are skipped.
+template <typename T, bool PV, bool BATCH>
+__global__ __launch_bounds__(kBlock) void k_light_items(
+    int64_t nv, int64_t ne, const int64_t* __restrict__ off, const T* __restrict__ tails,
+    const uint8_t* __restrict__ oflags, const int64_t* __restrict__ ooff,
+    const int32_t* __restrict__ ot, int light_cut, uint64_t* __restrict__ counters,
+    uint32_t* __restrict__ itcnt, Item* __restrict__ items, unsigned long long* __restrict__ pv,
+    const int32_t* __restrict__ gid, unsigned long long* __restrict__ gtotal) {
+  __shared__ int64_t s_off[kOffCap + 1];  // input offsets of the chunk's rows
   __shared__ int64_t s_lohi[2];
   __shared__ uint64_t s_red[kBlock / kWave];
   __shared__ unsigned long long s_g[BATCH ? kGraphSlots : 1];
-  const int64_t nep = ooff[nv];
   const int64_t eb = (int64_t)blockIdx.x * kChunk;
-  if (eb >= nep) return;  // whole workgroup leaves together
-  const int64_t ee = min(nep, eb + (int64_t)kChunk);
+  if (eb >= ne) return;  // whole workgroup leaves together
+  const int64_t ee = min(ne, eb + (int64_t)kChunk);
   if (BATCH && threadIdx.x < kGraphSlots) s_g[threadIdx.x] = 0;
-  const TileRows r = tile_rows(ooff, nv, eb, ee, s_off, s_lohi);
+  const TileRows r = tile_rows(off, nv, eb, ee, s_off, s_lohi);
   const int32_t g0 = BATCH ? gid[r.ulo] : 0;
 
-  // Eight contiguous edges per thread: one row search, then a walk.
-  uint64_t hits_all = 0;
+  uint64_t hits_all = 0, listed = 0;
   const int64_t e0 = eb + (int64_t)threadIdx.x * kPerThread;
-  int32_t ids[kPerThread];
-  if (e0 < ee) load_ids(ot, e0, ee, ids);
-  int64_t u = e0 < ee ? tile_row_of(r, s_off, ooff, e0) : 0;
-  int64_t a0 = e0 < ee ? tile_off(r, s_off, ooff, u) : 0;
-  int64_t a1 = e0 < ee ? tile_off(r, s_off, ooff, u + 1) : 0;
-  uint32_t own_run = 0;  // heavy edges of row u that u owns (source-owned)
-#pragma unroll 1
-  for (int j = 0; j < kPerThread; j++) {
-    const int64_t e = e0 + j;
-    if (e >= ee) break;
-    while (e >= a1) {
-      if (own_run) atomicAdd(&itcnt[u], own_run);
-      own_run = 0;
-      ++u;
-      a0 = a1;
-      a1 = tile_off(r, s_off, ooff, u + 1);
+  if (e0 < ee) {
+    int64_t ids[kPerThread];
+    load_run<T>(tails, e0, ee, ids);
+    const uint32_t fl = oflags[e0 / kPerThread];
+    // The eight neighbours' oriented offsets {ooff[v], ooff[v+1]} are gathered
+    // before any is used: eight 16-byte loads in flight per thread.
+    int64_t gb[kPerThread], ge[kPerThread];
+#pragma unroll
+    for (int j = 0; j < kPerThread; j++) {
+      const int64_t v = ids[j];
+      const bool ok = e0 + j < ee && v >= 0 && v < nv;
+      const v2i64_a8 t = *reinterpret_cast<const v2i64_a8*>(ooff + (ok ? v : 0));
+      gb[j] = t.x;
+      ge[j] = t.y;
     }
-    const int64_t v = ids[j];
-    const int64_t b0 = ooff[v], b1 = ooff[v + 1];
-    const int64_t du = a1 - a0, dv = b1 - b0;
-    if (du > light_cut || dv > light_cut) {  // heavy: item of the longer list's owner
-      if (du >= dv)
-        own_run++;
-      else
-        atomicAdd(&itcnt[v], 1u);
-      continue;
-    }
-    const EdgeLists L = edge_lists(a0, a1, b0, b1);
-    if (L.s1 == L.s0) continue;
-    const int64_t mmin = u < v ? u : v;
-    uint64_t hits = 0, at_min = 0;
-    int64_t pos = L.l0;
-    const int32_t lmax = ot[L.l1 - 1];
-    for (int64_t k = L.s0; k < L.s1; k++) {
-      const int32_t w = ot[k];
-      if (w > lmax) break;
-      pos = lower_bound_dev<int32_t>(ot, pos, L.l1, w);
-      if (ot[pos] == w) {
-        hits++;
-        if constexpr (PV) {
-          if ((int64_t)w < mmin)
-            atomicAdd(&pv[w], 1ull);
-          else
-            at_min++;
+    int64_t u = tile_row_of(r, s_off, off, e0);
+    int64_t nxt = tile_off(r, s_off, off, u + 1);
+    int64_t a0 = ooff[u], a1 = ooff[u + 1];
+    uint32_t own = 0;  // entries of this run that row u owns
+    int64_t vb[kPerThread];
+    int32_t vl[kPerThread];
+    auto flush = [&]() {
+      if (!own) return;
+      const uint32_t cnt = (uint32_t)__popc(own);
+      listed += cnt;
+      const int64_t base = tile_off(r, s_off, off, u) + (int64_t)atomicAdd(&itcnt[u], cnt);
+      uint32_t rank = 0;
+#pragma unroll
+      for (int jj = 0; jj < kPerThread; jj++)
+        if ((own >> jj) & 1u) items[base + rank++] = Item{vb[jj], vl[jj], (int32_t)ids[jj]};
+      own = 0;
+    };
+#pragma unroll
+    for (int j = 0; j < kPerThread; j++) {
+      const int64_t e = e0 + j;
+      if (e >= ee) break;
+      while (e >= nxt) {
+        flush();
+        ++u;
+        nxt = tile_off(r, s_off, off, u + 1);
+        a0 = a1;
+        a1 = ooff[u + 1];
+      }
+      const int64_t v = ids[j];
+      if (v < 0 || v >= nv || v == u) continue;  // out of range (unvalidated input) or self-loop
+      const bool kept = (fl >> j) & 1u;
+      const int64_t b0 = gb[j], b1 = ge[j];
+      const int64_t du = a1 - a0, dv = b1 - b0;
+      if (du > light_cut || dv > light_cut) {  // heavy: listed under the longer list's owner
+        if ((kept ? du >= dv : du > dv) && dv > 0) {  // an empty N+(v) closes nothing
+          own |= 1u << j;
+          vb[j] = b0;
+          vl[j] = (int32_t)dv;
         }
-        pos++;
-        if (pos >= L.l1) break;
+        continue;
+      }
+      if (!kept) continue;  // light in-edge: counted from its source row
+      const EdgeLists L = edge_lists(a0, a1, b0, b1);
+      if (L.s1 == L.s0) continue;
+      const int64_t mmin = u < v ? u : v;
+      uint64_t hits = 0, at_min = 0;
+      int64_t pos = L.l0;
+      const int32_t lmax = ot[L.l1 - 1];
+      for (int64_t k = L.s0; k < L.s1; k++) {
+        const int32_t w = ot[k];
+        if (w > lmax) break;
+        pos = lower_bound_dev<int32_t>(ot, pos, L.l1, w);
+        if (ot[pos] == w) {
+          hits++;
+          if constexpr (PV) {
+            if ((int64_t)w < mmin)
+              atomicAdd(&pv[w], 1ull);
+            else
+              at_min++;
+          }
+          pos++;
+          if (pos >= L.l1) break;
+        }
+      }
+      if constexpr (PV) {
+        if (at_min) atomicAdd(&pv[mmin], (unsigned long long)at_min);
+      }
+      if constexpr (BATCH) {
+        if (hits) {
+          const int32_t slot = gid[u] - g0;
+          if (slot < kGraphSlots)
+            atomicAdd(&s_g[slot], (unsigned long long)hits);
+          else
+            atomicAdd(&gtotal[gid[u]], (unsigned long long)hits);
+        }
+      } else {
+        hits_all += hits;
       }
     }
-    if constexpr (PV) {
-      if (at_min) atomicAdd(&pv[mmin], (unsigned long long)at_min);
-    }
-    if constexpr (BATCH) {
-      if (hits) {
-        const int32_t slot = gid[u] - g0;
-        if (slot < kGraphSlots)
-          atomicAdd(&s_g[slot], (unsigned long long)hits);
-        else
-          atomicAdd(&gtotal[gid[u]], (unsigned long long)hits);
-      }
-    } else {
-      hits_all += hits;
-    }
+    flush();
   }
-  if (own_run) atomicAdd(&itcnt[u], own_run);
   if constexpr (BATCH) {
     __syncthreads();
     if (threadIdx.x < kGraphSlots && s_g[threadIdx.x])
@@ -204,119 +267,71 @@ __global__ __launch_bounds__(kBlock) void k_count_light(
     const uint64_t bsum = block_sum_u64<kBlock>(hits_all, s_red);
     if (threadIdx.x == 0 && bsum) atomicAdd((unsigned long long*)&counters[kTotal], (unsigned long long)bsum);
   }
+  __syncthreads();
+  const uint64_t bl = block_sum_u64<kBlock>(listed, s_red);
+  if (threadIdx.x == 0 && bl) atomicAdd((unsigned long long*)&counters[kHeavyItems], (unsigned long long)bl);
 }
 
-// Writes the item lists: y's record goes to items[itoff[x] ..] of its owner x.
-// itcnt counts down to zero (slot order within a list is irrelevant).
-__global__ __launch_bounds__(kBlock) void k_fill_items(int64_t nv, const int64_t* __restrict__ ooff,
-                                                       const int32_t* __restrict__ ot, int light_cut,
-                                                       const int64_t* __restrict__ itoff,
-                                                       uint32_t* __restrict__ itcnt,
-                                                       Item* __restrict__ items) {
-  __shared__ int64_t s_off[kOffCap + 1];
-  __shared__ int64_t s_lohi[2];
-  const int64_t nep = ooff[nv];
-  const int64_t eb = (int64_t)blockIdx.x * kChunk;
-  if (eb >= nep) return;
-  const int64_t ee = min(nep, eb + (int64_t)kChunk);
-  const TileRows r = tile_rows(ooff, nv, eb, ee, s_off, s_lohi);
-  const int64_t e0 = eb + (int64_t)threadIdx.x * kPerThread;
-  if (e0 >= ee) return;
-  int32_t ids[kPerThread];
-  load_ids(ot, e0, ee, ids);
-  int64_t u = tile_row_of(r, s_off, ooff, e0);
-  int64_t a0 = tile_off(r, s_off, ooff, u), a1 = tile_off(r, s_off, ooff, u + 1);
-  uint32_t own = 0;  // positions of this row's source-owned heavy edges
-  int64_t vb[kPerThread];
-  int32_t vl[kPerThread];
-  // One atomic reserves a contiguous range for a run of source-owned items.
-  auto flush = [&]() {
-    if (!own) return;
-    const uint32_t cnt = (uint32_t)__popc(own);
-    const int64_t base = itoff[u] + (int64_t)(atomicSub(&itcnt[u], cnt) - cnt);
-    uint32_t rank = 0;
-#pragma unroll
-    for (int jj = 0; jj < kPerThread; jj++)
-      if ((own >> jj) & 1u) items[base + rank++] = Item{vb[jj], vl[jj], ids[jj]};
-    own = 0;
-  };
-#pragma unroll
-  for (int j = 0; j < kPerThread; j++) {
-    const int64_t e = e0 + j;
-    if (e < ee) {
-      while (e >= a1) {
-        flush();
-        ++u;
-        a0 = a1;
-        a1 = tile_off(r, s_off, ooff, u + 1);
-      }
-      const int64_t v = ids[j];
-      const int64_t b0 = ooff[v];
-      const int64_t du = a1 - a0, dv = ooff[v + 1] - b0;
-      if (du > light_cut || dv > light_cut) {
-        if (du >= dv) {
-          own |= 1u << j;
-          vb[j] = b0;
-          vl[j] = (int32_t)dv;
-        } else {
-          const uint32_t k = atomicSub(&itcnt[v], 1u) - 1u;
-          items[itoff[v] + k] = Item{a0, (int32_t)du, (int32_t)u};
-        }
-      }
-    }
-  }
-  flush();
-}
-
-__global__ __launch_bounds__(256) void k_tile_counts(int64_t nv, const int64_t* __restrict__ itoff,
+__global__ __launch_bounds__(256) void k_tile_counts(int64_t nv, const uint32_t* __restrict__ itcnt,
                                                      uint32_t* __restrict__ tcnt) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < nv; x += stride)
-    tcnt[x] = (uint32_t)((itoff[x + 1] - itoff[x] + kItemsPerTile - 1) / kItemsPerTile);
+    tcnt[x] = (itcnt[x] + kItemsPerTile - 1) / kItemsPerTile;
 }
 
-// One descriptor per tile: {owner x, item count, first item (low, high word)}.
-__global__ __launch_bounds__(256) void k_tile_desc(int64_t nv, const int64_t* __restrict__ itoff,
+// One descriptor per tile; x's items start at off[x].
+__global__ __launch_bounds__(256) void k_tile_desc(int64_t nv, const int64_t* __restrict__ off,
+                                                   const int64_t* __restrict__ ooff,
+                                                   const uint32_t* __restrict__ itcnt,
                                                    const int64_t* __restrict__ tileoff,
-                                                   int4* __restrict__ desc) {
+                                                   TileDesc* __restrict__ desc) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < nv; x += stride) {
     const int64_t t0 = tileoff[x], t1 = tileoff[x + 1];
     if (t0 == t1) continue;
-    const int64_t ib = itoff[x], ie = itoff[x + 1];
+    const int64_t ib = off[x], cnt = itcnt[x], a0 = ooff[x];
+    const int32_t dx = (int32_t)(ooff[x + 1] - a0);
     for (int64_t t = t0; t < t1; t++) {
-      const int64_t i0 = ib + (t - t0) * kItemsPerTile;
-      const int64_t n = ie - i0 < kItemsPerTile ? ie - i0 : kItemsPerTile;
-      desc[t] = make_int4((int32_t)x, (int32_t)n, (int32_t)(uint32_t)(uint64_t)i0,
-                          (int32_t)((uint64_t)i0 >> 32));
+      const int64_t k = (t - t0) * kItemsPerTile;
+      const int64_t n = cnt - k < kItemsPerTile ? cnt - k : kItemsPerTile;
+      desc[t] = TileDesc{ib + k, a0, (int32_t)x, (int32_t)n, dx, 0};
     }
   }
 }
 
-// LDS hash of N+(x): buckets of four ids, empty slots hold -1.  A bucket that
-// was full when a later key hashed to it has the top bit of its last slot set
-// ("spilled"); that key went to the next bucket (linear probing by bucket).  A
-// lookup reads its home bucket with one 16-byte LDS read and stops at the first
-// bucket that holds the id or has not spilled.  Ids are < 2^31 - 1, so neither
-// -1 nor a marked slot equals a key.
+// LDS hash of N+(x): buckets of four ids; empty slots hold kEmpty = 2^31 - 1.
+// A bucket that was full when a later key hashed to it has the top bit of its
+// last slot set ("spilled"); that key went to the next bucket (linear probing
+// by bucket).  A lookup reads its home bucket with one 16-byte LDS read and
+// stops at the first bucket that holds the id or has not spilled.  Ids are
+// below 2^31 - 1, so neither an empty nor a marked slot equals an id, and the
+// padding id -1 matches nothing.
+constexpr int32_t kEmpty = 0x7fffffff;
+
+// Multiplicative hash on the full-rate 24-bit multiplier; the xor folds the
+// high id bits into the multiplied ones.
 __device__ __forceinline__ uint32_t bucket_of(int32_t w, int bits) {
-  return ((uint32_t)w * 0x9E3779B1u) >> (32 - bits);
+  const uint32_t a = (uint32_t)w ^ ((uint32_t)w >> 13);
+  return (uint32_t)__umul24(a, 0x9E3779u) >> (32 - bits);  // unsigned shift: index < 2^bits
 }
 
-__device__ __forceinline__ bool bucket_has(const int4 t, int32_t w) {
-  return (t.x == w) | (t.y == w) | (t.z == w) | ((t.w & 0x7fffffff) == w);
+// 0 when the bucket holds w (slot values xor w, minimum over the four slots).
+__device__ __forceinline__ uint32_t bucket_miss(const int4 t, int32_t w) {
+  const uint32_t x = (uint32_t)(t.x ^ w), y = (uint32_t)(t.y ^ w), z = (uint32_t)(t.z ^ w);
+  const uint32_t q = (uint32_t)((t.w & 0x7fffffff) ^ w);
+  return min(min(x, y), min(z, q));
 }
 
-__device__ __forceinline__ bool bucket_spilled(const int4 t) { return t.w < -1; }
+__device__ __forceinline__ bool bucket_spilled(const int4 t) { return t.w < 0; }
 
 __device__ __forceinline__ void tab_insert(int4* tab, uint32_t bmask, int bits, int32_t key) {
   uint32_t b = bucket_of(key, bits);
   for (;;) {
     int* s = reinterpret_cast<int*>(tab + b);
-    if (atomicCAS(s, -1, key) == -1) return;
-    if (atomicCAS(s + 1, -1, key) == -1) return;
-    if (atomicCAS(s + 2, -1, key) == -1) return;
-    if (atomicCAS(s + 3, -1, key) == -1) return;
+    if (atomicCAS(s, kEmpty, key) == kEmpty) return;
+    if (atomicCAS(s + 1, kEmpty, key) == kEmpty) return;
+    if (atomicCAS(s + 2, kEmpty, key) == kEmpty) return;
+    if (atomicCAS(s + 3, kEmpty, key) == kEmpty) return;
     atomicOr(s + 3, (int)0x80000000);
     b = (b + 1) & bmask;
   }
@@ -324,30 +339,34 @@ __device__ __forceinline__ void tab_insert(int4* tab, uint32_t bmask, int bits, 
 
 // Continues a lookup past a spilled home bucket b.  Ends: at most half the
 // slots are used, so some bucket is not full and therefore not spilled.
-__device__ __noinline__ bool tab_continue(const int4* tab, uint32_t b, uint32_t bmask, int32_t w) {
+__device__ __forceinline__ bool tab_continue(const int4* tab, uint32_t b, uint32_t bmask, int32_t w) {
   for (;;) {
     b = (b + 1) & bmask;
     const int4 t = tab[b];
-    if (bucket_has(t, w)) return true;
+    if (bucket_miss(t, w) == 0) return true;
     if (!bucket_spilled(t)) return false;
   }
 }
 
 template <bool PV, bool BATCH, int RC, int WPE>
 __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(WPE, 8))) void k_count_groups(
-    const int64_t* __restrict__ ooff, const int32_t* __restrict__ ot, const Item* __restrict__ items,
-    const int4* __restrict__ tdesc, const int64_t* __restrict__ ntiles_p,
-    uint64_t* __restrict__ counters, unsigned long long* __restrict__ pv,
-    const int32_t* __restrict__ gid, unsigned long long* __restrict__ gtotal) {
+    const int32_t* __restrict__ ot, const Item* __restrict__ items, const TileDesc* __restrict__ tdesc,
+    const int64_t* __restrict__ ntiles_p, uint64_t* __restrict__ counters,
+    unsigned long long* __restrict__ pv, const int32_t* __restrict__ gid,
+    unsigned long long* __restrict__ gtotal) {
   __shared__ int4 s_tab[kTabBuckets];
-  __shared__ int64_t s_body[kItemsPerTile];      // first 16-byte aligned id of each item list
+  __shared__ int64_t s_base[kItemsPerTile];      // first id of each item list
+  __shared__ int32_t s_len[kItemsPerTile];       // its length
   __shared__ int32_t s_pre[kItemsPerTile + 1];   // exclusive scan of the items' chunk counts
   __shared__ int32_t s_y[PV ? kItemsPerTile : 1];
+  __shared__ uint16_t s_lut[kLutBlocks];          // item holding chunk 64*b
+  __shared__ unsigned long long s_starts[kLutBlocks];  // items starting inside block b (bit = offset)
   __shared__ int64_t s_scan[kGroupThreads / kWave];
   __shared__ uint64_t s_red[kGroupThreads / kWave];
-  __shared__ int4 s_desc;
+  __shared__ int4 s_desc[2][2];  // TileDesc of the current and of the next tile (ping-pong)
   __shared__ uint32_t s_tile;
   constexpr int kT = kGroupThreads;
+  constexpr int kJ = (kItemsPerTile + kT - 1) / kT;  // items per thread in the tile set-up
   const int tid = threadIdx.x;
   const int lane = tid & (kWave - 1);
   const int wave = tid / kWave;
@@ -355,74 +374,152 @@ __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(W
   const uint32_t ntiles = (uint32_t)*ntiles_p;
   unsigned int* queue = reinterpret_cast<unsigned int*>(&counters[kTileQueue]);
 
-  // Work queue, fetched ahead by thread 0: while tile T runs, the descriptor of
+  // Work queue, fetched ahead by wave 0: while tile T runs, the descriptor of
   // T+1 is loaded and the index of T+2 is drawn.
+  auto draw = [&]() -> uint32_t {
+    uint32_t v = 0;
+    if (lane == 0) v = atomicAdd(queue, 1u);
+    return __builtin_amdgcn_readfirstlane(v);
+  };
+  // Descriptors are copied global -> LDS by wave 0's lanes 0-1 with one 16-byte
+  // LDS-DMA load each (no registers held across the tile).
+  const int4* __restrict__ tdesc4 = reinterpret_cast<const int4*>(tdesc);
+  auto fetch_desc = [&](uint32_t t, int slot) {
+    if (lane < 2) __builtin_amdgcn_global_load_lds(tdesc4 + 2 * (size_t)t + lane, &s_desc[slot][0], 16, 0, 0);
+  };
   uint32_t t_nx = 0, t_nx2 = 0;
-  int4 d_nx = make_int4(0, 0, 0, 0);
-  if (tid == 0) {
-    const uint32_t t = atomicAdd(queue, 1u);
-    s_tile = t;
-    s_desc = t < ntiles ? tdesc[t] : make_int4(0, 0, 0, 0);
-    t_nx = atomicAdd(queue, 1u);
+  int par = 0;
+  if (wave == 0) {
+    const uint32_t t = draw();
+    if (lane == 0) s_tile = t;
+    if (t < ntiles) fetch_desc(t, 0);
+    t_nx = draw();
+    __builtin_amdgcn_s_waitcnt(0);  // the LDS-DMA copy has landed
   }
   __syncthreads();
   for (;;) {
     const uint32_t tile = s_tile;
     if (tile >= ntiles) break;  // uniform across the workgroup
-    const int4 d = s_desc;
-    if (tid == 0) {
+    TileDesc d;
+    {
+      const int4 u0 = s_desc[par][0], u1 = s_desc[par][1];
+      d.i0 = (int64_t)(((uint64_t)(uint32_t)u0.y << 32) | (uint32_t)u0.x);
+      d.a0 = (int64_t)(((uint64_t)(uint32_t)u0.w << 32) | (uint32_t)u0.z);
+      d.x = u1.x;
+      d.nitems = u1.y;
+      d.dx = u1.z;
+    }
+    if (wave == 0) {
       t_nx2 = t_nx;
       if (t_nx < ntiles) {
-        d_nx = tdesc[t_nx];
-        t_nx2 = atomicAdd(queue, 1u);
+        fetch_desc(t_nx, par ^ 1);  // the other slot was last read before the previous barrier
+        t_nx2 = draw();
       }
     }
     const int64_t x = d.x;
-    const int nitems = d.y;
-    const int64_t i0 = (int64_t)(((uint64_t)(uint32_t)d.w << 32) | (uint64_t)(uint32_t)d.z);
-    const int64_t a0 = ooff[x], dx = ooff[x + 1] - a0;
+    const int nitems = d.nitems;
+    const int64_t i0 = d.i0, a0 = d.a0, dx = d.dx;
     const bool hashed = dx <= 2 * kTabBuckets;  // at most two ids per bucket on average
     int bits = 4;
     while ((1ll << bits) < 2 * dx && bits < kTabLog) bits++;
     const uint32_t bmask = (1u << bits) - 1u;
 
-    // This thread's item: aligned body of 16-byte chunks plus at most three
-    // leading and three trailing ids, loaded now and probed once the table is up.
-    int32_t nbody = 0, nhead = 0, ntail = 0;
-    int32_t sc[6];
-    if (tid < nitems) {
-      const Item it = items[i0 + tid];
-      int head = (int)((4 - (it.base & 3)) & 3);
-      if (head > it.len) head = it.len;
-      nbody = (it.len - head) >> 2;
-      nhead = head;
-      ntail = it.len - head - 4 * nbody;
-      const int64_t body = it.base + head;
-      s_body[tid] = body;
-      if constexpr (PV) s_y[tid] = it.y;
-      const int64_t tl = body + 4 * (int64_t)nbody;
+    // Items k = tid*kJ + j: each list is streamed as ceil(len/4) 16-byte chunks
+    // from its first id (dword-aligned loads; the last chunk is masked).
+    int32_t nc[kJ];
+    int32_t nchunks = 0;
 #pragma unroll
-      for (int i = 0; i < 3; i++) {
-        sc[i] = i < nhead ? ot[it.base + i] : -1;
-        sc[3 + i] = i < ntail ? ot[tl + i] : -1;
+    for (int j = 0; j < kJ; j++) {
+      const int k = tid * kJ + j;
+      nc[j] = 0;
+      if (k < nitems) {
+        const Item it = items[i0 + k];
+        s_base[k] = it.base;
+        s_len[k] = it.len;
+        if constexpr (PV) s_y[k] = it.y;
+        nc[j] = (it.len + 3) >> 2;
+        nchunks += nc[j];
       }
-    } else {
-#pragma unroll
-      for (int i = 0; i < 6; i++) sc[i] = -1;
     }
     if (hashed)
-      for (int b = tid; b <= (int)bmask; b += kT) s_tab[b] = make_int4(-1, -1, -1, -1);
-    int64_t nb_all = 0;
-    const int64_t pre = block_excl_scan_i64<kT>(nbody, s_scan, &nb_all);
-    if (tid < nitems) s_pre[tid] = (int32_t)pre;
-    if (tid == 0) s_pre[nitems] = (int32_t)nb_all;
-    __syncthreads();  // table cleared, chunk scan published
+      for (int b = tid; b <= (int)bmask; b += kT) s_tab[b] = make_int4(kEmpty, kEmpty, kEmpty, kEmpty);
+    int64_t nch_all = 0;
+    int64_t pre = block_excl_scan_i64<kT>(nchunks, s_scan, &nch_all);
+#pragma unroll
+    for (int j = 0; j < kJ; j++) {
+      const int k = tid * kJ + j;
+      if (k < nitems) s_pre[k] = (int32_t)pre;
+      pre += nc[j];
+    }
+    if (tid == 0) s_pre[nitems] = (int32_t)nch_all;
+    // Entries past the last item never compare <= a chunk index, so the
+    // binary search below needs no bounds test.
+    for (int k = nitems + 1 + tid; k <= kItemsPerTile; k += kT) s_pre[k] = INT32_MAX;
+    const int nchunk = (int)nch_all;
+    const int nblk = (nchunk + kWave - 1) / kWave;
+    const bool lut = nblk <= kLutBlocks;  // uniform
+    if (lut)
+      for (int b = tid; b < nblk; b += kT) s_starts[b] = 0ull;
+    __syncthreads();  // table cleared, item table and chunk scan published
+
+    // Chunk -> item.  Chunks are taken 64 at a time, one per lane, from
+    // 64-aligned blocks, so a wave's lanes share one block b: the item is
+    // s_lut[b] (the item holding the block's first chunk) plus the items that
+    // start inside the block at or before the lane.  Larger tiles search s_pre.
+    auto search = [&](int c) {
+      int kk = 0;
+#pragma unroll
+      for (int step = kItemsPerTile / 2; step > 0; step >>= 1)
+        kk = s_pre[kk + step] <= c ? kk + step : kk;
+      return kk;
+    };
+    if (lut) {
+      for (int b = tid; b < nblk; b += kT) s_lut[b] = (uint16_t)search(b * kWave);
+#pragma unroll
+      for (int j = 0; j < kJ; j++) {
+        const int k = tid * kJ + j;
+        if (k < nitems) {
+          const int c = s_pre[k];
+          if (c & (kWave - 1)) atomicOr(&s_starts[c / kWave], 1ull << (c & (kWave - 1)));
+        }
+      }
+    }
+    // Bits 1..lane: item starts inside the block up to and including this lane.
+    const unsigned long long upto = (2ull << lane) - 2ull;
+    // Chunks: round r covers chunks [r*kT*RC, ...); wave w takes the 64*RC
+    // chunks from r*kT*RC + w*64*RC, lane l the chunks ... + q*64 + l.  Chunks
+    // past the end load the last chunk again and count as no ids.
+    auto locate = [&](int c0, int (&kq)[RC], int4 (&vq)[RC], int (&rq)[RC]) {
+#pragma unroll
+      for (int q = 0; q < RC; q++) {
+        const int c = c0 + wave * (kWave * RC) + q * kWave + lane;
+        const int cc = c < nchunk ? c : nchunk - 1;
+        int kk;
+        if (lut) {
+          const int b = cc / kWave;
+          kk = s_lut[b] + __popcll(s_starts[b] & upto);
+          if (c >= nchunk) kk = nitems - 1;
+        } else {
+          kk = search(cc);
+        }
+        const int cl = cc - s_pre[kk];
+        kq[q] = kk;
+        rq[q] = c < nchunk ? s_len[kk] - 4 * cl : 0;  // ids of this chunk inside the list
+        const v4i_a4 t = *reinterpret_cast<const v4i_a4*>(ot + s_base[kk] + 4 * (int64_t)cl);
+        vq[q] = make_int4(t.x, t.y, t.z, t.w);
+      }
+    };
+    __syncthreads();  // chunk lookup built
+    constexpr int kRound = kT * RC;
+    int kA[RC], rA[RC], kB[RC], rB[RC];
+    int4 vA[RC], vB[RC];
+    if (nchunk > 0) locate(0, kA, vA, rA);  // first loads fly while the table is filled
     if (hashed)
       for (int64_t k = tid; k < dx; k += kT) tab_insert(s_tab, bmask, bits, ot[a0 + k]);
     __syncthreads();  // table complete
 
     uint32_t hits = 0, at_x = 0;  // below 2^32: a tile streams fewer than 2^33 ids over 512 threads
-    auto credit = [&](int32_t w, int k) {
+    auto credit = [&](int32_t w, int k) {  // a hit, with its per-vertex credit
       hits++;
       if constexpr (PV) {
         const int64_t yy = s_y[k];
@@ -434,88 +531,75 @@ __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(W
           atomicAdd(&pv[m], 1ull);
       }
     };
-    // Membership of four ids of item k (negative ids are padding).
-    auto probe4 = [&](const int32_t (&w)[4], int k) {
-      if (hashed) {
-        uint32_t b[4];
-        int4 t[4];
+    // Membership in the LDS hash of the ids of one chunk of item k (r of its
+    // four ids lie inside the list).  The common case is branch-free: four
+    // bucket reads, an xor/min test, a carry-add of the hits; one branch takes
+    // the rare continuations past spilled buckets.
+    auto probe_hash = [&](const int4 v, int k, int r) {
+      const int32_t w[4] = {r > 0 ? v.x : -1, r > 1 ? v.y : -1, r > 2 ? v.z : -1, r > 3 ? v.w : -1};
+      int4 t[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) t[j] = s_tab[bucket_of(w[j], bits)];
+      uint32_t hitm = 0;
+      bool need = false;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const bool h = bucket_miss(t[j], w[j]) == 0;
+        if constexpr (PV)
+          hitm |= (uint32_t)h << j;
+        else
+          hits += h;
+        need |= !h && bucket_spilled(t[j]) && w[j] >= 0;
+      }
+      if (need) {
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-          b[j] = bucket_of(w[j], bits);
-          t[j] = s_tab[b[j]];
-        }
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-          if (w[j] < 0) continue;
-          bool h = bucket_has(t[j], w[j]);
-          if (!h && bucket_spilled(t[j])) h = tab_continue(s_tab, b[j], bmask, w[j]);
-          if (h) credit(w[j], k);
-        }
-      } else {
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-          if (w[j] < 0) continue;
-          const int64_t p = lower_bound_dev<int32_t>(ot, a0, a0 + dx, w[j]);
-          if (p < a0 + dx && ot[p] == w[j]) credit(w[j], k);
+          if (w[j] < 0 || bucket_miss(t[j], w[j]) == 0 || !bucket_spilled(t[j])) continue;
+          if (tab_continue(s_tab, bucket_of(w[j], bits), bmask, w[j])) {
+            if constexpr (PV)
+              hitm |= 1u << j;
+            else
+              hits++;
+          }
         }
       }
+      if constexpr (PV) {
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+          if ((hitm >> j) & 1u) credit(w[j], k);
+      }
     };
-    {
-      const int k = tid < nitems ? tid : 0;
-      const int32_t h4[4] = {sc[0], sc[1], sc[2], -1};
-      const int32_t t4[4] = {sc[3], sc[4], sc[5], -1};
-      probe4(h4, k);
-      probe4(t4, k);
-    }
+    // Lists too long for the LDS hash: binary search in N+(x) in global memory.
+    auto probe_search = [&](const int4 v, int k, int r) {
+      const int32_t w[4] = {r > 0 ? v.x : -1, r > 1 ? v.y : -1, r > 2 ? v.z : -1, r > 3 ? v.w : -1};
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        if (w[j] < 0) continue;
+        const int64_t p = lower_bound_dev<int32_t>(ot, a0, a0 + dx, w[j]);
+        if (p < a0 + dx && ot[p] == w[j]) credit(w[j], k);
+      }
+    };
 
-    // Body chunks: round r covers chunks [r*kT*RC, ...); wave w takes
-    // a contiguous window of 64*RC of them, lane l the chunks
-    // window + q*64 + l.  A chunk's item is found by binary search in s_pre for
-    // q = 0 and by stepping forward for the next q.
-    const int nchunk = (int)nb_all;
-    auto locate = [&](int c0, int (&kq)[RC], int4 (&vq)[RC]) {
-      int c = c0 + wave * (kWave * RC) + lane;
-      int kk = 0;
-      if (c < nchunk) {
+    if (hashed && nchunk > 0) {
+      // Two register buffers alternate by hand (no loop-carried copies) and the
+      // next round is always issued (past the end it re-reads the last chunk,
+      // masked), so every round waits only for its own loads while the next
+      // round's are in flight.
+      for (int c0 = 0;; c0 += 2 * kRound) {
+        locate(c0 + kRound, kB, vB, rB);
 #pragma unroll
-        for (int step = kItemsPerTile / 2; step > 0; step >>= 1) {
-          const int j = kk + step;
-          if (j < nitems && s_pre[j] <= c) kk = j;
-        }
+        for (int q = 0; q < RC; q++) probe_hash(vA[q], kA[q], rA[q]);
+        if (c0 + kRound >= nchunk) break;  // uniform
+        locate(c0 + 2 * kRound, kA, vA, rA);
+#pragma unroll
+        for (int q = 0; q < RC; q++) probe_hash(vB[q], kB[q], rB[q]);
+        if (c0 + 2 * kRound >= nchunk) break;  // uniform
       }
+    } else if (!hashed) {
+      for (int c0 = 0; c0 < nchunk; c0 += kRound) {
+        if (c0 > 0) locate(c0, kA, vA, rA);
 #pragma unroll
-      for (int q = 0; q < RC; q++) {
-        if (c < nchunk) {
-          while (s_pre[kk + 1] <= c) kk++;
-          kq[q] = kk;
-          vq[q] = *reinterpret_cast<const int4*>(ot + s_body[kk] + 4 * (int64_t)(c - s_pre[kk]));
-        } else {
-          kq[q] = 0;
-          vq[q] = make_int4(-1, -1, -1, -1);
-        }
-        c += kWave;
-      }
-    };
-    constexpr int kRound = kT * RC;
-    int kc[RC];
-    int4 vc[RC];
-    if (nchunk > 0) locate(0, kc, vc);
-    for (int c0 = 0; c0 < nchunk; c0 += kRound) {
-      int kn[RC];
-      int4 vn[RC];
-      const bool more = c0 + kRound < nchunk;  // uniform
-      if (more) locate(c0 + kRound, kn, vn);
-#pragma unroll
-      for (int q = 0; q < RC; q++) {
-        const int32_t w4[4] = {vc[q].x, vc[q].y, vc[q].z, vc[q].w};
-        probe4(w4, kc[q]);
-      }
-      if (more) {
-#pragma unroll
-        for (int q = 0; q < RC; q++) {
-          kc[q] = kn[q];
-          vc[q] = vn[q];
-        }
+        for (int q = 0; q < RC; q++) probe_search(vA[q], kA[q], rA[q]);
       }
     }
 
@@ -525,17 +609,18 @@ __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(W
       const uint64_t bx = block_sum_u64<kT>((uint64_t)at_x, s_red);
       if (tid == 0 && bx) atomicAdd(&pv[x], (unsigned long long)bx);
     }
-    if (tid == 0) {
-      if (bh) {
-        if constexpr (BATCH)
-          atomicAdd(&gtotal[gid[x]], (unsigned long long)bh);
-        else
-          atomicAdd((unsigned long long*)&counters[kTotal], (unsigned long long)bh);
-      }
-      s_tile = t_nx;
-      s_desc = d_nx;
-      t_nx = t_nx2;
+    if (tid == 0 && bh) {
+      if constexpr (BATCH)
+        atomicAdd(&gtotal[gid[x]], (unsigned long long)bh);
+      else
+        atomicAdd((unsigned long long*)&counters[kTotal], (unsigned long long)bh);
     }
+    if (wave == 0) {
+      __builtin_amdgcn_s_waitcnt(0);  // the next descriptor's LDS-DMA copy has landed
+      if (lane == 0) s_tile = t_nx;
+    }
+    t_nx = t_nx2;
+    par ^= 1;
     __syncthreads();  // next tile published; LDS free for reuse
   }
 }
@@ -585,8 +670,10 @@ static int64_t tiles_bound(int64_t nv, int64_t ne) {
   return b < ne + 1 ? b : ne + 1;
 }
 
-int count_oriented(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne_bound, uint64_t* pv,
-                   const int32_t* gid, uint64_t* gtotal, hipStream_t st, hipEvent_t* evs) {
+template <typename T>
+int count_oriented(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne_bound, const int64_t* in_off,
+                   const T* in_tails, uint64_t* pv, const int32_t* gid, uint64_t* gtotal,
+                   hipStream_t st, hipEvent_t* evs) {
   if (nv <= 0 || ne_bound <= 0) return XT_OK;
   // A tile's chunk indices are 32-bit: its lists hold at most ne_bound ids.
   if (ne_bound >= kMaxCountEntries) {
@@ -596,22 +683,25 @@ int count_oriented(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne_bound, uint64
   const bool batch = gid != nullptr;
   const int64_t nchunks = (ne_bound + kChunk - 1) / kChunk;
   XT_TRY(ensure(ctx, s.itcnt, (size_t)nv * sizeof(uint32_t)));
-  XT_TRY(ensure(ctx, s.itoff, (size_t)(nv + 1) * sizeof(int64_t)));
   XT_TRY(ensure(ctx, s.items, (size_t)ne_bound * sizeof(Item)));
   XT_TRY(ensure(ctx, s.tileoff, (size_t)(nv + 1) * sizeof(int64_t)));
-  XT_TRY(ensure(ctx, s.tiledesc, (size_t)tiles_bound(nv, ne_bound) * sizeof(int4)));
+  XT_TRY(ensure(ctx, s.tiledesc, (size_t)tiles_bound(nv, ne_bound) * sizeof(TileDesc)));
   XT_TRY(ensure(ctx, s.deg, (size_t)nv * sizeof(uint32_t)));  // reused as per-owner tile counts
   uint64_t* counters = s.counters.as<uint64_t>();
   XT_TRY(fill_u64(ctx, counters + kTileQueue, 1, 0, st));
   XT_TRY(fill_u32(ctx, s.itcnt.as<uint32_t>(), nv, 0u, st));
   const int64_t* ooff = s.ooff.as<int64_t>();
   const int32_t* ot = s.otails.as<int32_t>();
+  const uint8_t* oflags = s.oflags.as<uint8_t>();
   unsigned long long* pvp = (unsigned long long*)pv;
   unsigned long long* gt = (unsigned long long*)gtotal;
   const int cut = ctx->small_cut;
-#define XT_LIGHT(P, B)                                                                             \
-  hipLaunchKernelGGL((k_count_light<P, B>), dim3((unsigned)nchunks), dim3(kBlock), 0, st, nv, ooff, \
-                     ot, cut, counters, s.itcnt.as<uint32_t>(), pvp, gid, gt)
+  int64_t g = (nv + 255) / 256;
+  if (g > 65536) g = 65536;
+#define XT_LIGHT(P, B)                                                                              \
+  hipLaunchKernelGGL((k_light_items<T, P, B>), dim3((unsigned)nchunks), dim3(kBlock), 0, st, nv,     \
+                     ne_bound, in_off, in_tails, oflags, ooff, ot, cut, counters,                   \
+                     s.itcnt.as<uint32_t>(), s.items.as<Item>(), pvp, gid, gt)
   if (pv && batch)
     XT_LIGHT(true, true);
   else if (pv)
@@ -623,26 +713,19 @@ int count_oriented(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne_bound, uint64
 #undef XT_LIGHT
   XT_HIP(ctx, hipGetLastError());
   if (evs) XT_HIP(ctx, hipEventRecord(evs[0], st));
-  XT_TRY(scan_u32_to_i64(ctx, s, s.itcnt.as<uint32_t>(), nv, s.itoff.as<int64_t>(), st));
-  hipLaunchKernelGGL(k_fill_items, dim3((unsigned)nchunks), dim3(kBlock), 0, st, nv, ooff, ot, cut,
-                     (const int64_t*)s.itoff.as<int64_t>(), s.itcnt.as<uint32_t>(),
-                     s.items.as<Item>());
-  XT_HIP(ctx, hipGetLastError());
-  int64_t g = (nv + 255) / 256;
-  if (g > 65536) g = 65536;
   hipLaunchKernelGGL(k_tile_counts, dim3((unsigned)g), dim3(256), 0, st, nv,
-                     (const int64_t*)s.itoff.as<int64_t>(), s.deg.as<uint32_t>());
+                     (const uint32_t*)s.itcnt.as<uint32_t>(), s.deg.as<uint32_t>());
   XT_HIP(ctx, hipGetLastError());
   XT_TRY(scan_u32_to_i64(ctx, s, s.deg.as<uint32_t>(), nv, s.tileoff.as<int64_t>(), st));
-  hipLaunchKernelGGL(k_tile_desc, dim3((unsigned)g), dim3(256), 0, st, nv,
-                     (const int64_t*)s.itoff.as<int64_t>(), (const int64_t*)s.tileoff.as<int64_t>(),
-                     s.tiledesc.as<int4>());
+  hipLaunchKernelGGL(k_tile_desc, dim3((unsigned)g), dim3(256), 0, st, nv, in_off, ooff,
+                     (const uint32_t*)s.itcnt.as<uint32_t>(), (const int64_t*)s.tileoff.as<int64_t>(),
+                     s.tiledesc.as<TileDesc>());
   XT_HIP(ctx, hipGetLastError());
   if (evs) XT_HIP(ctx, hipEventRecord(evs[1], st));
   const unsigned groups = (unsigned)(ctx->num_cus * ctx->group_blocks);
 #define XT_GROUPS(P, B, R, W)                                                                      \
-  hipLaunchKernelGGL((k_count_groups<P, B, R, W>), dim3(groups), dim3(kGroupThreads), 0, st, ooff,   \
-                     ot, (const Item*)s.items.as<Item>(), (const int4*)s.tiledesc.as<int4>(),       \
+  hipLaunchKernelGGL((k_count_groups<P, B, R, W>), dim3(groups), dim3(kGroupThreads), 0, st, ot,     \
+                     (const Item*)s.items.as<Item>(), (const TileDesc*)s.tiledesc.as<TileDesc>(),   \
                      (const int64_t*)s.tileoff.as<int64_t>() + nv, counters, pvp, gid, gt)
 #define XT_GROUPS_R(P, B)          \
   if (ctx->group_round == 1)       \
@@ -666,6 +749,11 @@ int count_oriented(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne_bound, uint64
   return XT_OK;
 }
 
+template int count_oriented<int32_t>(xt_ctx*, Scratch&, int64_t, int64_t, const int64_t*, const int32_t*,
+                                     uint64_t*, const int32_t*, uint64_t*, hipStream_t, hipEvent_t*);
+template int count_oriented<int64_t>(xt_ctx*, Scratch&, int64_t, int64_t, const int64_t*, const int64_t*,
+                                     uint64_t*, const int32_t*, uint64_t*, hipStream_t, hipEvent_t*);
+
 int alg_stats(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne_bound, hipStream_t st) {
   if (nv <= 0 || ne_bound <= 0) return XT_OK;
   const int64_t nchunks = (ne_bound + kChunk - 1) / kChunk;
@@ -687,14 +775,13 @@ int reserve(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne, bool per_vertex) {
   XT_TRY(ensure(ctx, s.tailcnt, (size_t)nchunks * sizeof(uint32_t)));
   XT_TRY(ensure(ctx, s.qscan, (size_t)(nchunks + 1) * sizeof(int64_t)));
   XT_TRY(ensure(ctx, s.ooff, (size_t)(nvv + 1) * sizeof(int64_t)));
-  XT_TRY(ensure(ctx, s.otails, (size_t)nee * sizeof(int32_t)));
+  XT_TRY(ensure(ctx, s.otails, (size_t)(nee + kOtailsPad) * sizeof(int32_t)));
   XT_TRY(ensure(ctx, s.counters, kNumSlots * sizeof(uint64_t)));
   XT_TRY(ensure(ctx, s.oflags, (size_t)(nchunks * kBlock)));
   XT_TRY(ensure(ctx, s.itcnt, (size_t)nvv * sizeof(uint32_t)));
-  XT_TRY(ensure(ctx, s.itoff, (size_t)(nvv + 1) * sizeof(int64_t)));
   XT_TRY(ensure(ctx, s.items, (size_t)nee * sizeof(Item)));
   XT_TRY(ensure(ctx, s.tileoff, (size_t)(nvv + 1) * sizeof(int64_t)));
-  XT_TRY(ensure(ctx, s.tiledesc, (size_t)tiles_bound(nvv, nee) * sizeof(int4)));
+  XT_TRY(ensure(ctx, s.tiledesc, (size_t)tiles_bound(nvv, nee) * sizeof(TileDesc)));
   const size_t st = scan_tmp_bytes(nvv) > scan_tmp_bytes(nchunks) ? scan_tmp_bytes(nvv) : scan_tmp_bytes(nchunks);
   XT_TRY(ensure(ctx, s.scan_tmp, st));
   if (per_vertex) XT_TRY(ensure(ctx, s.pv, (size_t)nvv * sizeof(uint64_t)));
@@ -703,7 +790,7 @@ int reserve(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne, bool per_vertex) {
 
 void release(Scratch& s) {
   DevBuf* bufs[] = {&s.deg, &s.dplus, &s.tailcnt, &s.qscan, &s.ooff, &s.otails, &s.counters,
-                    &s.oflags, &s.itcnt, &s.itoff, &s.items, &s.tileoff, &s.tiledesc, &s.pv, &s.scan_tmp, &s.in_off,
+                    &s.oflags, &s.itcnt, &s.items, &s.tileoff, &s.tiledesc, &s.pv, &s.scan_tmp, &s.in_off,
                     &s.in_tails};
   for (DevBuf* b : bufs) {
     if (b->p) (void)hipFree(b->p);

@@ -54,24 +54,6 @@ __global__ void k_check_offsets(int64_t nv, int64_t ne, const int64_t* __restric
   if (bad) atomicAdd((unsigned long long*)&counters[kBadOff], (unsigned long long)bad);
 }
 
-// Loads the kPerThread entries of one thread; a full aligned run of int32 ids
-// is fetched as two 16-byte loads.
-template <typename T>
-__device__ __forceinline__ void load_run(const T* __restrict__ tails, int64_t e0, int64_t ee,
-                                         int64_t (&vals)[kPerThread]) {
-  if constexpr (sizeof(T) == 4) {
-    if (e0 + kPerThread <= ee) {
-      const int4* p = reinterpret_cast<const int4*>(tails + e0);
-      const int4 a = p[0], b = p[1];
-      vals[0] = a.x; vals[1] = a.y; vals[2] = a.z; vals[3] = a.w;
-      vals[4] = b.x; vals[5] = b.y; vals[6] = b.z; vals[7] = b.w;
-      return;
-    }
-  }
-#pragma unroll
-  for (int j = 0; j < kPerThread; j++) vals[j] = (e0 + j < ee) ? (int64_t)tails[e0 + j] : 0;
-}
-
 template <int MODE>
 __device__ __forceinline__ uint64_t orient_key(const uint16_t* __restrict__ deg, int64_t x) {
   if constexpr (MODE == kOrientDegree)
@@ -141,9 +123,26 @@ __global__ __launch_bounds__(kBlock) void k_orient_count(
   if (e0 < ee) {
     int64_t vals[kPerThread];
     load_run<T>(tails, e0, ee, vals);
+    // All eight neighbour keys are gathered before any is used (eight loads in
+    // flight instead of one); ids out of range read vertex 0 and are counted bad.
+    uint16_t dv[kPerThread];
+#pragma unroll
+    for (int j = 0; j < kPerThread; j++) {
+      const int64_t v = vals[j];
+      const bool ok = e0 + j < ee && v >= 0 && v < nv;
+      dv[j] = MODE == kOrientDegree ? deg[ok ? v : 0] : 0;
+    }
     int64_t u = chunk_row_of(r, s_off, off, e0);
+    int64_t cur = chunk_off(r, s_off, off, u);
     int64_t nxt = chunk_off(r, s_off, off, u + 1);
-    uint64_t ku = orient_key<MODE>(deg, u);
+    // key of the row itself, from its offsets (same saturation as k_degree)
+    auto row_key = [&](int64_t uu, int64_t d) -> uint64_t {
+      if constexpr (MODE == kOrientDegree)
+        return ((uint64_t)(d < 65535 ? d : 65535) << 32) | (uint64_t)uu;
+      else
+        return (uint64_t)uu;
+    };
+    uint64_t ku = row_key(u, nxt - cur);
     uint32_t run = 0, fl = 0;
     auto flush = [&](int64_t uu, uint32_t cnt) {
       if (cnt == 0) return;
@@ -162,13 +161,15 @@ __global__ __launch_bounds__(kBlock) void k_orient_count(
           flush(u, run);
           run = 0;
           ++u;
+          cur = nxt;
           nxt = chunk_off(r, s_off, off, u + 1);
-          ku = orient_key<MODE>(deg, u);
+          ku = row_key(u, nxt - cur);
         }
         const int64_t v = vals[j];
+        const uint64_t kv = MODE == kOrientDegree ? (((uint64_t)dv[j] << 32) | (uint64_t)v) : (uint64_t)v;
         if (v < 0 || v >= nv) {
           bad++;
-        } else if (orient_key<MODE>(deg, v) > ku) {
+        } else if (kv > ku) {
           run++;
           fl |= 1u << j;
         }
@@ -379,7 +380,7 @@ template <typename T>
 int orient(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne, const int64_t* off, const T* tails,
            int mode, hipStream_t st) {
   XT_TRY(ensure(ctx, s.ooff, (size_t)(nv + 1) * sizeof(int64_t)));
-  XT_TRY(ensure(ctx, s.otails, (size_t)(ne > 0 ? ne : 1) * sizeof(int32_t)));
+  XT_TRY(ensure(ctx, s.otails, (size_t)(ne + kOtailsPad) * sizeof(int32_t)));
   XT_TRY(ensure(ctx, s.dplus, (size_t)(nv > 0 ? nv : 1) * sizeof(uint32_t)));
   if (nv <= 0) return XT_OK;
   if (ne <= 0) return fill_i64(ctx, s.ooff.as<int64_t>(), nv + 1, 0, st);

@@ -107,6 +107,25 @@ __device__ __forceinline__ int64_t block_excl_scan_i64(int64_t v, int64_t* s_tmp
   return before + inc - v;
 }
 
+// Loads the kPerThread (8) entries of one thread; a full aligned run of int32
+// ids is fetched as two 16-byte loads.
+template <typename T, int kRun>
+__device__ __forceinline__ void load_run(const T* __restrict__ tails, int64_t e0, int64_t ee,
+                                         int64_t (&vals)[kRun]) {
+  static_assert(kRun == 8, "runs of eight entries");
+  if constexpr (sizeof(T) == 4) {
+    if (e0 + kRun <= ee) {
+      const int4* p = reinterpret_cast<const int4*>(tails + e0);
+      const int4 a = p[0], b = p[1];
+      vals[0] = a.x; vals[1] = a.y; vals[2] = a.z; vals[3] = a.w;
+      vals[4] = b.x; vals[5] = b.y; vals[6] = b.z; vals[7] = b.w;
+      return;
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < kRun; j++) vals[j] = (e0 + j < ee) ? (int64_t)tails[e0 + j] : 0;
+}
+
 // SplitMix64 output function (Steele, Lea, Flood, OOPSLA 2014).
 __device__ __forceinline__ uint64_t mix64_dev(uint64_t z) {
   z += 0x9E3779B97F4A7C15ULL;

@@ -17,6 +17,9 @@ namespace xt {
 constexpr int kBlock = 256;
 constexpr int kPerThread = 8;
 constexpr int kChunk = kBlock * kPerThread;  // 2048
+// The oriented ids are read in 16-byte chunks from each list's first id; the
+// last chunk of the last list may read up to three ids past |E+|.
+constexpr int64_t kOtailsPad = 4;
 constexpr int kOffCap = 2048;
 
 // Counter slots in Scratch::counters (uint64).
@@ -30,6 +33,7 @@ enum Slot : int {
   kTileQueue = 8,      // next tile of the persistent group kernel
   kStatMinHeavy = 9,   // same sum over heavy edges only
   kStatHeavyEdges = 10,// number of heavy edges
+  kHeavyItems = 11,    // heavy edges listed under their owners by the counting pass
   kNumSlots = 16
 };
 
@@ -53,14 +57,13 @@ struct Scratch {
   DevBuf tailcnt;   // uint32[nchunks]     flagged entries of each chunk's last vertex
   DevBuf qscan;     // int64[nchunks+1]    exclusive scan of tailcnt
   DevBuf ooff;      // int64[nv+1]         oriented offsets
-  DevBuf otails;    // int32[ne]           oriented out-lists, ascending ids (|E+| <= ne)
+  DevBuf otails;    // int32[ne+4]         oriented out-lists, ascending ids (|E+| <= ne), padded
   DevBuf counters;  // uint64[16]          total, tile queue, input-error flags, statistics
   DevBuf oflags;    // uint8[ne/8]         kept-entry mask per 8 CSR entries (orientation)
   DevBuf itcnt;     // uint32[nv]          heavy edges owned by each vertex
-  DevBuf itoff;     // int64[nv+1]         exclusive scan of itcnt
-  DevBuf items;     // 16 B[ne]            {offset, length, id} of the other endpoint of each owner's heavy edges
+  DevBuf items;     // 16 B[ne]            item lists: owner x's {offset, length, id} records from items[off[x]]
   DevBuf tileoff;   // int64[nv+1]         exclusive scan of tiles per owner
-  DevBuf tiledesc;  // int4[nv+ne/256+1]   {owner, items, first item} of each tile
+  DevBuf tiledesc;  // 32 B[nv+ne/256+1]  {first item, ooff[x], x, items, d+(x)} of each tile
   DevBuf pv;        // uint64[nv]          per-vertex counts
   DevBuf scan_tmp;  // int64[..]           block sums of the scans
   DevBuf in_off;    // int64[nv+1]         staged input offsets (xt_count_csr)
@@ -114,8 +117,10 @@ int orient(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne, const int64_t* off, 
 // and the intersection kernel drains a device tile queue with a fixed grid.
 // evs (nullable): two events recorded after the light-edge kernel and after
 // the item lists are built.
-int count_oriented(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne_bound, uint64_t* pv,
-                   const int32_t* gid, uint64_t* gtotal, hipStream_t st, hipEvent_t* evs);
+template <typename T>
+int count_oriented(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne_bound, const int64_t* in_off,
+                   const T* in_tails, uint64_t* pv, const int32_t* gid, uint64_t* gtotal,
+                   hipStream_t st, hipEvent_t* evs);
 int alg_stats(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne_bound, hipStream_t st);
 // Pre-sizes every scratch buffer for graphs up to (nv, ne).
 int reserve(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne, bool per_vertex);
