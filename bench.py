@@ -16,6 +16,7 @@ of the same graph, extrapolated.
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -26,7 +27,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = "edges/s + %HBM roofline, R-MAT s26 (1 GPU); batched graphs/s, p99 latency"
 HBM_PEAK_GBS = 8000.0
-PROFILE_PMC = os.path.join(ROOT, "profiles", "count_rmat_s26_pmc.json")
+# rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload; its line
+# "traffic_bytes_per_launch: N" gives the measured HBM bytes of one launch.
+PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "k_count_groups_rmat_s26_traffic.md")
 
 
 def log(*a):
@@ -65,9 +68,10 @@ def roofline(r):
     achieved = r["group_bytes"] / (r["group_ms"] * 1e-3) / 1e9
     whole = r["alg_bytes"] / (r["total_ms"] * 1e-3) / 1e9
     traffic = None
-    if os.path.exists(PROFILE_PMC):
-        with open(PROFILE_PMC) as f:
-            traffic = json.load(f).get("group_hbm_bytes")
+    if os.path.exists(PROFILE_TRAFFIC):
+        with open(PROFILE_TRAFFIC) as f:
+            m = re.search(r"^traffic_bytes_per_launch:\s*(\d+)", f.read(), re.M)
+        traffic = int(m.group(1)) if m else None
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "kernel": "k_count_groups", "kernel_ms": round(r["group_ms"], 3),

@@ -308,11 +308,15 @@ __global__ __launch_bounds__(256) void k_tile_desc(int64_t nv, const int64_t* __
 // padding id -1 matches nothing.
 constexpr int32_t kEmpty = 0x7fffffff;
 
-// Multiplicative hash on the full-rate 24-bit multiplier; the xor folds the
-// high id bits into the multiplied ones.
-__device__ __forceinline__ uint32_t bucket_of(int32_t w, int bits) {
-  const uint32_t a = (uint32_t)w ^ ((uint32_t)w >> 13);
-  return (uint32_t)__umul24(a, 0x9E3779u) >> (32 - bits);  // unsigned shift: index < 2^bits
+// Byte offset of w's home bucket: (w xor (w >> 11)) mod 2048, times 16 -- two
+// shifts and one three-input bitwise op.
+__device__ __forceinline__ uint32_t bucket_addr(int32_t w) {
+  const uint32_t u = (uint32_t)w;
+  return ((u << 4) ^ (u >> (kTabLog - 4))) & ((uint32_t)(kTabBuckets - 1) << 4);
+}
+
+__device__ __forceinline__ const int4& bucket_at(const int4* tab, uint32_t addr) {
+  return *reinterpret_cast<const int4*>(reinterpret_cast<const char*>(tab) + addr);
 }
 
 // 0 when the bucket holds w (slot values xor w, minimum over the four slots).
@@ -324,8 +328,9 @@ __device__ __forceinline__ uint32_t bucket_miss(const int4 t, int32_t w) {
 
 __device__ __forceinline__ bool bucket_spilled(const int4 t) { return t.w < 0; }
 
-__device__ __forceinline__ void tab_insert(int4* tab, uint32_t bmask, int bits, int32_t key) {
-  uint32_t b = bucket_of(key, bits);
+// Inserts key; sets *spilled when a full bucket had to be passed.
+__device__ __forceinline__ void tab_insert(int4* tab, int32_t key, int* spilled) {
+  uint32_t b = bucket_addr(key) >> 4;
   for (;;) {
     int* s = reinterpret_cast<int*>(tab + b);
     if (atomicCAS(s, kEmpty, key) == kEmpty) return;
@@ -333,15 +338,16 @@ __device__ __forceinline__ void tab_insert(int4* tab, uint32_t bmask, int bits, 
     if (atomicCAS(s + 2, kEmpty, key) == kEmpty) return;
     if (atomicCAS(s + 3, kEmpty, key) == kEmpty) return;
     atomicOr(s + 3, (int)0x80000000);
-    b = (b + 1) & bmask;
+    *spilled = 1;
+    b = (b + 1) & (kTabBuckets - 1);
   }
 }
 
 // Continues a lookup past a spilled home bucket b.  Ends: at most half the
 // slots are used, so some bucket is not full and therefore not spilled.
-__device__ __forceinline__ bool tab_continue(const int4* tab, uint32_t b, uint32_t bmask, int32_t w) {
+__device__ __forceinline__ bool tab_continue(const int4* tab, uint32_t b, int32_t w) {
   for (;;) {
-    b = (b + 1) & bmask;
+    b = (b + 1) & (kTabBuckets - 1);
     const int4 t = tab[b];
     if (bucket_miss(t, w) == 0) return true;
     if (!bucket_spilled(t)) return false;
@@ -365,6 +371,7 @@ __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(W
   __shared__ uint64_t s_red[kGroupThreads / kWave];
   __shared__ int4 s_desc[2][2];  // TileDesc of the current and of the next tile (ping-pong)
   __shared__ uint32_t s_tile;
+  __shared__ int s_spilled;  // some bucket of the current table spilled
   constexpr int kT = kGroupThreads;
   constexpr int kJ = (kItemsPerTile + kT - 1) / kT;  // items per thread in the tile set-up
   const int tid = threadIdx.x;
@@ -420,9 +427,6 @@ __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(W
     const int nitems = d.nitems;
     const int64_t i0 = d.i0, a0 = d.a0, dx = d.dx;
     const bool hashed = dx <= 2 * kTabBuckets;  // at most two ids per bucket on average
-    int bits = 4;
-    while ((1ll << bits) < 2 * dx && bits < kTabLog) bits++;
-    const uint32_t bmask = (1u << bits) - 1u;
 
     // Items k = tid*kJ + j: each list is streamed as ceil(len/4) 16-byte chunks
     // from its first id (dword-aligned loads; the last chunk is masked).
@@ -442,7 +446,8 @@ __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(W
       }
     }
     if (hashed)
-      for (int b = tid; b <= (int)bmask; b += kT) s_tab[b] = make_int4(kEmpty, kEmpty, kEmpty, kEmpty);
+      for (int b = tid; b < kTabBuckets; b += kT) s_tab[b] = make_int4(kEmpty, kEmpty, kEmpty, kEmpty);
+    if (tid == 0) s_spilled = 0;
     int64_t nch_all = 0;
     int64_t pre = block_excl_scan_i64<kT>(nchunks, s_scan, &nch_all);
 #pragma unroll
@@ -479,45 +484,49 @@ __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(W
       for (int j = 0; j < kJ; j++) {
         const int k = tid * kJ + j;
         if (k < nitems) {
-          const int c = s_pre[k];
+          const uint32_t c = (uint32_t)s_pre[k];
           if (c & (kWave - 1)) atomicOr(&s_starts[c / kWave], 1ull << (c & (kWave - 1)));
         }
       }
     }
     // Bits 1..lane: item starts inside the block up to and including this lane.
     const unsigned long long upto = (2ull << lane) - 2ull;
-    // Chunks: round r covers chunks [r*kT*RC, ...); wave w takes the 64*RC
-    // chunks from r*kT*RC + w*64*RC, lane l the chunks ... + q*64 + l.  Chunks
-    // past the end load the last chunk again and count as no ids.
-    auto locate = [&](int c0, int (&kq)[RC], int4 (&vq)[RC], int (&rq)[RC]) {
+    // Chunks go out in 64-chunk blocks, one chunk per lane: in round r, wave w
+    // takes blocks (r*RC + q)*8 + w for q < RC, so even a small tile spreads
+    // over all eight waves.  A wave whose block lies past the end still issues
+    // its load (the last chunk again, counted as no ids) so that every round
+    // has the same loads in flight, but skips the lookup and the probes.
+    constexpr int kWaves = kT / kWave;
+    auto blk = [&](int round, int q) { return (round * RC + q) * kWaves + wave; };
+    auto locate = [&](int round, int (&kq)[RC], int4 (&vq)[RC], int (&rq)[RC]) {
 #pragma unroll
       for (int q = 0; q < RC; q++) {
-        const int c = c0 + wave * (kWave * RC) + q * kWave + lane;
-        const int cc = c < nchunk ? c : nchunk - 1;
-        int kk;
-        if (lut) {
-          const int b = cc / kWave;
-          kk = s_lut[b] + __popcll(s_starts[b] & upto);
+        const int b = blk(round, q);
+        int kk = nitems - 1, cc = nchunk - 1;
+        rq[q] = 0;
+        if (b < nblk) {  // uniform per wave
+          const int c = b * kWave + lane;
+          cc = c < nchunk ? c : nchunk - 1;
+          kk = lut ? (int)s_lut[b] + __popcll(s_starts[b] & upto) : search(cc);
           if (c >= nchunk) kk = nitems - 1;
-        } else {
-          kk = search(cc);
+          rq[q] = c < nchunk ? s_len[kk] - 4 * (cc - s_pre[kk]) : 0;  // ids of this chunk in the list
         }
         const int cl = cc - s_pre[kk];
         kq[q] = kk;
-        rq[q] = c < nchunk ? s_len[kk] - 4 * cl : 0;  // ids of this chunk inside the list
         const v4i_a4 t = *reinterpret_cast<const v4i_a4*>(ot + s_base[kk] + 4 * (int64_t)cl);
         vq[q] = make_int4(t.x, t.y, t.z, t.w);
       }
     };
     __syncthreads();  // chunk lookup built
-    constexpr int kRound = kT * RC;
+    constexpr int kRound = kT * RC;  // chunks per round
     int kA[RC], rA[RC], kB[RC], rB[RC];
     int4 vA[RC], vB[RC];
     if (nchunk > 0) locate(0, kA, vA, rA);  // first loads fly while the table is filled
     if (hashed)
-      for (int64_t k = tid; k < dx; k += kT) tab_insert(s_tab, bmask, bits, ot[a0 + k]);
+      for (int64_t k = tid; k < dx; k += kT) tab_insert(s_tab, ot[a0 + k], &s_spilled);
     __syncthreads();  // table complete
 
+    const bool spl = s_spilled != 0;  // uniform: read after the table barrier
     uint32_t hits = 0, at_x = 0;  // below 2^32: a tile streams fewer than 2^33 ids over 512 threads
     auto credit = [&](int32_t w, int k) {  // a hit, with its per-vertex credit
       hits++;
@@ -532,14 +541,14 @@ __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(W
       }
     };
     // Membership in the LDS hash of the ids of one chunk of item k (r of its
-    // four ids lie inside the list).  The common case is branch-free: four
-    // bucket reads, an xor/min test, a carry-add of the hits; one branch takes
-    // the rare continuations past spilled buckets.
+    // four ids lie inside the list): four bucket reads, an xor/min test and a
+    // carry-add of the hits, branch-free.  Only tables with a spilled bucket
+    // take the (uniform) branch to the rare continuations.
     auto probe_hash = [&](const int4 v, int k, int r) {
       const int32_t w[4] = {r > 0 ? v.x : -1, r > 1 ? v.y : -1, r > 2 ? v.z : -1, r > 3 ? v.w : -1};
       int4 t[4];
 #pragma unroll
-      for (int j = 0; j < 4; j++) t[j] = s_tab[bucket_of(w[j], bits)];
+      for (int j = 0; j < 4; j++) t[j] = bucket_at(s_tab, bucket_addr(w[j]));
       uint32_t hitm = 0;
       bool need = false;
 #pragma unroll
@@ -551,15 +560,17 @@ __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(W
           hits += h;
         need |= !h && bucket_spilled(t[j]) && w[j] >= 0;
       }
-      if (need) {
+      if (spl) {  // uniform: only tables with a spilled bucket can need this
+        if (need) {
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-          if (w[j] < 0 || bucket_miss(t[j], w[j]) == 0 || !bucket_spilled(t[j])) continue;
-          if (tab_continue(s_tab, bucket_of(w[j], bits), bmask, w[j])) {
-            if constexpr (PV)
-              hitm |= 1u << j;
-            else
-              hits++;
+          for (int j = 0; j < 4; j++) {
+            if (w[j] < 0 || bucket_miss(t[j], w[j]) == 0 || !bucket_spilled(t[j])) continue;
+            if (tab_continue(s_tab, bucket_addr(w[j]) >> 4, w[j])) {
+              if constexpr (PV)
+                hitm |= 1u << j;
+              else
+                hits++;
+            }
           }
         }
       }
@@ -580,26 +591,29 @@ __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(W
       }
     };
 
+    // Two register buffers alternate by hand (no loop-carried copies) and the
+    // next round is always issued (past the end it re-reads the last chunk,
+    // masked), so every round waits only for its own loads while the next
+    // round's are in flight.
     if (hashed && nchunk > 0) {
-      // Two register buffers alternate by hand (no loop-carried copies) and the
-      // next round is always issued (past the end it re-reads the last chunk,
-      // masked), so every round waits only for its own loads while the next
-      // round's are in flight.
-      for (int c0 = 0;; c0 += 2 * kRound) {
-        locate(c0 + kRound, kB, vB, rB);
+      for (int r0 = 0;; r0 += 2) {
+        locate(r0 + 1, kB, vB, rB);
 #pragma unroll
-        for (int q = 0; q < RC; q++) probe_hash(vA[q], kA[q], rA[q]);
-        if (c0 + kRound >= nchunk) break;  // uniform
-        locate(c0 + 2 * kRound, kA, vA, rA);
+        for (int q = 0; q < RC; q++)
+          if (blk(r0, q) < nblk) probe_hash(vA[q], kA[q], rA[q]);  // uniform per wave
+        if ((r0 + 1) * kRound >= nchunk) break;  // uniform
+        locate(r0 + 2, kA, vA, rA);
 #pragma unroll
-        for (int q = 0; q < RC; q++) probe_hash(vB[q], kB[q], rB[q]);
-        if (c0 + 2 * kRound >= nchunk) break;  // uniform
+        for (int q = 0; q < RC; q++)
+          if (blk(r0 + 1, q) < nblk) probe_hash(vB[q], kB[q], rB[q]);
+        if ((r0 + 2) * kRound >= nchunk) break;  // uniform
       }
     } else if (!hashed) {
-      for (int c0 = 0; c0 < nchunk; c0 += kRound) {
-        if (c0 > 0) locate(c0, kA, vA, rA);
+      for (int r0 = 0; r0 * kRound < nchunk; r0++) {
+        if (r0 > 0) locate(r0, kA, vA, rA);
 #pragma unroll
-        for (int q = 0; q < RC; q++) probe_search(vA[q], kA[q], rA[q]);
+        for (int q = 0; q < RC; q++)
+          if (blk(r0, q) < nblk) probe_search(vA[q], kA[q], rA[q]);
       }
     }
 
