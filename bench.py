@@ -141,6 +141,8 @@ def serve_workload(ctx, rate, n_req=4000, pool=400, seed=7):
         o, t = dg.download()
         graphs.append((o, t))
         dg.close()
+    # the request arrays in the layout the C ABI takes (no conversion per submit)
+    graphs = [(np.ascontiguousarray(o, np.int64), np.ascontiguousarray(t, np.int32)) for o, t in graphs]
     kinds = rng.integers(0, pool, n_req)
     big = rng.random(n_req) < 0.01
     kinds[big] = pool + rng.integers(0, 3, big.sum())

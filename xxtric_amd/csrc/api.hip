@@ -30,7 +30,10 @@ static int launch_copy16(xt_ctx* ctx, void* dst, const void* src, size_t bytes, 
 }
 
 int set_err(xt_ctx* ctx, int code, const std::string& msg) {
-  if (ctx) ctx->err = msg;
+  if (ctx) {
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);  // the server's submitters set errors too
+    ctx->err = msg;
+  }
   return code;
 }
 

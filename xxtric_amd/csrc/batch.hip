@@ -332,23 +332,57 @@ int batch_read(xt_batch* b, uint64_t* totals, int64_t n) {
   return XT_OK;
 }
 
-int batch_add_host(xt_batch* b, int64_t nv, int64_t ne, const int64_t* off, const int32_t* tails) {
-  xt_ctx* ctx = b->ctx;
-  if (nv < 1 || ne < 0 || !off || (ne > 0 && !tails)) return set_err(ctx, XT_ERR_ARG, "bad graph");
-  if (off[0] != 0 || off[nv] != ne) return set_err(ctx, XT_ERR_INPUT, "offsets must start at 0 and end at ne");
-  for (int64_t i = 0; i < nv; i++)
-    if (off[i] > off[i + 1]) return set_err(ctx, XT_ERR_INPUT, "offsets are not monotone");
-  for (int64_t e = 0; e < ne; e++)
-    if (tails[e] < 0 || tails[e] >= nv) return set_err(ctx, XT_ERR_INPUT, "neighbour id out of range [0, nv)");
-  XT_TRY(batch_check_room(b, nv, ne));
-  if (!b->h_loff) {  // pinned staging, allocated on the first host add
-    hipError_t e = hipHostMalloc((void**)&b->h_loff, (size_t)(b->max_v + b->max_g) * sizeof(int64_t));
-    if (e == hipSuccess) e = hipHostMalloc((void**)&b->h_ltails, (size_t)b->max_e * sizeof(int32_t));
-    if (e == hipSuccess) e = hipHostMalloc((void**)&b->h_rec, (size_t)b->max_g * kRecWords * sizeof(int64_t));
-    if (e == hipSuccess) e = hipMalloc((void**)&b->d_rec, (size_t)b->max_g * kRecWords * sizeof(int64_t));
-    if (e == hipSuccess && !b->ev_stage) e = hipEventCreateWithFlags(&b->ev_stage, hipEventDisableTiming);
-    if (e != hipSuccess) return hip_err(ctx, e, "batch staging buffers");
+// Checks a host CSR graph of local ids and, when off_out/tails_out are given,
+// copies it in the same pass; an error message, or nullptr.  The loops are
+// branch-free so the compiler vectorises them.
+const char* check_host_graph(int64_t nv, int64_t ne, const int64_t* off, const int32_t* tails,
+                             int64_t* off_out, int32_t* tails_out) {
+  if (nv < 1 || ne < 0 || !off || (ne > 0 && !tails)) return "bad graph";
+  if (off[0] != 0 || off[nv] != ne) return "offsets must start at 0 and end at ne";
+  int64_t down = 0;
+  if (off_out) {
+    off_out[0] = 0;
+    for (int64_t i = 0; i < nv; i++) {
+      const int64_t a = off[i], b = off[i + 1];
+      off_out[i + 1] = b;
+      down |= (int64_t)(b < a);
+    }
+  } else {
+    for (int64_t i = 0; i < nv; i++) down |= (int64_t)(off[i + 1] < off[i]);
   }
+  if (down) return "offsets are not monotone";
+  uint32_t out = 0;
+  const uint32_t n = (uint32_t)nv;  // nv < 2^31: negative ids wrap above it
+  if (tails_out) {
+    for (int64_t e = 0; e < ne; e++) {
+      const int32_t v = tails[e];
+      tails_out[e] = v;
+      out |= (uint32_t)((uint32_t)v >= n);
+    }
+  } else {
+    for (int64_t e = 0; e < ne; e++) out |= (uint32_t)((uint32_t)tails[e] >= n);
+  }
+  if (out) return "neighbour id out of range [0, nv)";
+  return nullptr;
+}
+
+// Allocates the pinned staging mirror (once per batch).
+int batch_stage_reserve(xt_batch* b) {
+  if (b->h_loff) return XT_OK;
+  hipError_t e = hipHostMalloc((void**)&b->h_loff, (size_t)(b->max_v + b->max_g) * sizeof(int64_t));
+  if (e == hipSuccess) e = hipHostMalloc((void**)&b->h_ltails, (size_t)b->max_e * sizeof(int32_t));
+  if (e == hipSuccess) e = hipHostMalloc((void**)&b->h_rec, (size_t)b->max_g * kRecWords * sizeof(int64_t));
+  if (e == hipSuccess) e = hipMalloc((void**)&b->d_rec, (size_t)b->max_g * kRecWords * sizeof(int64_t));
+  if (e == hipSuccess && !b->ev_stage) e = hipEventCreateWithFlags(&b->ev_stage, hipEventDisableTiming);
+  if (e != hipSuccess) return hip_err(b->ctx, e, "batch staging buffers");
+  return XT_OK;
+}
+
+// Stages a graph that check_host_graph accepted: copies it into the pinned
+// mirror; the next launch moves it to the device.
+int batch_stage_host(xt_batch* b, int64_t nv, int64_t ne, const int64_t* off, const int32_t* tails) {
+  XT_TRY(batch_check_room(b, nv, ne));
+  XT_TRY(batch_stage_reserve(b));
   XT_TRY(batch_stage_idle(b));
   std::memcpy(b->h_loff + b->LO, off, (size_t)(nv + 1) * sizeof(int64_t));
   if (ne > 0) std::memcpy(b->h_ltails + b->E, tails, (size_t)ne * sizeof(int32_t));
@@ -357,6 +391,13 @@ int batch_add_host(xt_batch* b, int64_t nv, int64_t ne, const int64_t* off, cons
   b->E += ne;
   b->LO += nv + 1;
   return XT_OK;
+}
+
+int batch_add_host(xt_batch* b, int64_t nv, int64_t ne, const int64_t* off, const int32_t* tails) {
+  if (nv >= (1LL << 31) - 1) return set_err(b->ctx, XT_ERR_ARG, "nv must be below 2^31-1");
+  const char* bad = check_host_graph(nv, ne, off, tails, nullptr, nullptr);
+  if (bad) return set_err(b->ctx, nv < 1 || ne < 0 || !off ? XT_ERR_ARG : XT_ERR_INPUT, bad);
+  return batch_stage_host(b, nv, ne, off, tails);
 }
 
 }  // namespace xt

@@ -13,6 +13,7 @@
 #include <cmath>
 #include <condition_variable>
 #include <deque>
+#include <memory>
 #include <thread>
 #include <unordered_map>
 #include <vector>
@@ -24,7 +25,10 @@ int batch_launch(xt_batch* b, int mode, uint32_t flags);
 int batch_poll(xt_batch* b);
 int batch_wait(xt_batch* b);
 int batch_read(xt_batch* b, uint64_t* totals, int64_t n);
-int batch_add_host(xt_batch* b, int64_t nv, int64_t ne, const int64_t* off, const int32_t* tails);
+const char* check_host_graph(int64_t nv, int64_t ne, const int64_t* off, const int32_t* tails,
+                             int64_t* off_out, int32_t* tails_out);
+int batch_stage_reserve(xt_batch* b);
+int batch_stage_host(xt_batch* b, int64_t nv, int64_t ne, const int64_t* off, const int32_t* tails);
 }  // namespace xt
 
 using namespace xt;
@@ -41,8 +45,8 @@ enum ReqState : int { kQueued = 0, kRunning = 1, kDone = 2 };
 struct Req {
   uint64_t ticket = 0;
   int64_t nv = 0, ne = 0;
-  std::vector<int64_t> off;
-  std::vector<int32_t> tails;
+  std::unique_ptr<int64_t[]> off;  // the request's copy (not zero-filled first)
+  std::unique_ptr<int32_t[]> tails;
   double arrival_us = 0, done_us = 0;
   uint64_t total = 0;
   int64_t footprint = 0;
@@ -153,7 +157,7 @@ void worker_main(xt_server* s) {
         }
       }
       for (Req* r : take) {
-        int rc = batch_add_host(free_slot->b, r->nv, r->ne, r->off.data(), r->tails.data());
+        int rc = batch_stage_host(free_slot->b, r->nv, r->ne, r->off.get(), r->tails.get());
         if (rc != XT_OK) {
           std::lock_guard<std::mutex> lk(s->mu);
           s->worker_err = rc;
@@ -161,8 +165,8 @@ void worker_main(xt_server* s) {
         free_slot->reqs.push_back(r);
         free_slot->V += r->nv;
         free_slot->E += r->ne;
-        std::vector<int64_t>().swap(r->off);  // host copy no longer needed
-        std::vector<int32_t>().swap(r->tails);
+        r->off.reset();  // host copy no longer needed
+        r->tails.reset();
       }
       if (!free_slot->reqs.empty()) {
         int rc = batch_launch(free_slot->b, XT_BATCH_PERSISTENT, XT_ORIENT_DEGREE);
@@ -206,6 +210,7 @@ int xt_serve_create(xt_ctx* ctx, const xt_serve_config* cfg, xt_server** out) {
   s->slots.resize(s->cfg.max_inflight);
   for (ServeSlot& sl : s->slots) {
     int rc = xt_batch_create(ctx, s->slot_v, s->slot_e, s->slot_g, &sl.b);
+    if (rc == XT_OK) rc = batch_stage_reserve(sl.b);  // pinned staging now, not on the first request
     if (rc != XT_OK) {
       for (ServeSlot& o : s->slots)
         if (o.b) xt_batch_destroy(o.b);
@@ -223,19 +228,34 @@ int xt_serve_submit(xt_server* s, int64_t nv, int64_t ne, const int64_t* offsets
                     double arrival_us, uint64_t* ticket) {
   if (!s || !ticket || nv < 1 || ne < 0 || !offsets || (ne > 0 && !tails)) return XT_ERR_ARG;
   const int64_t fp = kBytesPerVertex * nv + kBytesPerEntry * ne;
-  std::lock_guard<std::mutex> lk(s->mu);
-  s->submitted++;
-  if (nv > s->slot_v || ne > s->slot_e || fp > s->cfg.hbm_budget_bytes ||
-      (int64_t)s->queue.size() >= s->cfg.max_queue) {
-    s->rejected++;
-    return set_err(s->ctx, XT_ERR_CAPACITY, "request rejected: larger than a batch slot, the HBM budget or the queue");
-  }
+  auto reject = [&](int code, const char* msg) {
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      s->submitted++;
+      s->rejected++;
+    }
+    return set_err(s->ctx, code, msg);  // outside the server lock
+  };
+  if (nv > s->slot_v || ne > s->slot_e || fp > s->cfg.hbm_budget_bytes)
+    return reject(XT_ERR_CAPACITY, "request rejected: larger than a batch slot, the HBM budget or the queue");
+  // Checked and copied in one pass, in the caller's thread, outside the server lock.
   Req* r = new Req();
-  r->ticket = s->next_ticket++;
   r->nv = nv;
   r->ne = ne;
-  r->off.assign(offsets, offsets + nv + 1);
-  r->tails.assign(tails, tails + ne);
+  r->off.reset(new int64_t[nv + 1]);
+  r->tails.reset(new int32_t[ne > 0 ? ne : 1]);
+  if (const char* bad = check_host_graph(nv, ne, offsets, tails, r->off.get(), r->tails.get())) {
+    delete r;
+    return reject(XT_ERR_INPUT, bad);
+  }
+  std::unique_lock<std::mutex> lk(s->mu);
+  if ((int64_t)s->queue.size() >= s->cfg.max_queue) {
+    lk.unlock();
+    delete r;
+    return reject(XT_ERR_CAPACITY, "request rejected: larger than a batch slot, the HBM budget or the queue");
+  }
+  s->submitted++;
+  r->ticket = s->next_ticket++;
   r->arrival_us = arrival_us >= 0 ? arrival_us : s->now();
   r->footprint = fp;
   if (s->first_arrival < 0 || r->arrival_us < s->first_arrival) s->first_arrival = r->arrival_us;
