@@ -52,6 +52,8 @@ struct alignas(16) Item {
 typedef int v4i_a4 __attribute__((ext_vector_type(4), aligned(4)));
 // Two consecutive offsets at an 8-byte aligned address (one 16-byte load).
 typedef long long v2i64_a8 __attribute__((ext_vector_type(2), aligned(8)));
+// Two consecutive 32-bit offsets at a 4-byte aligned address (one 8-byte load).
+typedef unsigned int v2u32_a4 __attribute__((ext_vector_type(2), aligned(4)));
 
 // One group tile: up to kItemsPerTile items of owner x and x's own out-list
 // (32 bytes: the group kernel reads it as two int4 values in this field order).
@@ -143,10 +145,10 @@ __device__ __forceinline__ void load_ids(const int32_t* __restrict__ ot, int64_t
 // Every oriented edge is therefore either counted here (once, from its source
 // row) or listed once under its owner; in-edges and edges owned by the other
 // endpoint are skipped.
-template <typename T, bool PV, bool BATCH>
+template <typename T, typename OT, bool PV, bool BATCH>
 __global__ __launch_bounds__(kBlock) void k_light_items(
     int64_t nv, int64_t ne, const int64_t* __restrict__ off, const T* __restrict__ tails,
-    const uint8_t* __restrict__ oflags, const int64_t* __restrict__ ooff,
+    const uint8_t* __restrict__ oflags, const int64_t* __restrict__ ooff, const OT* __restrict__ goff,
     const int32_t* __restrict__ ot, int light_cut, uint64_t* __restrict__ counters,
     uint32_t* __restrict__ itcnt, Item* __restrict__ items, unsigned long long* __restrict__ pv,
     const int32_t* __restrict__ gid, unsigned long long* __restrict__ gtotal) {
@@ -167,16 +169,23 @@ __global__ __launch_bounds__(kBlock) void k_light_items(
     int64_t ids[kPerThread];
     load_run<T>(tails, e0, ee, ids);
     const uint32_t fl = oflags[e0 / kPerThread];
-    // The eight neighbours' oriented offsets {ooff[v], ooff[v+1]} are gathered
-    // before any is used: eight 16-byte loads in flight per thread.
+    // The eight neighbours' oriented offsets {goff[v], goff[v+1]} are gathered
+    // before any is used: eight loads in flight per thread.  goff is ooff
+    // itself or, when |E+| < 2^32, its 32-bit copy (half the gathered array).
     int64_t gb[kPerThread], ge[kPerThread];
 #pragma unroll
     for (int j = 0; j < kPerThread; j++) {
       const int64_t v = ids[j];
       const bool ok = e0 + j < ee && v >= 0 && v < nv;
-      const v2i64_a8 t = *reinterpret_cast<const v2i64_a8*>(ooff + (ok ? v : 0));
-      gb[j] = t.x;
-      ge[j] = t.y;
+      if constexpr (sizeof(OT) == 8) {
+        const v2i64_a8 t = *reinterpret_cast<const v2i64_a8*>(goff + (ok ? v : 0));
+        gb[j] = t.x;
+        ge[j] = t.y;
+      } else {
+        const v2u32_a4 t = *reinterpret_cast<const v2u32_a4*>(goff + (ok ? v : 0));
+        gb[j] = t.x;
+        ge[j] = t.y;
+      }
     }
     int64_t u = tile_row_of(r, s_off, off, e0);
     int64_t nxt = tile_off(r, s_off, off, u + 1);
@@ -270,6 +279,14 @@ __global__ __launch_bounds__(kBlock) void k_light_items(
   __syncthreads();
   const uint64_t bl = block_sum_u64<kBlock>(listed, s_red);
   if (threadIdx.x == 0 && bl) atomicAdd((unsigned long long*)&counters[kHeavyItems], (unsigned long long)bl);
+}
+
+// 32-bit copy of the oriented offsets (used when |E+| < 2^32).
+__global__ __launch_bounds__(256) void k_narrow_offsets(int64_t n, const int64_t* __restrict__ ooff,
+                                                        uint32_t* __restrict__ o32) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    o32[i] = (uint32_t)ooff[i];
 }
 
 __global__ __launch_bounds__(256) void k_tile_counts(int64_t nv, const uint32_t* __restrict__ itcnt,
@@ -710,20 +727,35 @@ int count_oriented(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne_bound, const 
   unsigned long long* pvp = (unsigned long long*)pv;
   unsigned long long* gt = (unsigned long long*)gtotal;
   const int cut = ctx->small_cut;
-  int64_t g = (nv + 255) / 256;
+  int64_t g = (nv + 1 + 255) / 256;
   if (g > 65536) g = 65536;
-#define XT_LIGHT(P, B)                                                                              \
-  hipLaunchKernelGGL((k_light_items<T, P, B>), dim3((unsigned)nchunks), dim3(kBlock), 0, st, nv,     \
-                     ne_bound, in_off, in_tails, oflags, ooff, ot, cut, counters,                   \
+  const bool narrow = ne_bound < (1ll << 32);  // every oriented offset fits 32 bits
+  uint32_t* o32 = nullptr;
+  if (narrow) {
+    XT_TRY(ensure(ctx, s.ooff32, (size_t)(nv + 1) * sizeof(uint32_t)));
+    o32 = s.ooff32.as<uint32_t>();
+    hipLaunchKernelGGL(k_narrow_offsets, dim3((unsigned)g), dim3(256), 0, st, nv + 1, ooff, o32);
+    XT_HIP(ctx, hipGetLastError());
+  }
+#define XT_LIGHT(OT, GO, P, B)                                                                      \
+  hipLaunchKernelGGL((k_light_items<T, OT, P, B>), dim3((unsigned)nchunks), dim3(kBlock), 0, st, nv, \
+                     ne_bound, in_off, in_tails, oflags, ooff, (const OT*)(GO), ot, cut, counters, \
                      s.itcnt.as<uint32_t>(), s.items.as<Item>(), pvp, gid, gt)
-  if (pv && batch)
-    XT_LIGHT(true, true);
-  else if (pv)
-    XT_LIGHT(true, false);
-  else if (batch)
-    XT_LIGHT(false, true);
-  else
-    XT_LIGHT(false, false);
+#define XT_LIGHT_PB(OT, GO)      \
+  if (pv && batch)               \
+    XT_LIGHT(OT, GO, true, true); \
+  else if (pv)                   \
+    XT_LIGHT(OT, GO, true, false); \
+  else if (batch)                \
+    XT_LIGHT(OT, GO, false, true); \
+  else                           \
+    XT_LIGHT(OT, GO, false, false);
+  if (narrow) {
+    XT_LIGHT_PB(uint32_t, o32);
+  } else {
+    XT_LIGHT_PB(int64_t, ooff);
+  }
+#undef XT_LIGHT_PB
 #undef XT_LIGHT
   XT_HIP(ctx, hipGetLastError());
   if (evs) XT_HIP(ctx, hipEventRecord(evs[0], st));
@@ -796,6 +828,7 @@ int reserve(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne, bool per_vertex) {
   XT_TRY(ensure(ctx, s.items, (size_t)nee * sizeof(Item)));
   XT_TRY(ensure(ctx, s.tileoff, (size_t)(nvv + 1) * sizeof(int64_t)));
   XT_TRY(ensure(ctx, s.tiledesc, (size_t)tiles_bound(nvv, nee) * sizeof(TileDesc)));
+  if (nee < (1ll << 32)) XT_TRY(ensure(ctx, s.ooff32, (size_t)(nvv + 1) * sizeof(uint32_t)));
   const size_t st = scan_tmp_bytes(nvv) > scan_tmp_bytes(nchunks) ? scan_tmp_bytes(nvv) : scan_tmp_bytes(nchunks);
   XT_TRY(ensure(ctx, s.scan_tmp, st));
   if (per_vertex) XT_TRY(ensure(ctx, s.pv, (size_t)nvv * sizeof(uint64_t)));
@@ -804,7 +837,7 @@ int reserve(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne, bool per_vertex) {
 
 void release(Scratch& s) {
   DevBuf* bufs[] = {&s.deg, &s.dplus, &s.tailcnt, &s.qscan, &s.ooff, &s.otails, &s.counters,
-                    &s.oflags, &s.itcnt, &s.items, &s.tileoff, &s.tiledesc, &s.pv, &s.scan_tmp, &s.in_off,
+                    &s.oflags, &s.itcnt, &s.items, &s.tileoff, &s.tiledesc, &s.ooff32, &s.pv, &s.scan_tmp, &s.in_off,
                     &s.in_tails};
   for (DevBuf* b : bufs) {
     if (b->p) (void)hipFree(b->p);
