@@ -149,12 +149,16 @@ template <typename T, typename OT, bool PV, bool BATCH>
 __global__ __launch_bounds__(kBlock) void k_light_items(
     int64_t nv, int64_t ne, const int64_t* __restrict__ off, const T* __restrict__ tails,
     const uint8_t* __restrict__ oflags, const int64_t* __restrict__ ooff, const OT* __restrict__ goff,
-    const int32_t* __restrict__ ot, int light_cut, uint64_t* __restrict__ counters,
-    uint32_t* __restrict__ itcnt, Item* __restrict__ items, unsigned long long* __restrict__ pv,
-    const int32_t* __restrict__ gid, unsigned long long* __restrict__ gtotal) {
+    const uint16_t* __restrict__ dp16, const int32_t* __restrict__ ot, int light_cut,
+    uint64_t* __restrict__ counters, uint32_t* __restrict__ itcnt, Item* __restrict__ items,
+    unsigned long long* __restrict__ pv, const int32_t* __restrict__ gid,
+    unsigned long long* __restrict__ gtotal) {
   __shared__ int64_t s_off[kOffCap + 1];  // input offsets of the chunk's rows
   __shared__ int64_t s_lohi[2];
   __shared__ uint64_t s_red[kBlock / kWave];
+  __shared__ int64_t s_scan[kBlock / kWave];
+  __shared__ uint16_t s_rank[kChunk + 1];    // owned entries before each entry of the chunk
+  __shared__ uint32_t s_bb[2];               // reserved starts of the first and last row's items
   __shared__ unsigned long long s_g[BATCH ? kGraphSlots : 1];
   const int64_t eb = (int64_t)blockIdx.x * kChunk;
   if (eb >= ne) return;  // whole workgroup leaves together
@@ -163,73 +167,132 @@ __global__ __launch_bounds__(kBlock) void k_light_items(
   const TileRows r = tile_rows(off, nv, eb, ee, s_off, s_lohi);
   const int32_t g0 = BATCH ? gid[r.ulo] : 0;
 
-  uint64_t hits_all = 0, listed = 0;
+  uint64_t hits_all = 0;
   const int64_t e0 = eb + (int64_t)threadIdx.x * kPerThread;
+  int32_t ids[kPerThread];  // valid ids are below nv < 2^31
+  uint32_t valid = 0, light = 0, own = 0;
+  int64_t gb[kPerThread], ge[kPerThread];
+  int64_t u0 = 0;
   if (e0 < ee) {
-    int64_t ids[kPerThread];
-    load_run<T>(tails, e0, ee, ids);
-    const uint32_t fl = oflags[e0 / kPerThread];
-    // The eight neighbours' oriented offsets {goff[v], goff[v+1]} are gathered
-    // before any is used: eight loads in flight per thread.  goff is ooff
-    // itself or, when |E+| < 2^32, its 32-bit copy (half the gathered array).
-    int64_t gb[kPerThread], ge[kPerThread];
+    {
+      int64_t raw[kPerThread];
+      load_run<T>(tails, e0, ee, raw);
 #pragma unroll
-    for (int j = 0; j < kPerThread; j++) {
-      const int64_t v = ids[j];
-      const bool ok = e0 + j < ee && v >= 0 && v < nv;
-      if constexpr (sizeof(OT) == 8) {
-        const v2i64_a8 t = *reinterpret_cast<const v2i64_a8*>(goff + (ok ? v : 0));
-        gb[j] = t.x;
-        ge[j] = t.y;
-      } else {
-        const v2u32_a4 t = *reinterpret_cast<const v2u32_a4*>(goff + (ok ? v : 0));
-        gb[j] = t.x;
-        ge[j] = t.y;
+      for (int j = 0; j < kPerThread; j++) {
+        const bool ok = e0 + j < ee && raw[j] >= 0 && raw[j] < nv;
+        valid |= (uint32_t)ok << j;
+        ids[j] = ok ? (int32_t)raw[j] : 0;
       }
     }
-    int64_t u = tile_row_of(r, s_off, off, e0);
+    const uint32_t fl = oflags[e0 / kPerThread];
+    // 1. The eight neighbours' out-degrees, saturated at 65535: 2-byte gathers
+    //    from an array small enough to stay in the device's last-level cache.
+    //    Both rows of an edge see the same two saturated values, so exactly
+    //    one of them owns it.
+    int dv16[kPerThread];
+#pragma unroll
+    for (int j = 0; j < kPerThread; j++) dv16[j] = dp16[ids[j]];
+    // 2. Classify along the rows: light out-edges are intersected here; heavy
+    //    edges owned by this row are listed.
+    u0 = tile_row_of(r, s_off, off, e0);
+    {
+      int64_t u = u0;
+      int64_t nxt = tile_off(r, s_off, off, u + 1);
+      int du16 = (int)min(ooff[u + 1] - ooff[u], (int64_t)65535);
+#pragma unroll
+      for (int j = 0; j < kPerThread; j++) {
+        const int64_t e = e0 + j;
+        if (e >= ee) break;
+        while (e >= nxt) {
+          ++u;
+          nxt = tile_off(r, s_off, off, u + 1);
+          du16 = (int)min(ooff[u + 1] - ooff[u], (int64_t)65535);
+        }
+        if (!((valid >> j) & 1u) || ids[j] == u) continue;  // out of range (unvalidated input) or self-loop
+        const bool kept = (fl >> j) & 1u;
+        const int du = du16, dv = dv16[j];
+        if (du > light_cut || dv > light_cut) {  // heavy: listed under the longer list's owner
+          if ((kept ? du >= dv : du > dv) && dv > 0) own |= 1u << j;  // an empty N+(v) closes nothing
+        } else if (kept && dv > 0) {  // light in-edges are counted from their source row
+          light |= 1u << j;
+        }
+      }
+    }
+    // 3. Lists of the light and owned partners (about half the entries),
+    //    gathered together.
+#pragma unroll
+    for (int j = 0; j < kPerThread; j++) {
+      gb[j] = ge[j] = 0;
+      if (((light | own) >> j) & 1u) {
+        if constexpr (sizeof(OT) == 8) {
+          const v2i64_a8 t = *reinterpret_cast<const v2i64_a8*>(goff + ids[j]);
+          gb[j] = t.x;
+          ge[j] = t.y;
+        } else {
+          const v2u32_a4 t = *reinterpret_cast<const v2u32_a4*>(goff + ids[j]);
+          gb[j] = t.x;
+          ge[j] = t.y;
+        }
+      }
+    }
+  }
+  // 4. Item slots without per-row atomics: a block scan ranks the chunk's
+  //    owned entries in entry order.  A row inside the chunk writes its items
+  //    at off[x] + (rank within the row) and its count directly; only the
+  //    chunk's first and last rows, which may continue in a neighbouring chunk,
+  //    reserve their range with one atomic each.
+  int64_t nown = 0;
+  const int64_t pre = block_excl_scan_i64<kBlock>((int64_t)__popc(own), s_scan, &nown);
+  if (e0 < ee) {
+#pragma unroll
+    for (int j = 0; j < kPerThread; j++)
+      if (e0 + j < ee) s_rank[e0 + j - eb] = (uint16_t)(pre + __popc(own & ((1u << j) - 1u)));
+  }
+  if (threadIdx.x == 0) s_rank[ee - eb] = (uint16_t)nown;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    auto reserve_row = [&](int64_t uu, int slot) {
+      const int64_t ro = tile_off(r, s_off, off, uu), re = tile_off(r, s_off, off, uu + 1);
+      s_bb[slot] = 0;
+      if (ro < eb || re > ee) {  // continues into a neighbouring chunk
+        const int cnt = s_rank[min(re, ee) - eb] - s_rank[max(ro, eb) - eb];
+        if (cnt) s_bb[slot] = atomicAdd(&itcnt[uu], (uint32_t)cnt);
+      }
+    };
+    reserve_row(r.ulo, 0);
+    if (r.uhi != r.ulo) reserve_row(r.uhi, 1);
+  }
+  __syncthreads();
+  if (e0 < ee) {
+    // 5. Along the rows again: write the owned items and intersect the light edges.
+    int64_t u = u0;
+    int64_t cur = tile_off(r, s_off, off, u);
     int64_t nxt = tile_off(r, s_off, off, u + 1);
     int64_t a0 = ooff[u], a1 = ooff[u + 1];
-    uint32_t own = 0;  // entries of this run that row u owns
-    int64_t vb[kPerThread];
-    int32_t vl[kPerThread];
-    auto flush = [&]() {
-      if (!own) return;
-      const uint32_t cnt = (uint32_t)__popc(own);
-      listed += cnt;
-      const int64_t base = tile_off(r, s_off, off, u) + (int64_t)atomicAdd(&itcnt[u], cnt);
-      uint32_t rank = 0;
-#pragma unroll
-      for (int jj = 0; jj < kPerThread; jj++)
-        if ((own >> jj) & 1u) items[base + rank++] = Item{vb[jj], vl[jj], (int32_t)ids[jj]};
-      own = 0;
-    };
 #pragma unroll
     for (int j = 0; j < kPerThread; j++) {
       const int64_t e = e0 + j;
       if (e >= ee) break;
       while (e >= nxt) {
-        flush();
         ++u;
+        cur = nxt;
         nxt = tile_off(r, s_off, off, u + 1);
         a0 = a1;
         a1 = ooff[u + 1];
       }
-      const int64_t v = ids[j];
-      if (v < 0 || v >= nv || v == u) continue;  // out of range (unvalidated input) or self-loop
-      const bool kept = (fl >> j) & 1u;
-      const int64_t b0 = gb[j], b1 = ge[j];
-      const int64_t du = a1 - a0, dv = b1 - b0;
-      if (du > light_cut || dv > light_cut) {  // heavy: listed under the longer list's owner
-        if ((kept ? du >= dv : du > dv) && dv > 0) {  // an empty N+(v) closes nothing
-          own |= 1u << j;
-          vb[j] = b0;
-          vl[j] = (int32_t)dv;
-        }
-        continue;
+      const bool inner = cur >= eb && nxt <= ee;
+      if (inner && e == cur) {  // the row's first entry publishes its item count
+        const int cnt = s_rank[nxt - eb] - s_rank[cur - eb];
+        if (cnt) itcnt[u] = (uint32_t)cnt;
       }
-      if (!kept) continue;  // light in-edge: counted from its source row
-      const EdgeLists L = edge_lists(a0, a1, b0, b1);
+      if ((own >> j) & 1u) {
+        const int64_t rank = s_rank[e - eb] - s_rank[max(cur, eb) - eb];
+        const int64_t extra = inner ? 0 : (u == r.ulo ? (int64_t)s_bb[0] : (int64_t)s_bb[1]);
+        items[cur + extra + rank] = Item{gb[j], (int32_t)(ge[j] - gb[j]), ids[j]};
+      }
+      if (!((light >> j) & 1u)) continue;
+      const int64_t v = ids[j];
+      const EdgeLists L = edge_lists(a0, a1, gb[j], ge[j]);
       if (L.s1 == L.s0) continue;
       const int64_t mmin = u < v ? u : v;
       uint64_t hits = 0, at_min = 0;
@@ -266,7 +329,6 @@ __global__ __launch_bounds__(kBlock) void k_light_items(
         hits_all += hits;
       }
     }
-    flush();
   }
   if constexpr (BATCH) {
     __syncthreads();
@@ -276,9 +338,17 @@ __global__ __launch_bounds__(kBlock) void k_light_items(
     const uint64_t bsum = block_sum_u64<kBlock>(hits_all, s_red);
     if (threadIdx.x == 0 && bsum) atomicAdd((unsigned long long*)&counters[kTotal], (unsigned long long)bsum);
   }
-  __syncthreads();
-  const uint64_t bl = block_sum_u64<kBlock>(listed, s_red);
-  if (threadIdx.x == 0 && bl) atomicAdd((unsigned long long*)&counters[kHeavyItems], (unsigned long long)bl);
+  if (threadIdx.x == 0 && nown) atomicAdd((unsigned long long*)&counters[kHeavyItems], (unsigned long long)nown);
+}
+
+// Out-degrees saturated at 65535, from the oriented offsets.
+__global__ __launch_bounds__(256) void k_dplus16(int64_t nv, const int64_t* __restrict__ ooff,
+                                                 uint16_t* __restrict__ dp16) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < nv; x += stride) {
+    const int64_t d = ooff[x + 1] - ooff[x];
+    dp16[x] = (uint16_t)(d < 65535 ? d : 65535);
+  }
 }
 
 // 32-bit copy of the oriented offsets (used when |E+| < 2^32).
@@ -455,10 +525,11 @@ __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(W
       nc[j] = 0;
       if (k < nitems) {
         const Item it = items[i0 + k];
+        const int len = it.len;
         s_base[k] = it.base;
-        s_len[k] = it.len;
+        s_len[k] = len;
         if constexpr (PV) s_y[k] = it.y;
-        nc[j] = (it.len + 3) >> 2;
+        nc[j] = (len + 3) >> 2;
         nchunks += nc[j];
       }
     }
@@ -737,10 +808,16 @@ int count_oriented(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne_bound, const 
     hipLaunchKernelGGL(k_narrow_offsets, dim3((unsigned)g), dim3(256), 0, st, nv + 1, ooff, o32);
     XT_HIP(ctx, hipGetLastError());
   }
+  // s.deg (the orientation's degree keys until now) takes the 16-bit out-degrees,
+  // later the tile counts.
+  uint16_t* dp16 = s.deg.as<uint16_t>();
+  hipLaunchKernelGGL(k_dplus16, dim3((unsigned)g), dim3(256), 0, st, nv, ooff, dp16);
+  XT_HIP(ctx, hipGetLastError());
 #define XT_LIGHT(OT, GO, P, B)                                                                      \
   hipLaunchKernelGGL((k_light_items<T, OT, P, B>), dim3((unsigned)nchunks), dim3(kBlock), 0, st, nv, \
-                     ne_bound, in_off, in_tails, oflags, ooff, (const OT*)(GO), ot, cut, counters, \
-                     s.itcnt.as<uint32_t>(), s.items.as<Item>(), pvp, gid, gt)
+                     ne_bound, in_off, in_tails, oflags, ooff, (const OT*)(GO),                    \
+                     (const uint16_t*)dp16, ot, cut, counters, s.itcnt.as<uint32_t>(),           \
+                     s.items.as<Item>(), pvp, gid, gt)
 #define XT_LIGHT_PB(OT, GO)      \
   if (pv && batch)               \
     XT_LIGHT(OT, GO, true, true); \

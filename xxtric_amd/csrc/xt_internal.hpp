@@ -62,7 +62,7 @@ struct Scratch {
   DevBuf counters;  // uint64[16]          total, tile queue, input-error flags, statistics
   DevBuf oflags;    // uint8[ne/8]         kept-entry mask per 8 CSR entries (orientation)
   DevBuf itcnt;     // uint32[nv]          heavy edges owned by each vertex
-  DevBuf items;     // 16 B[ne]            item lists: owner x's {offset, length, id} records from items[off[x]]
+  DevBuf items;     // 16 B[ne]            item lists: owner x's {ooff[y], d+(y), y} records from items[off[x]]
   DevBuf tileoff;   // int64[nv+1]         exclusive scan of tiles per owner
   DevBuf tiledesc;  // 32 B[nv+ne/256+1]  {first item, ooff[x], x, items, d+(x)} of each tile
   DevBuf pv;        // uint64[nv]          per-vertex counts
