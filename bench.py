@@ -197,7 +197,9 @@ def main():
     ap.add_argument("--scale", type=int, default=26)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-extras", action="store_true")
-    ap.add_argument("--serve-rate", type=float, default=20000.0)
+    ap.add_argument("--serve-rate", type=float, default=5000.0,
+                    help="offered Poisson arrival rate (graphs/s) of the serving line; "
+                         "--serve-sweep measures capacity")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the 1 GiB copy kernel used to scale PMC byte counters")
     ap.add_argument("--serve-sweep", action="store_true",
