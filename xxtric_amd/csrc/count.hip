@@ -7,18 +7,21 @@
 // Edges are binned by the longer of their two out-lists:
 //  * light edges (both lists <= light_cut): one thread per edge walks the
 //    shorter list and searches each id in the longer one (lower_bound restarted
-//    after the previous hit) -- k_count_light, which also classifies the rest;
+//    after the previous hit);
 //  * heavy edges: the edge belongs to its endpoint x with the longer list (ties
 //    to the source).  Each such x gets an item list -- one 16-byte record
-//    {offset, length, id} per other endpoint y of its heavy edges
-//    (k_fill_items).  The item lists are cut into tiles of up to 256 items with
-//    one descriptor each (k_tile_desc).  k_count_groups, a persistent kernel,
-//    draws tiles from a device work queue, stages N+(x) once per tile in an LDS
-//    hash table of 4-slot buckets, and streams the items' shorter lists N+(y)
-//    through it: the tile's lists are concatenated into one run of 16-byte
-//    chunks that all 512 threads read with coalesced loads, the next round of
-//    loads in flight while the current one is probed.  Each oriented edge reads
-//    only its shorter list; the longer one is read once per tile.
+//    {offset, length, id} per other endpoint y of its heavy edges.
+// Both happen in one pass over the symmetric input rows (k_light_items): entry
+// (x, w) of row x is an out-edge or an in-edge of x, so every oriented edge is
+// seen from both of its rows and is counted or listed from exactly one of them.
+// The item lists are cut into tiles of up to 256 items with one descriptor each
+// (k_tile_desc).  k_count_groups, a persistent kernel, draws tiles from a
+// device work queue, stages N+(x) once per tile in an LDS hash table of 4-slot
+// buckets, and streams the items' shorter lists N+(y) through it: the tile's
+// lists are concatenated into one run of 16-byte chunks that all 512 threads
+// read with coalesced loads, the next round of loads in flight while the
+// current one is probed.  Each oriented edge reads only its shorter list; the
+// longer one is read once per tile.
 // Reductions: per-thread uint64 -> wave64 shuffle tree -> LDS -> one 64-bit
 // atomic per workgroup (per tile for the group kernel).
 //
