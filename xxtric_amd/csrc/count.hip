@@ -53,6 +53,9 @@ struct alignas(16) Item {
 
 // Four ids at a 4-byte aligned address (one 16-byte load on gfx950).
 typedef int v4i_a4 __attribute__((ext_vector_type(4), aligned(4)));
+// The same in the global address space (a load through an address computed as
+// an integer stays a global load, not a flat one).
+typedef const __attribute__((address_space(1))) v4i_a4 gv4i_a4;
 // Two consecutive offsets at an 8-byte aligned address (one 16-byte load).
 typedef long long v2i64_a8 __attribute__((ext_vector_type(2), aligned(8)));
 // Two consecutive 32-bit offsets at a 4-byte aligned address (one 8-byte load).
@@ -398,11 +401,11 @@ __global__ __launch_bounds__(256) void k_tile_desc(int64_t nv, const int64_t* __
 // padding id -1 matches nothing.
 constexpr int32_t kEmpty = 0x7fffffff;
 
-// Byte offset of w's home bucket: (w xor (w >> 11)) mod 2048, times 16 -- two
-// shifts and one three-input bitwise op.
+// Byte offset of w's home bucket: w mod 2048, times 16 (a shift and a mask).
+// Ids that are close together, the usual shape of a neighbourhood, fall into
+// distinct buckets; ids that agree in their low 11 bits share one and spill.
 __device__ __forceinline__ uint32_t bucket_addr(int32_t w) {
-  const uint32_t u = (uint32_t)w;
-  return ((u << 4) ^ (u >> (kTabLog - 4))) & ((uint32_t)(kTabBuckets - 1) << 4);
+  return ((uint32_t)w << 4) & ((uint32_t)(kTabBuckets - 1) << 4);
 }
 
 __device__ __forceinline__ const int4& bucket_at(const int4* tab, uint32_t addr) {
@@ -451,10 +454,11 @@ __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(W
     unsigned long long* __restrict__ pv, const int32_t* __restrict__ gid,
     unsigned long long* __restrict__ gtotal) {
   __shared__ int4 s_tab[kTabBuckets];
-  __shared__ int64_t s_base[kItemsPerTile];      // first id of each item list
-  __shared__ int32_t s_len[kItemsPerTile];       // its length
+  // Per item k: {adj (64-bit), end, y}.  Chunk c of item k (16 bytes) starts at
+  // byte address adj + 16*c and holds end - 4*c ids of the list (mod 2^32;
+  // <= 0 past the item's last chunk).
+  __shared__ int4 s_rec[kItemsPerTile];
   __shared__ int32_t s_pre[kItemsPerTile + 1];   // exclusive scan of the items' chunk counts
-  __shared__ int32_t s_y[PV ? kItemsPerTile : 1];
   __shared__ uint16_t s_lut[kLutBlocks];          // item holding chunk 64*b
   __shared__ unsigned long long s_starts[kLutBlocks];  // items starting inside block b (bit = offset)
   __shared__ int64_t s_scan[kGroupThreads / kWave];
@@ -466,7 +470,7 @@ __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(W
   constexpr int kJ = (kItemsPerTile + kT - 1) / kT;  // items per thread in the tile set-up
   const int tid = threadIdx.x;
   const int lane = tid & (kWave - 1);
-  const int wave = tid / kWave;
+  const int wave = __builtin_amdgcn_readfirstlane(tid / kWave);  // uniform per wave
   // Tile indices fit 32 bits: there are fewer than nv + ne/256 + 1 < 2^32 tiles.
   const uint32_t ntiles = (uint32_t)*ntiles_p;
   unsigned int* queue = reinterpret_cast<unsigned int*>(&counters[kTileQueue]);
@@ -521,18 +525,15 @@ __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(W
     // Items k = tid*kJ + j: each list is streamed as ceil(len/4) 16-byte chunks
     // from its first id (dword-aligned loads; the last chunk is masked).
     int32_t nc[kJ];
+    Item it[kJ];
     int32_t nchunks = 0;
 #pragma unroll
     for (int j = 0; j < kJ; j++) {
       const int k = tid * kJ + j;
       nc[j] = 0;
       if (k < nitems) {
-        const Item it = items[i0 + k];
-        const int len = it.len;
-        s_base[k] = it.base;
-        s_len[k] = len;
-        if constexpr (PV) s_y[k] = it.y;
-        nc[j] = (len + 3) >> 2;
+        it[j] = items[i0 + k];
+        nc[j] = (it[j].len + 3) >> 2;
         nchunks += nc[j];
       }
     }
@@ -544,7 +545,12 @@ __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(W
 #pragma unroll
     for (int j = 0; j < kJ; j++) {
       const int k = tid * kJ + j;
-      if (k < nitems) s_pre[k] = (int32_t)pre;
+      if (k < nitems) {
+        s_pre[k] = (int32_t)pre;
+        const uint64_t adj = (uint64_t)(uintptr_t)(ot + it[j].base) - 16ull * (uint64_t)pre;
+        s_rec[k] = make_int4((int32_t)(uint32_t)adj, (int32_t)(uint32_t)(adj >> 32),
+                             (int32_t)((uint32_t)it[j].len + 4u * (uint32_t)pre), it[j].y);
+      }
       pre += nc[j];
     }
     if (tid == 0) s_pre[nitems] = (int32_t)nch_all;
@@ -593,18 +599,15 @@ __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(W
 #pragma unroll
       for (int q = 0; q < RC; q++) {
         const int b = blk(round, q);
-        int kk = nitems - 1, cc = nchunk - 1;
-        rq[q] = 0;
-        if (b < nblk) {  // uniform per wave
-          const int c = b * kWave + lane;
-          cc = c < nchunk ? c : nchunk - 1;
-          kk = lut ? (int)s_lut[b] + __popcll(s_starts[b] & upto) : search(cc);
-          if (c >= nchunk) kk = nitems - 1;
-          rq[q] = c < nchunk ? s_len[kk] - 4 * (cc - s_pre[kk]) : 0;  // ids of this chunk in the list
-        }
-        const int cl = cc - s_pre[kk];
+        const int c = b * kWave + lane;
+        const int cc = c < nchunk ? c : nchunk - 1;  // past the end: the last chunk again
+        int kk = nitems - 1;
+        if (b < nblk) kk = lut ? (int)s_lut[b] + __popcll(s_starts[b] & upto) : search(cc);  // uniform
+        const int4 rec = s_rec[kk];
         kq[q] = kk;
-        const v4i_a4 t = *reinterpret_cast<const v4i_a4*>(ot + s_base[kk] + 4 * (int64_t)cl);
+        rq[q] = (int32_t)((uint32_t)rec.z - 4u * (uint32_t)c);  // <= 0 past the end
+        const uint64_t adj = ((uint64_t)(uint32_t)rec.y << 32) | (uint32_t)rec.x;
+        const v4i_a4 t = *reinterpret_cast<const gv4i_a4*>(adj + 16ull * (uint32_t)cc);
         vq[q] = make_int4(t.x, t.y, t.z, t.w);
       }
     };
@@ -619,10 +622,13 @@ __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(W
 
     const bool spl = s_spilled != 0;  // uniform: read after the table barrier
     uint32_t hits = 0, at_x = 0;  // below 2^32: a tile streams fewer than 2^33 ids over 512 threads
+    // Hits counted per wave (the same value in every lane): below 2^32, since the
+    // waves take the tile's chunks in turns and it streams fewer than 2^33 ids.
+    uint32_t hits_w = 0;
     auto credit = [&](int32_t w, int k) {  // a hit, with its per-vertex credit
       hits++;
       if constexpr (PV) {
-        const int64_t yy = s_y[k];
+        const int64_t yy = s_rec[k].w;
         int64_t m = x < yy ? x : yy;
         if ((int64_t)w < m) m = w;
         if (m == x)
@@ -671,6 +677,37 @@ __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(W
           if ((hitm >> j) & 1u) credit(w[j], k);
       }
     };
+    // Counting without per-vertex credit: each id is compared with the four
+    // slots of its home bucket as raw words; the compare results are lane
+    // masks whose population counts are summed per wave.  Slot 3 of a spilled
+    // bucket carries the mark, so an id there compares unequal -- like every
+    // other miss in a spilled bucket it goes to the continuation, which tests
+    // the bucket again with the mark removed and walks on.
+    auto probe_count = [&](const int4 v, int r) {
+      const int32_t w[4] = {v.x, v.y, v.z, v.w};
+      int4 t[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) t[j] = bucket_at(s_tab, bucket_addr(w[j]));
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const uint64_t hit = (__builtin_amdgcn_ballot_w64(t[j].x == w[j]) | __builtin_amdgcn_ballot_w64(t[j].y == w[j]) |
+                              __builtin_amdgcn_ballot_w64(t[j].z == w[j]) | __builtin_amdgcn_ballot_w64(t[j].w == w[j])) &
+                             __builtin_amdgcn_ballot_w64(j < r);
+        hits_w += (uint32_t)__popcll(hit);
+      }
+      // A lane whose four buckets include a spilled one takes the continuation
+      // (only tables with a spilled bucket can have one; uniform test).
+      if (spl && __builtin_amdgcn_ballot_w64((t[0].w | t[1].w | t[2].w | t[3].w) < 0)) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          if (j >= r || t[j].w >= 0) continue;
+          if ((t[j].w & 0x7fffffff) == w[j])
+            hits++;  // in slot 3 of a spilled bucket: the raw compare missed it
+          else if (bucket_miss(t[j], w[j]) != 0 && tab_continue(s_tab, bucket_addr(w[j]) >> 4, w[j]))
+            hits++;
+        }
+      }
+    };
     // Lists too long for the LDS hash: binary search in N+(x) in global memory.
     auto probe_search = [&](const int4 v, int k, int r) {
       const int32_t w[4] = {r > 0 ? v.x : -1, r > 1 ? v.y : -1, r > 2 ? v.z : -1, r > 3 ? v.w : -1};
@@ -682,6 +719,12 @@ __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(W
       }
     };
 
+    auto probe = [&](const int4 v, int k, int r) {
+      if constexpr (PV)
+        probe_hash(v, k, r);
+      else
+        probe_count(v, r);
+    };
     // Two register buffers alternate by hand (no loop-carried copies) and the
     // next round is always issued (past the end it re-reads the last chunk,
     // masked), so every round waits only for its own loads while the next
@@ -691,12 +734,12 @@ __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(W
         locate(r0 + 1, kB, vB, rB);
 #pragma unroll
         for (int q = 0; q < RC; q++)
-          if (blk(r0, q) < nblk) probe_hash(vA[q], kA[q], rA[q]);  // uniform per wave
+          if (blk(r0, q) < nblk) probe(vA[q], kA[q], rA[q]);  // uniform per wave
         if ((r0 + 1) * kRound >= nchunk) break;  // uniform
         locate(r0 + 2, kA, vA, rA);
 #pragma unroll
         for (int q = 0; q < RC; q++)
-          if (blk(r0 + 1, q) < nblk) probe_hash(vB[q], kB[q], rB[q]);
+          if (blk(r0 + 1, q) < nblk) probe(vB[q], kB[q], rB[q]);
         if ((r0 + 2) * kRound >= nchunk) break;  // uniform
       }
     } else if (!hashed) {
@@ -708,7 +751,7 @@ __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(W
       }
     }
 
-    const uint64_t bh = block_sum_u64<kT>((uint64_t)hits, s_red);
+    const uint64_t bh = block_sum_u64<kT>((uint64_t)hits + (lane == 0 ? (uint64_t)hits_w : 0ull), s_red);
     if constexpr (PV) {
       __syncthreads();
       const uint64_t bx = block_sum_u64<kT>((uint64_t)at_x, s_red);
