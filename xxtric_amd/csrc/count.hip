@@ -112,6 +112,22 @@ __device__ __forceinline__ TileRows tile_rows(const int64_t* __restrict__ ooff, 
   return r;
 }
 
+// Rows of input chunk c from the per-chunk row table, offsets staged in LDS.
+__device__ __forceinline__ TileRows tile_rows_at(const int64_t* __restrict__ off,
+                                                 const int64_t* __restrict__ crows, int64_t nv,
+                                                 int64_t c, int64_t* s_off) {
+  const RowSpan sp = chunk_span(crows, c, nv);
+  TileRows r;
+  r.ulo = sp.lo;
+  r.uhi = sp.hi;
+  r.span = r.uhi - r.ulo + 1;
+  r.in_lds = r.span <= kOffCap;
+  if (r.in_lds)
+    for (int64_t k = threadIdx.x; k <= r.span; k += blockDim.x) s_off[k] = off[r.ulo + k];
+  __syncthreads();
+  return r;
+}
+
 __device__ __forceinline__ int64_t tile_row_of(const TileRows& r, const int64_t* s_off,
                                                const int64_t* __restrict__ ooff, int64_t e) {
   return r.in_lds ? r.ulo + lds_row_of(s_off, (int)r.span, e) : row_of(ooff, r.ulo, r.uhi, e);
@@ -155,12 +171,11 @@ template <typename T, typename OT, bool PV, bool BATCH>
 __global__ __launch_bounds__(kBlock) void k_light_items(
     int64_t nv, int64_t ne, const int64_t* __restrict__ off, const T* __restrict__ tails,
     const uint8_t* __restrict__ oflags, const int64_t* __restrict__ ooff, const OT* __restrict__ goff,
-    const uint16_t* __restrict__ dp16, const int32_t* __restrict__ ot, int light_cut,
-    uint64_t* __restrict__ counters, uint32_t* __restrict__ itcnt, Item* __restrict__ items,
+    const uint16_t* __restrict__ dp16, const int32_t* __restrict__ ot, const int64_t* __restrict__ crows,
+    int light_cut, uint64_t* __restrict__ counters, uint32_t* __restrict__ itcnt, Item* __restrict__ items,
     unsigned long long* __restrict__ pv, const int32_t* __restrict__ gid,
     unsigned long long* __restrict__ gtotal) {
   __shared__ int64_t s_off[kOffCap + 1];  // input offsets of the chunk's rows
-  __shared__ int64_t s_lohi[2];
   __shared__ uint64_t s_red[kBlock / kWave];
   __shared__ int64_t s_scan[kBlock / kWave];
   __shared__ uint16_t s_rank[kChunk + 1];    // owned entries before each entry of the chunk
@@ -170,7 +185,7 @@ __global__ __launch_bounds__(kBlock) void k_light_items(
   if (eb >= ne) return;  // whole workgroup leaves together
   const int64_t ee = min(ne, eb + (int64_t)kChunk);
   if (BATCH && threadIdx.x < kGraphSlots) s_g[threadIdx.x] = 0;
-  const TileRows r = tile_rows(off, nv, eb, ee, s_off, s_lohi);
+  const TileRows r = tile_rows_at(off, crows, nv, (int64_t)blockIdx.x, s_off);
   const int32_t g0 = BATCH ? gid[r.ulo] : 0;
 
   uint64_t hits_all = 0;
@@ -862,7 +877,8 @@ int count_oriented(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne_bound, const 
 #define XT_LIGHT(OT, GO, P, B)                                                                      \
   hipLaunchKernelGGL((k_light_items<T, OT, P, B>), dim3((unsigned)nchunks), dim3(kBlock), 0, st, nv, \
                      ne_bound, in_off, in_tails, oflags, ooff, (const OT*)(GO),                    \
-                     (const uint16_t*)dp16, ot, cut, counters, s.itcnt.as<uint32_t>(),           \
+                     (const uint16_t*)dp16, ot, (const int64_t*)s.crows.as<int64_t>(), cut,       \
+                     counters, s.itcnt.as<uint32_t>(),                                             \
                      s.items.as<Item>(), pvp, gid, gt)
 #define XT_LIGHT_PB(OT, GO)      \
   if (pv && batch)               \
@@ -947,6 +963,7 @@ int reserve(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne, bool per_vertex) {
   XT_TRY(ensure(ctx, s.otails, (size_t)(nee + kOtailsPad) * sizeof(int32_t)));
   XT_TRY(ensure(ctx, s.counters, kNumSlots * sizeof(uint64_t)));
   XT_TRY(ensure(ctx, s.oflags, (size_t)(nchunks * kBlock)));
+  XT_TRY(ensure(ctx, s.crows, (size_t)(2 * nchunks) * sizeof(int64_t)));
   XT_TRY(ensure(ctx, s.itcnt, (size_t)nvv * sizeof(uint32_t)));
   XT_TRY(ensure(ctx, s.items, (size_t)nee * sizeof(Item)));
   XT_TRY(ensure(ctx, s.tileoff, (size_t)(nvv + 1) * sizeof(int64_t)));
@@ -960,7 +977,7 @@ int reserve(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne, bool per_vertex) {
 
 void release(Scratch& s) {
   DevBuf* bufs[] = {&s.deg, &s.dplus, &s.tailcnt, &s.qscan, &s.ooff, &s.otails, &s.counters,
-                    &s.oflags, &s.itcnt, &s.items, &s.tileoff, &s.tiledesc, &s.ooff32, &s.pv, &s.scan_tmp, &s.in_off,
+                    &s.oflags, &s.crows, &s.itcnt, &s.items, &s.tileoff, &s.tiledesc, &s.ooff32, &s.pv, &s.scan_tmp, &s.in_off,
                     &s.in_tails};
   for (DevBuf* b : bufs) {
     if (b->p) (void)hipFree(b->p);

@@ -8,7 +8,9 @@
 // Work split: the CSR entries are cut into fixed chunks of kChunk (2048)
 // entries, one 256-thread workgroup per chunk, eight contiguous entries per
 // thread.  A hub row therefore spans many workgroups instead of serialising on
-// one wave.  The rows a chunk touches are staged in LDS when they fit.
+// one wave.  The rows a chunk touches are staged in LDS when they fit; the
+// first and last row of every chunk come from a table built once per count
+// (k_chunk_rows), not from a search per workgroup.
 //
 //  k_orient_count: out-degree of every row (LDS counters, one global atomic per
 //                  row and chunk) and, per chunk, the out-entries of the chunk's
@@ -68,6 +70,23 @@ struct ChunkRows {
   bool in_lds;
 };
 
+// Rows of chunk c from the per-chunk row table, offsets staged in LDS.
+__device__ __forceinline__ ChunkRows chunk_rows_at(const int64_t* __restrict__ off,
+                                                   const int64_t* __restrict__ crows, int64_t nv,
+                                                   int64_t c, int64_t* s_off) {
+  const RowSpan sp = chunk_span(crows, c, nv);
+  ChunkRows r;
+  r.ulo = sp.lo;
+  r.uhi = sp.hi;
+  r.span = r.uhi - r.ulo + 1;
+  r.in_lds = r.span <= kOffCap;
+  if (r.in_lds)
+    for (int64_t k = threadIdx.x; k <= r.span; k += blockDim.x) s_off[k] = off[r.ulo + k];
+  __syncthreads();
+  return r;
+}
+
+// The same found by binary search (validation runs before the table exists).
 __device__ __forceinline__ ChunkRows chunk_rows(const int64_t* __restrict__ off, int64_t nv,
                                                 int64_t eb, int64_t ee, int64_t* s_off,
                                                 int64_t* s_lohi) {
@@ -98,22 +117,38 @@ __device__ __forceinline__ int64_t chunk_off(const ChunkRows& r, const int64_t* 
   return r.in_lds ? s_off[u - r.ulo] : off[u];
 }
 
+// First and last row of every kChunk-entry chunk: crows[2c] holds the row of
+// entry c*kChunk, crows[2c+1] the row of the chunk's last entry.  One thread
+// per row writes the chunk boundaries that fall inside the row, so every slot
+// is written once when the index is valid (rows outside [0, ne] are skipped).
+__global__ __launch_bounds__(256) void k_chunk_rows(int64_t nv, int64_t ne, const int64_t* __restrict__ off,
+                                                    int64_t* __restrict__ crows) {
+  const int64_t nchunks = (ne + kChunk - 1) / kChunk;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < nv; u += stride) {
+    const int64_t a = off[u], b = off[u + 1];
+    if (a < 0 || b > ne || a >= b) continue;
+    for (int64_t c = (a + kChunk - 1) / kChunk; c * kChunk < b; c++) crows[2 * c] = u;  // c*K in [a, b)
+    for (int64_t c1 = a / kChunk + 1; c1 * kChunk <= b; c1++) crows[2 * c1 - 1] = u;   // (c1*K - 1) in [a, b)
+    if (b == ne) crows[2 * nchunks - 1] = u;  // the last chunk ends at entry ne - 1
+  }
+}
+
 // ---------------------------------------------------------------------------
 template <typename T, int MODE>
 __global__ __launch_bounds__(kBlock) void k_orient_count(
     int64_t nv, int64_t ne, const int64_t* __restrict__ off, const T* __restrict__ tails,
-    const uint16_t* __restrict__ deg, uint32_t* __restrict__ dplus,
+    const uint16_t* __restrict__ deg, const int64_t* __restrict__ crows, uint32_t* __restrict__ dplus,
     uint32_t* __restrict__ tailcnt, uint64_t* __restrict__ counters,
     uint8_t* __restrict__ oflags) {
   __shared__ int64_t s_off[kOffCap + 1];
   __shared__ uint32_t s_cnt[kOffCap];
-  __shared__ int64_t s_lohi[2];
   __shared__ uint32_t s_tail;
   const int64_t c = blockIdx.x;
   const int64_t eb = c * kChunk;
   const int64_t ee = min(ne, eb + (int64_t)kChunk);
   if (threadIdx.x == 0) s_tail = 0;
-  const ChunkRows r = chunk_rows(off, nv, eb, ee, s_off, s_lohi);
+  const ChunkRows r = chunk_rows_at(off, crows, nv, c, s_off);
   if (r.in_lds)
     for (int64_t k = threadIdx.x; k < r.span; k += kBlock) s_cnt[k] = 0;
   __syncthreads();
@@ -195,15 +230,14 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_orient_fill(
     int64_t nv, int64_t ne, const int64_t* __restrict__ off, const T* __restrict__ tails,
     const uint8_t* __restrict__ oflags, const int64_t* __restrict__ ooff,
-    const int64_t* __restrict__ q, int32_t* __restrict__ otails) {
+    const int64_t* __restrict__ q, const int64_t* __restrict__ crows, int32_t* __restrict__ otails) {
   __shared__ int64_t s_off[kOffCap + 1];
   __shared__ uint16_t s_x[kChunk];
   __shared__ int64_t s_scan[kBlock / kWave];
-  __shared__ int64_t s_lohi[2];
   const int64_t c = blockIdx.x;
   const int64_t eb = c * kChunk;
   const int64_t ee = min(ne, eb + (int64_t)kChunk);
-  const ChunkRows r = chunk_rows(off, nv, eb, ee, s_off, s_lohi);
+  const ChunkRows r = chunk_rows_at(off, crows, nv, c, s_off);
 
   const int64_t e0 = eb + (int64_t)threadIdx.x * kPerThread;
   int64_t vals[kPerThread];
@@ -388,6 +422,10 @@ int orient(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne, const int64_t* off, 
   XT_TRY(ensure(ctx, s.tailcnt, (size_t)nchunks * sizeof(uint32_t)));
   XT_TRY(ensure(ctx, s.qscan, (size_t)(nchunks + 1) * sizeof(int64_t)));
   XT_TRY(ensure(ctx, s.oflags, (size_t)(nchunks * kBlock)));
+  XT_TRY(ensure(ctx, s.crows, (size_t)(2 * nchunks) * sizeof(int64_t)));
+  int64_t* crows = s.crows.as<int64_t>();
+  hipLaunchKernelGGL(k_chunk_rows, dim3(grid_for(nv, 256)), dim3(256), 0, st, nv, ne, off, crows);
+  XT_HIP(ctx, hipGetLastError());
   uint16_t* deg = nullptr;
   if (mode == kOrientDegree) {
     XT_TRY(ensure(ctx, s.deg, (size_t)nv * sizeof(uint32_t)));
@@ -400,18 +438,18 @@ int orient(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne, const int64_t* off, 
   uint8_t* oflags = s.oflags.as<uint8_t>();
   if (mode == kOrientDegree)
     hipLaunchKernelGGL((k_orient_count<T, kOrientDegree>), dim3((unsigned)nchunks), dim3(kBlock), 0,
-                       st, nv, ne, off, tails, (const uint16_t*)deg, s.dplus.as<uint32_t>(),
-                       s.tailcnt.as<uint32_t>(), counters, oflags);
+                       st, nv, ne, off, tails, (const uint16_t*)deg, (const int64_t*)crows,
+                       s.dplus.as<uint32_t>(), s.tailcnt.as<uint32_t>(), counters, oflags);
   else
     hipLaunchKernelGGL((k_orient_count<T, kOrientId>), dim3((unsigned)nchunks), dim3(kBlock), 0, st,
-                       nv, ne, off, tails, (const uint16_t*)deg, s.dplus.as<uint32_t>(),
-                       s.tailcnt.as<uint32_t>(), counters, oflags);
+                       nv, ne, off, tails, (const uint16_t*)deg, (const int64_t*)crows,
+                       s.dplus.as<uint32_t>(), s.tailcnt.as<uint32_t>(), counters, oflags);
   XT_HIP(ctx, hipGetLastError());
   XT_TRY(scan_u32_to_i64(ctx, s, s.dplus.as<uint32_t>(), nv, s.ooff.as<int64_t>(), st));
   XT_TRY(scan_u32_to_i64(ctx, s, s.tailcnt.as<uint32_t>(), nchunks, s.qscan.as<int64_t>(), st));
   hipLaunchKernelGGL((k_orient_fill<T>), dim3((unsigned)nchunks), dim3(kBlock), 0, st, nv, ne, off,
                      tails, (const uint8_t*)oflags, (const int64_t*)s.ooff.as<int64_t>(),
-                     (const int64_t*)s.qscan.as<int64_t>(), s.otails.as<int32_t>());
+                     (const int64_t*)s.qscan.as<int64_t>(), (const int64_t*)crows, s.otails.as<int32_t>());
   XT_HIP(ctx, hipGetLastError());
   return XT_OK;
 }

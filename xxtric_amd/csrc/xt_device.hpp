@@ -24,6 +24,19 @@ __device__ __forceinline__ int64_t row_of(const int64_t* __restrict__ off, int64
   return lo;
 }
 
+// First and last row of input chunk c, read from the per-chunk table that
+// k_chunk_rows builds (clamped to [0, nv), so a table built from an invalid
+// index still keeps every access in range).
+struct RowSpan {
+  int64_t lo, hi;
+};
+__device__ __forceinline__ RowSpan chunk_span(const int64_t* __restrict__ crows, int64_t c, int64_t nv) {
+  int64_t lo = crows[2 * c], hi = crows[2 * c + 1];
+  lo = lo < 0 ? 0 : (lo > nv - 1 ? nv - 1 : lo);
+  hi = hi < lo ? lo : (hi > nv - 1 ? nv - 1 : hi);
+  return RowSpan{lo, hi};
+}
+
 // Same search over an LDS copy of off[base .. base+n] (n+1 entries).
 __device__ __forceinline__ int lds_row_of(const int64_t* s_off, int n, int64_t e) {
   int lo = 0, hi = n - 1;

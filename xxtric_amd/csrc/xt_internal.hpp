@@ -61,6 +61,7 @@ struct Scratch {
   DevBuf otails;    // int32[ne+4]         oriented out-lists, ascending ids (|E+| <= ne), padded
   DevBuf counters;  // uint64[16]          total, tile queue, input-error flags, statistics
   DevBuf oflags;    // uint8[ne/8]         kept-entry mask per 8 CSR entries (orientation)
+  DevBuf crows;     // int64[2*nchunks]    first and last row of each kChunk-entry input chunk
   DevBuf itcnt;     // uint32[nv]          heavy edges owned by each vertex
   DevBuf items;     // 16 B[ne]            item lists: owner x's {ooff[y], d+(y), y} records from items[off[x]]
   DevBuf tileoff;   // int64[nv+1]         exclusive scan of tiles per owner
