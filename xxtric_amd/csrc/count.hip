@@ -58,8 +58,6 @@ typedef int v4i_a4 __attribute__((ext_vector_type(4), aligned(4)));
 typedef const __attribute__((address_space(1))) v4i_a4 gv4i_a4;
 // Two consecutive offsets at an 8-byte aligned address (one 16-byte load).
 typedef long long v2i64_a8 __attribute__((ext_vector_type(2), aligned(8)));
-// Two consecutive 32-bit offsets at a 4-byte aligned address (one 8-byte load).
-typedef unsigned int v2u32_a4 __attribute__((ext_vector_type(2), aligned(4)));
 
 // One group tile: up to kItemsPerTile items of owner x and x's own out-list
 // (32 bytes: the group kernel reads it as two int4 values in this field order).
@@ -167,10 +165,16 @@ __device__ __forceinline__ void load_ids(const int32_t* __restrict__ ot, int64_t
 // Every oriented edge is therefore either counted here (once, from its source
 // row) or listed once under its owner; in-edges and edges owned by the other
 // endpoint are skipped.
-template <typename T, typename OT, bool PV, bool BATCH>
+//
+// NARROW (|E+| < 2^32): one 8-byte gather per entry, vrec[w] = ooff[w] |
+// d+(w) << 32, classifies the entry and places its item; exact out-degrees on
+// both rows of an edge pick the same owner.  Otherwise a 2-byte gather of the
+// saturated out-degree classifies every entry (both rows see the same
+// saturated pair) and the owned and light entries gather their offsets.
+template <typename T, bool NARROW, bool PV, bool BATCH>
 __global__ __launch_bounds__(kBlock) void k_light_items(
     int64_t nv, int64_t ne, const int64_t* __restrict__ off, const T* __restrict__ tails,
-    const uint8_t* __restrict__ oflags, const int64_t* __restrict__ ooff, const OT* __restrict__ goff,
+    const uint8_t* __restrict__ oflags, const int64_t* __restrict__ ooff, const uint64_t* __restrict__ vrec,
     const uint16_t* __restrict__ dp16, const int32_t* __restrict__ ot, const int64_t* __restrict__ crows,
     int light_cut, uint64_t* __restrict__ counters, uint32_t* __restrict__ itcnt, Item* __restrict__ items,
     unsigned long long* __restrict__ pv, const int32_t* __restrict__ gid,
@@ -206,20 +210,34 @@ __global__ __launch_bounds__(kBlock) void k_light_items(
       }
     }
     const uint32_t fl = oflags[e0 / kPerThread];
-    // 1. The eight neighbours' out-degrees, saturated at 65535: 2-byte gathers
-    //    from an array small enough to stay in the device's last-level cache.
-    //    Both rows of an edge see the same two saturated values, so exactly
-    //    one of them owns it.
-    int dv16[kPerThread];
+    // 1. The eight neighbours' out-degrees (and, NARROW, their list offsets),
+    //    eight gathers in flight.
+    int64_t dvv[kPerThread];
+    if constexpr (NARROW) {
+      uint64_t rec[kPerThread];
 #pragma unroll
-    for (int j = 0; j < kPerThread; j++) dv16[j] = dp16[ids[j]];
+      for (int j = 0; j < kPerThread; j++) rec[j] = vrec[ids[j]];
+#pragma unroll
+      for (int j = 0; j < kPerThread; j++) {
+        gb[j] = (int64_t)(uint32_t)rec[j];
+        dvv[j] = (int64_t)(rec[j] >> 32);
+        ge[j] = gb[j] + dvv[j];
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < kPerThread; j++) dvv[j] = dp16[ids[j]];
+    }
     // 2. Classify along the rows: light out-edges are intersected here; heavy
     //    edges owned by this row are listed.
     u0 = tile_row_of(r, s_off, off, e0);
     {
+      auto dplus_of = [&](int64_t uu) -> int64_t {
+        const int64_t d = ooff[uu + 1] - ooff[uu];
+        return NARROW ? d : (d < 65535 ? d : 65535);
+      };
       int64_t u = u0;
       int64_t nxt = tile_off(r, s_off, off, u + 1);
-      int du16 = (int)min(ooff[u + 1] - ooff[u], (int64_t)65535);
+      int64_t du_row = dplus_of(u);
 #pragma unroll
       for (int j = 0; j < kPerThread; j++) {
         const int64_t e = e0 + j;
@@ -227,11 +245,11 @@ __global__ __launch_bounds__(kBlock) void k_light_items(
         while (e >= nxt) {
           ++u;
           nxt = tile_off(r, s_off, off, u + 1);
-          du16 = (int)min(ooff[u + 1] - ooff[u], (int64_t)65535);
+          du_row = dplus_of(u);
         }
         if (!((valid >> j) & 1u) || ids[j] == u) continue;  // out of range (unvalidated input) or self-loop
         const bool kept = (fl >> j) & 1u;
-        const int du = du16, dv = dv16[j];
+        const int64_t du = du_row, dv = dvv[j];
         if (du > light_cut || dv > light_cut) {  // heavy: listed under the longer list's owner
           if ((kept ? du >= dv : du > dv) && dv > 0) own |= 1u << j;  // an empty N+(v) closes nothing
         } else if (kept && dv > 0) {  // light in-edges are counted from their source row
@@ -240,17 +258,13 @@ __global__ __launch_bounds__(kBlock) void k_light_items(
       }
     }
     // 3. Lists of the light and owned partners (about half the entries),
-    //    gathered together.
+    //    gathered together (NARROW has them from step 1).
+    if constexpr (!NARROW) {
 #pragma unroll
-    for (int j = 0; j < kPerThread; j++) {
-      gb[j] = ge[j] = 0;
-      if (((light | own) >> j) & 1u) {
-        if constexpr (sizeof(OT) == 8) {
-          const v2i64_a8 t = *reinterpret_cast<const v2i64_a8*>(goff + ids[j]);
-          gb[j] = t.x;
-          ge[j] = t.y;
-        } else {
-          const v2u32_a4 t = *reinterpret_cast<const v2u32_a4*>(goff + ids[j]);
+      for (int j = 0; j < kPerThread; j++) {
+        gb[j] = ge[j] = 0;
+        if (((light | own) >> j) & 1u) {
+          const v2i64_a8 t = *reinterpret_cast<const v2i64_a8*>(ooff + ids[j]);
           gb[j] = t.x;
           ge[j] = t.y;
         }
@@ -372,12 +386,15 @@ __global__ __launch_bounds__(256) void k_dplus16(int64_t nv, const int64_t* __re
   }
 }
 
-// 32-bit copy of the oriented offsets (used when |E+| < 2^32).
-__global__ __launch_bounds__(256) void k_narrow_offsets(int64_t n, const int64_t* __restrict__ ooff,
-                                                        uint32_t* __restrict__ o32) {
+// Per-vertex records ooff[x] | d+(x) << 32 (used when |E+| < 2^32, so both
+// fit 32 bits): one 8-byte gather gives a neighbour's list offset and length.
+__global__ __launch_bounds__(256) void k_vertex_records(int64_t nv, const int64_t* __restrict__ ooff,
+                                                        uint64_t* __restrict__ vrec) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-    o32[i] = (uint32_t)ooff[i];
+  for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < nv; x += stride) {
+    const int64_t a = ooff[x], b = ooff[x + 1];
+    vrec[x] = (uint64_t)(uint32_t)a | ((uint64_t)(uint32_t)(b - a) << 32);
+  }
 }
 
 __global__ __launch_bounds__(256) void k_tile_counts(int64_t nv, const uint32_t* __restrict__ itcnt,
@@ -861,38 +878,38 @@ int count_oriented(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne_bound, const 
   const int cut = ctx->small_cut;
   int64_t g = (nv + 1 + 255) / 256;
   if (g > 65536) g = 65536;
-  const bool narrow = ne_bound < (1ll << 32);  // every oriented offset fits 32 bits
-  uint32_t* o32 = nullptr;
+  const bool narrow = ne_bound < (1ll << 32);  // every oriented offset and out-degree fits 32 bits
+  uint64_t* vrec = nullptr;
+  uint16_t* dp16 = nullptr;
   if (narrow) {
-    XT_TRY(ensure(ctx, s.ooff32, (size_t)(nv + 1) * sizeof(uint32_t)));
-    o32 = s.ooff32.as<uint32_t>();
-    hipLaunchKernelGGL(k_narrow_offsets, dim3((unsigned)g), dim3(256), 0, st, nv + 1, ooff, o32);
-    XT_HIP(ctx, hipGetLastError());
-  }
-  // s.deg (the orientation's degree keys until now) takes the 16-bit out-degrees,
-  // later the tile counts.
-  uint16_t* dp16 = s.deg.as<uint16_t>();
-  hipLaunchKernelGGL(k_dplus16, dim3((unsigned)g), dim3(256), 0, st, nv, ooff, dp16);
-  XT_HIP(ctx, hipGetLastError());
-#define XT_LIGHT(OT, GO, P, B)                                                                      \
-  hipLaunchKernelGGL((k_light_items<T, OT, P, B>), dim3((unsigned)nchunks), dim3(kBlock), 0, st, nv, \
-                     ne_bound, in_off, in_tails, oflags, ooff, (const OT*)(GO),                    \
-                     (const uint16_t*)dp16, ot, (const int64_t*)s.crows.as<int64_t>(), cut,       \
-                     counters, s.itcnt.as<uint32_t>(),                                             \
-                     s.items.as<Item>(), pvp, gid, gt)
-#define XT_LIGHT_PB(OT, GO)      \
-  if (pv && batch)               \
-    XT_LIGHT(OT, GO, true, true); \
-  else if (pv)                   \
-    XT_LIGHT(OT, GO, true, false); \
-  else if (batch)                \
-    XT_LIGHT(OT, GO, false, true); \
-  else                           \
-    XT_LIGHT(OT, GO, false, false);
-  if (narrow) {
-    XT_LIGHT_PB(uint32_t, o32);
+    XT_TRY(ensure(ctx, s.vrec, (size_t)nv * sizeof(uint64_t)));
+    vrec = s.vrec.as<uint64_t>();
+    hipLaunchKernelGGL(k_vertex_records, dim3((unsigned)g), dim3(256), 0, st, nv, ooff, vrec);
   } else {
-    XT_LIGHT_PB(int64_t, ooff);
+    // s.deg (the orientation's degree keys until now) takes the 16-bit
+    // out-degrees, later the tile counts.
+    dp16 = s.deg.as<uint16_t>();
+    hipLaunchKernelGGL(k_dplus16, dim3((unsigned)g), dim3(256), 0, st, nv, ooff, dp16);
+  }
+  XT_HIP(ctx, hipGetLastError());
+#define XT_LIGHT(N, P, B)                                                                           \
+  hipLaunchKernelGGL((k_light_items<T, N, P, B>), dim3((unsigned)nchunks), dim3(kBlock), 0, st, nv,  \
+                     ne_bound, in_off, in_tails, oflags, ooff, (const uint64_t*)vrec,              \
+                     (const uint16_t*)dp16, ot, (const int64_t*)s.crows.as<int64_t>(), cut,       \
+                     counters, s.itcnt.as<uint32_t>(), s.items.as<Item>(), pvp, gid, gt)
+#define XT_LIGHT_PB(N)            \
+  if (pv && batch)                \
+    XT_LIGHT(N, true, true);      \
+  else if (pv)                    \
+    XT_LIGHT(N, true, false);     \
+  else if (batch)                 \
+    XT_LIGHT(N, false, true);     \
+  else                            \
+    XT_LIGHT(N, false, false);
+  if (narrow) {
+    XT_LIGHT_PB(true);
+  } else {
+    XT_LIGHT_PB(false);
   }
 #undef XT_LIGHT_PB
 #undef XT_LIGHT
@@ -968,7 +985,7 @@ int reserve(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne, bool per_vertex) {
   XT_TRY(ensure(ctx, s.items, (size_t)nee * sizeof(Item)));
   XT_TRY(ensure(ctx, s.tileoff, (size_t)(nvv + 1) * sizeof(int64_t)));
   XT_TRY(ensure(ctx, s.tiledesc, (size_t)tiles_bound(nvv, nee) * sizeof(TileDesc)));
-  if (nee < (1ll << 32)) XT_TRY(ensure(ctx, s.ooff32, (size_t)(nvv + 1) * sizeof(uint32_t)));
+  if (nee < (1ll << 32)) XT_TRY(ensure(ctx, s.vrec, (size_t)nvv * sizeof(uint64_t)));
   const size_t st = scan_tmp_bytes(nvv) > scan_tmp_bytes(nchunks) ? scan_tmp_bytes(nvv) : scan_tmp_bytes(nchunks);
   XT_TRY(ensure(ctx, s.scan_tmp, st));
   if (per_vertex) XT_TRY(ensure(ctx, s.pv, (size_t)nvv * sizeof(uint64_t)));
@@ -977,7 +994,7 @@ int reserve(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne, bool per_vertex) {
 
 void release(Scratch& s) {
   DevBuf* bufs[] = {&s.deg, &s.dplus, &s.tailcnt, &s.qscan, &s.ooff, &s.otails, &s.counters,
-                    &s.oflags, &s.crows, &s.itcnt, &s.items, &s.tileoff, &s.tiledesc, &s.ooff32, &s.pv, &s.scan_tmp, &s.in_off,
+                    &s.oflags, &s.crows, &s.itcnt, &s.items, &s.tileoff, &s.tiledesc, &s.vrec, &s.pv, &s.scan_tmp, &s.in_off,
                     &s.in_tails};
   for (DevBuf* b : bufs) {
     if (b->p) (void)hipFree(b->p);

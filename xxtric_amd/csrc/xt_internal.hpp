@@ -57,7 +57,7 @@ struct Scratch {
   DevBuf tailcnt;   // uint32[nchunks]     flagged entries of each chunk's last vertex
   DevBuf qscan;     // int64[nchunks+1]    exclusive scan of tailcnt
   DevBuf ooff;      // int64[nv+1]         oriented offsets
-  DevBuf ooff32;    // uint32[nv+1]        32-bit copy of ooff when |E+| < 2^32 (gathered per entry)
+  DevBuf vrec;      // uint64[nv]          ooff[x] | d+(x) << 32 when |E+| < 2^32 (gathered per entry)
   DevBuf otails;    // int32[ne+4]         oriented out-lists, ascending ids (|E+| <= ne), padded
   DevBuf counters;  // uint64[16]          total, tile queue, input-error flags, statistics
   DevBuf oflags;    // uint8[ne/8]         kept-entry mask per 8 CSR entries (orientation)
