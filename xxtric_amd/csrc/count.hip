@@ -28,6 +28,8 @@
 // Per-vertex credit (XT_PER_VERTEX): the triangle {a, b, w} belongs to
 // min(a, b, w) (xxtric.hpp:165); hits credited to the edge's endpoints are
 // summed in registers, hits credited to w use one atomic each.
+#include <type_traits>
+
 #include "xt_device.hpp"
 #include "xt_internal.hpp"
 
@@ -196,7 +198,9 @@ __global__ __launch_bounds__(kBlock) void k_light_items(
   const int64_t e0 = eb + (int64_t)threadIdx.x * kPerThread;
   int32_t ids[kPerThread];  // valid ids are below nv < 2^31
   uint32_t valid = 0, light = 0, own = 0;
-  int64_t gb[kPerThread], ge[kPerThread];
+  // Partner lists: base and length (32-bit each when NARROW).
+  using LT = typename std::conditional<NARROW, uint32_t, int64_t>::type;
+  LT gb[kPerThread], gl[kPerThread];
   int64_t u0 = 0;
   if (e0 < ee) {
     {
@@ -212,20 +216,18 @@ __global__ __launch_bounds__(kBlock) void k_light_items(
     const uint32_t fl = oflags[e0 / kPerThread];
     // 1. The eight neighbours' out-degrees (and, NARROW, their list offsets),
     //    eight gathers in flight.
-    int64_t dvv[kPerThread];
     if constexpr (NARROW) {
       uint64_t rec[kPerThread];
 #pragma unroll
       for (int j = 0; j < kPerThread; j++) rec[j] = vrec[ids[j]];
 #pragma unroll
       for (int j = 0; j < kPerThread; j++) {
-        gb[j] = (int64_t)(uint32_t)rec[j];
-        dvv[j] = (int64_t)(rec[j] >> 32);
-        ge[j] = gb[j] + dvv[j];
+        gb[j] = (uint32_t)rec[j];
+        gl[j] = (uint32_t)(rec[j] >> 32);
       }
     } else {
 #pragma unroll
-      for (int j = 0; j < kPerThread; j++) dvv[j] = dp16[ids[j]];
+      for (int j = 0; j < kPerThread; j++) gl[j] = dp16[ids[j]];  // saturated until step 3
     }
     // 2. Classify along the rows: light out-edges are intersected here; heavy
     //    edges owned by this row are listed.
@@ -249,7 +251,7 @@ __global__ __launch_bounds__(kBlock) void k_light_items(
         }
         if (!((valid >> j) & 1u) || ids[j] == u) continue;  // out of range (unvalidated input) or self-loop
         const bool kept = (fl >> j) & 1u;
-        const int64_t du = du_row, dv = dvv[j];
+        const int64_t du = du_row, dv = (int64_t)gl[j];
         if (du > light_cut || dv > light_cut) {  // heavy: listed under the longer list's owner
           if ((kept ? du >= dv : du > dv) && dv > 0) own |= 1u << j;  // an empty N+(v) closes nothing
         } else if (kept && dv > 0) {  // light in-edges are counted from their source row
@@ -262,11 +264,11 @@ __global__ __launch_bounds__(kBlock) void k_light_items(
     if constexpr (!NARROW) {
 #pragma unroll
       for (int j = 0; j < kPerThread; j++) {
-        gb[j] = ge[j] = 0;
+        gb[j] = gl[j] = 0;
         if (((light | own) >> j) & 1u) {
           const v2i64_a8 t = *reinterpret_cast<const v2i64_a8*>(ooff + ids[j]);
           gb[j] = t.x;
-          ge[j] = t.y;
+          gl[j] = t.y - t.x;
         }
       }
     }
@@ -323,11 +325,11 @@ __global__ __launch_bounds__(kBlock) void k_light_items(
       if ((own >> j) & 1u) {
         const int64_t rank = s_rank[e - eb] - s_rank[max(cur, eb) - eb];
         const int64_t extra = inner ? 0 : (u == r.ulo ? (int64_t)s_bb[0] : (int64_t)s_bb[1]);
-        items[cur + extra + rank] = Item{gb[j], (int32_t)(ge[j] - gb[j]), ids[j]};
+        items[cur + extra + rank] = Item{(int64_t)gb[j], (int32_t)gl[j], ids[j]};
       }
       if (!((light >> j) & 1u)) continue;
       const int64_t v = ids[j];
-      const EdgeLists L = edge_lists(a0, a1, gb[j], ge[j]);
+      const EdgeLists L = edge_lists(a0, a1, (int64_t)gb[j], (int64_t)gb[j] + (int64_t)gl[j]);
       if (L.s1 == L.s0) continue;
       const int64_t mmin = u < v ? u : v;
       uint64_t hits = 0, at_min = 0;
