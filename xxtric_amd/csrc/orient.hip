@@ -305,18 +305,27 @@ __global__ __launch_bounds__(256) void k_scan_reduce(const TIn* __restrict__ in,
   if (threadIdx.x == 0) bsum[blockIdx.x] = (int64_t)acc;
 }
 
+// The tile goes through LDS (padded one slot in 32) so that global loads and
+// stores stay coalesced while each thread scans 16 consecutive items.
 template <typename TIn>
 __global__ __launch_bounds__(256) void k_scan_apply(const TIn* __restrict__ in, int64_t n,
                                                     const int64_t* __restrict__ bpre,
                                                     int64_t* __restrict__ out) {
+  __shared__ int64_t s_v[kScanTile + kScanTile / 32];
   __shared__ int64_t s_tmp[256 / kWave];
-  const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+  auto pad = [](int i) { return i + (i >> 5); };
+  const int64_t base = (int64_t)blockIdx.x * kScanTile;
+#pragma unroll
+  for (int k = 0; k < kScanItems; k++) {
+    const int i = k * 256 + threadIdx.x;
+    s_v[pad(i)] = base + i < n ? (int64_t)in[base + i] : 0;
+  }
+  __syncthreads();
   int64_t v[kScanItems];
   int64_t loc = 0;
 #pragma unroll
   for (int k = 0; k < kScanItems; k++) {
-    const int64_t i = base + k;
-    v[k] = i < n ? (int64_t)in[i] : 0;
+    v[k] = s_v[pad((int)threadIdx.x * kScanItems + k)];
     loc += v[k];
   }
   int64_t total;
@@ -325,9 +334,14 @@ __global__ __launch_bounds__(256) void k_scan_apply(const TIn* __restrict__ in, 
   x += pre;
 #pragma unroll
   for (int k = 0; k < kScanItems; k++) {
-    const int64_t i = base + k;
-    if (i < n) out[i] = x;
+    s_v[pad((int)threadIdx.x * kScanItems + k)] = x;  // this thread's own slots
     x += v[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kScanItems; k++) {
+    const int i = k * 256 + threadIdx.x;
+    if (base + i < n) out[base + i] = s_v[pad(i)];
   }
   if (threadIdx.x == 0 && blockIdx.x == gridDim.x - 1) out[n] = pre + total;
 }
