@@ -61,6 +61,33 @@ def rmat_workload(ctx, scale, steps, warmup, seed=1):
     return g, out
 
 
+def rank0_slice(ctx, g, triangles, parts=8, steps=3):
+    """Rank 0 of a `parts`-way 1-D vertex split at its real shape: the owned rows
+    [0, nv/parts) plus the ghost rows of every neighbour, counted for the
+    triangles whose minimum vertex is owned (and the split by the middle vertex)."""
+    lo, hi = 0, g.nv // parts
+    t0 = time.perf_counter()
+    s = g.slice(lo, hi)
+    slice_s = time.perf_counter() - t0
+    owned, local, _ = s.count_owned(lo, hi)  # warm-up
+    ev = []
+    for _ in range(steps):
+        o, l, _ = s.count_owned(lo, hi)
+        assert (o, l) == (owned, local)
+        ev.append(ctx.timing())
+    whole = g.count_owned(lo, hi)[:2]
+    ms = float(np.median([e["total_ms"] for e in ev]))
+    out = {"owned_range": [lo, hi], "parts": parts, "slice_entries": s.ne, "whole_entries": g.ne,
+           "slice_build_s": round(slice_s, 3), "owned": owned, "local": local, "remote": owned - local,
+           "share_of_total": round(owned / triangles, 6) if triangles else None,
+           "ms": round(ms, 3), "edges_per_s": round((s.ne // 2) / (ms * 1e-3), 1),
+           "phases_ms": {k: round(float(np.median([e[k + "_ms"] for e in ev])), 3)
+                         for k in ("orient", "light", "group")},
+           "equal_on_whole_graph": list(whole) == [owned, local]}
+    s.close()
+    return out
+
+
 def roofline(r):
     # Dominant kernel: k_count_groups (heavy edges).  Its algorithmic bytes per
     # launch: 4 B per id of each heavy edge's shorter out-list plus 4 B per item
@@ -242,6 +269,11 @@ def main():
         "sum_min_heavy": r["sum_min_heavy"], "group_tiles": r["n_tiles"],
     }
     line["cpu_baseline"] = cpu_baseline(g, args.cpu_seconds)
+    if not args.no_extras:
+        try:
+            line["rank0_of_8"] = rank0_slice(ctx, g, r["triangles"])
+        except Exception as e:
+            line["rank0_of_8"] = {"error": str(e)}
     g.close()
     if not args.no_extras:
         try:

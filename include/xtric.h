@@ -141,6 +141,20 @@ int xt_graph_download(xt_ctx* ctx, const xt_graph* g, int64_t* offsets, int32_t*
  * per_vertex (host, nullable) requires XT_PER_VERTEX. */
 int xt_graph_count(xt_ctx* ctx, xt_graph* g, uint32_t flags, uint64_t* total,
                    uint64_t* per_vertex);
+/* One rank of a 1-D vertex split (graph.hpp:76-80: contiguous vertex blocks
+ * per rank), on one device.  xt_graph_slice keeps the rows of the owned
+ * vertices [lo, hi) and of all their neighbours (ghost rows), whole, and
+ * empties the rest; ids stay global.  xt_graph_count_owned counts the
+ * triangles whose minimum vertex is owned -- the ones the rank owning [lo, hi)
+ * credits under the reference's min-vertex rule (xxtric.hpp:165) -- into
+ * *owned, and into *local those whose middle vertex is owned too (the rank
+ * counts these itself; the others it forwards as (v, w) pairs to the middle
+ * vertex's rank, xxtric.hpp:189-204).  *total (nullable) gets every triangle
+ * of g.  The count is the same on g and on its slice for [lo, hi).  flags
+ * selects the orientation; XT_PER_VERTEX is rejected. */
+int xt_graph_slice(xt_ctx* ctx, const xt_graph* g, int64_t lo, int64_t hi, xt_graph** out);
+int xt_graph_count_owned(xt_ctx* ctx, xt_graph* g, int64_t lo, int64_t hi, uint32_t flags,
+                         uint64_t* owned, uint64_t* local, uint64_t* total);
 /* Algorithmic byte figure of one count of g under `flags` orientation:
  * 8(n+1) + 4|E+| + 4 * sum over oriented (u,v) of min(d+(u), d+(v)). */
 int xt_graph_alg_bytes(xt_ctx* ctx, xt_graph* g, uint32_t flags, uint64_t* bytes,

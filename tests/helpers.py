@@ -95,3 +95,22 @@ def sparse_closed_form(off, tails):
     T = (U @ U).multiply(U)
     pv = np.asarray(T.sum(axis=1)).ravel().astype(np.uint64)
     return int(pv.sum()), pv
+
+
+def min_mid_pairs(off, tails):
+    """For every triangle u < v < w, the pair (u, v) -- its minimum and middle
+    vertex -- with multiplicity, by brute-force intersection of the higher
+    neighbour lists (small graphs only)."""
+    off = np.asarray(off, np.int64)
+    tails = np.asarray(tails, np.int64)
+    n = len(off) - 1
+    up = [tails[off[u]:off[u + 1]][tails[off[u]:off[u + 1]] > u] for u in range(n)]
+    us, vs, cs = [], [], []
+    for u in range(n):
+        for v in up[u]:
+            c = len(np.intersect1d(up[u], up[v], assume_unique=True))
+            if c:
+                us.append(u)
+                vs.append(v)
+                cs.append(c)
+    return np.array(us, np.int64), np.array(vs, np.int64), np.array(cs, np.int64)

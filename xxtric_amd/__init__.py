@@ -194,6 +194,24 @@ class DeviceGraph:
                                                    ctypes.byref(total), capi.ptr(pv)))
         return (total.value, pv) if per_vertex else total.value
 
+    def count_owned(self, lo, hi, orient="degree"):
+        """Triangles whose minimum vertex lies in [lo, hi), and of those the ones
+        whose middle vertex does too (one rank of a 1-D vertex split); also the
+        graph's total."""
+        owned = ctypes.c_uint64()
+        local = ctypes.c_uint64()
+        total = ctypes.c_uint64()
+        self.ctx._check(self.ctx._L.xt_graph_count_owned(self.ctx.h, self.h, int(lo), int(hi), _flags(orient),
+                                                         ctypes.byref(owned), ctypes.byref(local),
+                                                         ctypes.byref(total)))
+        return owned.value, local.value, total.value
+
+    def slice(self, lo, hi):
+        """Rows of [lo, hi) and of their neighbours (ghosts) kept, others emptied."""
+        h = ctypes.c_void_p()
+        self.ctx._check(self.ctx._L.xt_graph_slice(self.ctx.h, self.h, int(lo), int(hi), ctypes.byref(h)))
+        return DeviceGraph(self.ctx, h)
+
     def download(self):
         off = np.empty(self.nv + 1, np.int64)
         tails = np.empty(self.ne, np.int32)

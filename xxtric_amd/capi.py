@@ -97,6 +97,8 @@ _SIGS = {
     "xt_graph_info": (_int, [_p, _p, _p]),
     "xt_graph_download": (_int, [_p, _p, _p, _p]),
     "xt_graph_count": (_int, [_p, _p, _u32, _p, _p]),
+    "xt_graph_slice": (_int, [_p, _p, _i64, _i64, _pp]),
+    "xt_graph_count_owned": (_int, [_p, _p, _i64, _i64, _u32, _p, _p, _p]),
     "xt_graph_alg_bytes": (_int, [_p, _p, _u32, _p, _p, _p]),
     "xt_graph_stats_get": (_int, [_p, _p, _u32, ctypes.POINTER(XtGraphStats)]),
     "xt_graph_destroy": (_int, [_p, _p]),

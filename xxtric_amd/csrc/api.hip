@@ -74,7 +74,7 @@ static int check_host_offsets(xt_ctx* ctx, int64_t nv, int64_t ne, const int64_t
 
 template <typename T>
 int count_device_csr(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne, const int64_t* off,
-                     const T* tails, uint32_t flags, hipStream_t st, bool timed) {
+                     const T* tails, uint32_t flags, hipStream_t st, bool timed, const OwnedRange* range) {
   const int mode = (flags & XT_ORIENT_ID) ? kOrientId : kOrientDegree;
   XT_TRY(ensure(ctx, s.counters, kNumSlots * sizeof(uint64_t)));
   XT_TRY(fill_u64(ctx, s.counters.as<uint64_t>(), kNumSlots, 0, st));
@@ -92,15 +92,15 @@ int count_device_csr(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne, const int6
   if (timed) XT_HIP(ctx, hipEventRecord(ctx->ev[kEvOriented], st));
   if (pv) XT_TRY(fill_u64(ctx, pv, nv, 0, st));
   XT_TRY(count_oriented<T>(ctx, s, nv, ne, off, tails, pv, nullptr, nullptr, st,
-                        timed ? &ctx->ev[kEvLight] : nullptr));
+                        timed ? &ctx->ev[kEvLight] : nullptr, range));
   if (timed) XT_HIP(ctx, hipEventRecord(ctx->ev[kEvCounted], st));
   return XT_OK;
 }
 
 template int count_device_csr<int32_t>(xt_ctx*, Scratch&, int64_t, int64_t, const int64_t*, const int32_t*,
-                                       uint32_t, hipStream_t, bool);
+                                       uint32_t, hipStream_t, bool, const OwnedRange*);
 template int count_device_csr<int64_t>(xt_ctx*, Scratch&, int64_t, int64_t, const int64_t*, const int64_t*,
-                                       uint32_t, hipStream_t, bool);
+                                       uint32_t, hipStream_t, bool, const OwnedRange*);
 
 // Reads the count, the input-error slots and the timing after a count.
 static int finish_count(xt_ctx* ctx, int64_t nv, uint32_t flags, hipStream_t st, bool timed,
@@ -452,6 +452,44 @@ int xt_graph_count(xt_ctx* ctx, xt_graph* g, uint32_t flags, uint64_t* total, ui
   hipStream_t st = ctx->stream;
   XT_TRY(count_device_csr<int32_t>(ctx, ctx->s, g->nv, g->ne, g->off, g->tails, flags, st, true));
   return finish_count(ctx, g->nv, flags, st, true, total, per_vertex);
+}
+
+int xt_graph_count_owned(xt_ctx* ctx, xt_graph* g, int64_t lo, int64_t hi, uint32_t flags, uint64_t* owned,
+                         uint64_t* local, uint64_t* total) {
+  if (!ctx || !g || !owned) return XT_ERR_ARG;
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  XT_HIP(ctx, hipSetDevice(ctx->device));
+  if (lo < 0 || hi < lo || hi > g->nv) return set_err(ctx, XT_ERR_ARG, "owned range must satisfy 0 <= lo <= hi <= nv");
+  if (flags & XT_PER_VERTEX) return set_err(ctx, XT_ERR_ARG, "an owned-range count has no per-vertex output");
+  hipStream_t st = ctx->stream;
+  const OwnedRange range{lo, hi};
+  XT_TRY(count_device_csr<int32_t>(ctx, ctx->s, g->nv, g->ne, g->off, g->tails, flags, st, true, &range));
+  uint64_t t = 0;
+  XT_TRY(finish_count(ctx, g->nv, flags, st, true, &t, nullptr));
+  uint64_t h[2] = {0, 0};
+  XT_HIP(ctx, hipMemcpy(h, ctx->s.counters.as<uint64_t>() + kOwned, sizeof(h), hipMemcpyDeviceToHost));
+  *owned = h[0];
+  if (local) *local = h[1];
+  if (total) *total = t;
+  return XT_OK;
+}
+
+int xt_graph_slice(xt_ctx* ctx, const xt_graph* g, int64_t lo, int64_t hi, xt_graph** out) {
+  if (!ctx || !g || !out) return XT_ERR_ARG;
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  XT_HIP(ctx, hipSetDevice(ctx->device));
+  *out = nullptr;
+  if (lo < 0 || hi < lo || hi > g->nv) return set_err(ctx, XT_ERR_ARG, "owned range must satisfy 0 <= lo <= hi <= nv");
+  xt_graph* s = new xt_graph();
+  const int rc = graph_slice(ctx, g, lo, hi, s);
+  if (rc != XT_OK) {
+    if (s->off) (void)hipFree(s->off);
+    if (s->tails) (void)hipFree(s->tails);
+    delete s;
+    return rc;
+  }
+  *out = s;
+  return XT_OK;
 }
 
 int xt_graph_alg_bytes(xt_ctx* ctx, xt_graph* g, uint32_t flags, uint64_t* bytes, uint64_t* sum_min,

@@ -46,6 +46,25 @@ constexpr int kGroupThreads = 512;     // eight waves share one staged hash tabl
 constexpr int kLutBlocks = 1024;       // chunk->item lookup for tiles of up to 64K chunks
 constexpr int64_t kMaxCountEntries = (1ll << 33) - (1ll << 16);
 
+// What a hit is credited to: the total only; per vertex, to the triangle's
+// minimum vertex (xxtric.hpp:165); or, for an owned vertex range [lo, hi),
+// to two totals -- triangles whose minimum vertex is owned, and of those the
+// ones whose middle vertex is owned too (a rank of a 1-D vertex split counts
+// the latter itself and forwards the rest as pairs, xxtric.hpp:189-204).
+constexpr int kCreditTotal = 0;
+constexpr int kCreditVertex = 1;
+constexpr int kCreditRange = 2;
+
+__device__ __forceinline__ void range_credit(int64_t a, int64_t b, int64_t c, int64_t lo, int64_t hi,
+                                             uint32_t& owned, uint32_t& local) {
+  const int64_t m = min(a, min(b, c)), M = max(a, max(b, c));
+  const int64_t mid = a + b + c - m - M;
+  if (m >= lo && m < hi) {
+    owned++;
+    if (mid < hi) local++;  // mid > m >= lo
+  }
+}
+
 // One heavy edge seen from its owner x: the other endpoint y and its out-list.
 struct alignas(16) Item {
   int64_t base;  // ooff[y]
@@ -173,14 +192,15 @@ __device__ __forceinline__ void load_ids(const int32_t* __restrict__ ot, int64_t
 // both rows of an edge pick the same owner.  Otherwise a 2-byte gather of the
 // saturated out-degree classifies every entry (both rows see the same
 // saturated pair) and the owned and light entries gather their offsets.
-template <typename T, bool NARROW, bool PV, bool BATCH>
+template <typename T, bool NARROW, int CM, bool BATCH>
 __global__ __launch_bounds__(kBlock) void k_light_items(
     int64_t nv, int64_t ne, const int64_t* __restrict__ off, const T* __restrict__ tails,
     const uint8_t* __restrict__ oflags, const int64_t* __restrict__ ooff, const uint64_t* __restrict__ vrec,
     const uint16_t* __restrict__ dp16, const int32_t* __restrict__ ot, const int64_t* __restrict__ crows,
     int light_cut, uint64_t* __restrict__ counters, uint32_t* __restrict__ itcnt, Item* __restrict__ items,
     unsigned long long* __restrict__ pv, const int32_t* __restrict__ gid,
-    unsigned long long* __restrict__ gtotal) {
+    unsigned long long* __restrict__ gtotal, int64_t rlo, int64_t rhi) {
+  constexpr bool PV = CM == kCreditVertex;
   __shared__ int64_t s_off[kOffCap + 1];  // input offsets of the chunk's rows
   __shared__ uint64_t s_red[kBlock / kWave];
   __shared__ int64_t s_scan[kBlock / kWave];
@@ -195,6 +215,7 @@ __global__ __launch_bounds__(kBlock) void k_light_items(
   const int32_t g0 = BATCH ? gid[r.ulo] : 0;
 
   uint64_t hits_all = 0;
+  uint32_t r_owned = 0, r_local = 0;  // owned-range credit
   const int64_t e0 = eb + (int64_t)threadIdx.x * kPerThread;
   int32_t ids[kPerThread];  // valid ids are below nv < 2^31
   uint32_t valid = 0, light = 0, own = 0;
@@ -346,6 +367,8 @@ __global__ __launch_bounds__(kBlock) void k_light_items(
               atomicAdd(&pv[w], 1ull);
             else
               at_min++;
+          } else if constexpr (CM == kCreditRange) {
+            range_credit(u, v, w, rlo, rhi, r_owned, r_local);
           }
           pos++;
           if (pos >= L.l1) break;
@@ -374,6 +397,14 @@ __global__ __launch_bounds__(kBlock) void k_light_items(
   } else {
     const uint64_t bsum = block_sum_u64<kBlock>(hits_all, s_red);
     if (threadIdx.x == 0 && bsum) atomicAdd((unsigned long long*)&counters[kTotal], (unsigned long long)bsum);
+  }
+  if constexpr (CM == kCreditRange) {
+    __syncthreads();
+    const uint64_t bo = block_sum_u64<kBlock>((uint64_t)r_owned, s_red);
+    if (threadIdx.x == 0 && bo) atomicAdd((unsigned long long*)&counters[kOwned], (unsigned long long)bo);
+    __syncthreads();
+    const uint64_t bl = block_sum_u64<kBlock>((uint64_t)r_local, s_red);
+    if (threadIdx.x == 0 && bl) atomicAdd((unsigned long long*)&counters[kLocal], (unsigned long long)bl);
   }
   if (threadIdx.x == 0 && nown) atomicAdd((unsigned long long*)&counters[kHeavyItems], (unsigned long long)nown);
 }
@@ -481,12 +512,14 @@ __device__ __forceinline__ bool tab_continue(const int4* tab, uint32_t b, int32_
   }
 }
 
-template <bool PV, bool BATCH, int RC, int WPE>
+template <int CM, bool BATCH, int RC, int WPE>
 __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(WPE, 8))) void k_count_groups(
     const int32_t* __restrict__ ot, const Item* __restrict__ items, const TileDesc* __restrict__ tdesc,
     const int64_t* __restrict__ ntiles_p, uint64_t* __restrict__ counters,
     unsigned long long* __restrict__ pv, const int32_t* __restrict__ gid,
-    unsigned long long* __restrict__ gtotal) {
+    unsigned long long* __restrict__ gtotal, int64_t rlo, int64_t rhi) {
+  constexpr bool PV = CM == kCreditVertex;
+  constexpr bool MASKS = CM != kCreditTotal;  // hits are credited one by one
   __shared__ int4 s_tab[kTabBuckets];
   // Per item k: {adj (64-bit), end, y}.  Chunk c of item k (16 bytes) starts at
   // byte address adj + 16*c and holds end - 4*c ids of the list (mod 2^32;
@@ -656,10 +689,11 @@ __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(W
 
     const bool spl = s_spilled != 0;  // uniform: read after the table barrier
     uint32_t hits = 0, at_x = 0;  // below 2^32: a tile streams fewer than 2^33 ids over 512 threads
+    uint32_t r_owned = 0, r_local = 0;  // owned-range credit
     // Hits counted per wave (the same value in every lane): below 2^32, since the
     // waves take the tile's chunks in turns and it streams fewer than 2^33 ids.
     uint32_t hits_w = 0;
-    auto credit = [&](int32_t w, int k) {  // a hit, with its per-vertex credit
+    auto credit = [&](int32_t w, int k) {  // a hit, with its per-vertex or owned-range credit
       hits++;
       if constexpr (PV) {
         const int64_t yy = s_rec[k].w;
@@ -669,6 +703,8 @@ __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(W
           at_x++;
         else
           atomicAdd(&pv[m], 1ull);
+      } else if constexpr (CM == kCreditRange) {
+        range_credit(x, (int64_t)s_rec[k].w, (int64_t)w, rlo, rhi, r_owned, r_local);
       }
     };
     // Membership in the LDS hash of the ids of one chunk of item k (r of its
@@ -685,7 +721,7 @@ __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(W
 #pragma unroll
       for (int j = 0; j < 4; j++) {
         const bool h = bucket_miss(t[j], w[j]) == 0;
-        if constexpr (PV)
+        if constexpr (MASKS)
           hitm |= (uint32_t)h << j;
         else
           hits += h;
@@ -697,7 +733,7 @@ __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(W
           for (int j = 0; j < 4; j++) {
             if (w[j] < 0 || bucket_miss(t[j], w[j]) == 0 || !bucket_spilled(t[j])) continue;
             if (tab_continue(s_tab, bucket_addr(w[j]) >> 4, w[j])) {
-              if constexpr (PV)
+              if constexpr (MASKS)
                 hitm |= 1u << j;
               else
                 hits++;
@@ -705,7 +741,7 @@ __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(W
           }
         }
       }
-      if constexpr (PV) {
+      if constexpr (MASKS) {
 #pragma unroll
         for (int j = 0; j < 4; j++)
           if ((hitm >> j) & 1u) credit(w[j], k);
@@ -754,7 +790,7 @@ __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(W
     };
 
     auto probe = [&](const int4 v, int k, int r) {
-      if constexpr (PV)
+      if constexpr (MASKS)
         probe_hash(v, k, r);
       else
         probe_count(v, r);
@@ -790,6 +826,14 @@ __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(W
       __syncthreads();
       const uint64_t bx = block_sum_u64<kT>((uint64_t)at_x, s_red);
       if (tid == 0 && bx) atomicAdd(&pv[x], (unsigned long long)bx);
+    }
+    if constexpr (CM == kCreditRange) {
+      __syncthreads();
+      const uint64_t bo = block_sum_u64<kT>((uint64_t)r_owned, s_red);
+      if (tid == 0 && bo) atomicAdd((unsigned long long*)&counters[kOwned], (unsigned long long)bo);
+      __syncthreads();
+      const uint64_t bl = block_sum_u64<kT>((uint64_t)r_local, s_red);
+      if (tid == 0 && bl) atomicAdd((unsigned long long*)&counters[kLocal], (unsigned long long)bl);
     }
     if (tid == 0 && bh) {
       if constexpr (BATCH)
@@ -855,7 +899,7 @@ static int64_t tiles_bound(int64_t nv, int64_t ne) {
 template <typename T>
 int count_oriented(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne_bound, const int64_t* in_off,
                    const T* in_tails, uint64_t* pv, const int32_t* gid, uint64_t* gtotal,
-                   hipStream_t st, hipEvent_t* evs) {
+                   hipStream_t st, hipEvent_t* evs, const OwnedRange* range) {
   if (nv <= 0 || ne_bound <= 0) return XT_OK;
   // A tile's chunk indices are 32-bit: its lists hold at most ne_bound ids.
   if (ne_bound >= kMaxCountEntries) {
@@ -863,6 +907,9 @@ int count_oriented(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne_bound, const 
                    "graph too large: the counting kernels take fewer than 2^33 - 2^16 entries");
   }
   const bool batch = gid != nullptr;
+  if (range && (pv || batch))
+    return set_err(ctx, XT_ERR_ARG, "an owned-range count takes neither per-vertex output nor a batch");
+  const int64_t rlo = range ? range->lo : 0, rhi = range ? range->hi : 0;
   const int64_t nchunks = (ne_bound + kChunk - 1) / kChunk;
   XT_TRY(ensure(ctx, s.itcnt, (size_t)nv * sizeof(uint32_t)));
   XT_TRY(ensure(ctx, s.items, (size_t)ne_bound * sizeof(Item)));
@@ -894,20 +941,22 @@ int count_oriented(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne_bound, const 
     hipLaunchKernelGGL(k_dplus16, dim3((unsigned)g), dim3(256), 0, st, nv, ooff, dp16);
   }
   XT_HIP(ctx, hipGetLastError());
-#define XT_LIGHT(N, P, B)                                                                           \
-  hipLaunchKernelGGL((k_light_items<T, N, P, B>), dim3((unsigned)nchunks), dim3(kBlock), 0, st, nv,  \
+#define XT_LIGHT(N, C, B)                                                                           \
+  hipLaunchKernelGGL((k_light_items<T, N, C, B>), dim3((unsigned)nchunks), dim3(kBlock), 0, st, nv,  \
                      ne_bound, in_off, in_tails, oflags, ooff, (const uint64_t*)vrec,              \
                      (const uint16_t*)dp16, ot, (const int64_t*)s.crows.as<int64_t>(), cut,       \
-                     counters, s.itcnt.as<uint32_t>(), s.items.as<Item>(), pvp, gid, gt)
-#define XT_LIGHT_PB(N)            \
-  if (pv && batch)                \
-    XT_LIGHT(N, true, true);      \
-  else if (pv)                    \
-    XT_LIGHT(N, true, false);     \
-  else if (batch)                 \
-    XT_LIGHT(N, false, true);     \
-  else                            \
-    XT_LIGHT(N, false, false);
+                     counters, s.itcnt.as<uint32_t>(), s.items.as<Item>(), pvp, gid, gt, rlo, rhi)
+#define XT_LIGHT_PB(N)                    \
+  if (range)                              \
+    XT_LIGHT(N, kCreditRange, false);     \
+  else if (pv && batch)                   \
+    XT_LIGHT(N, kCreditVertex, true);     \
+  else if (pv)                            \
+    XT_LIGHT(N, kCreditVertex, false);    \
+  else if (batch)                         \
+    XT_LIGHT(N, kCreditTotal, true);      \
+  else                                    \
+    XT_LIGHT(N, kCreditTotal, false);
   if (narrow) {
     XT_LIGHT_PB(true);
   } else {
@@ -927,10 +976,10 @@ int count_oriented(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne_bound, const 
   XT_HIP(ctx, hipGetLastError());
   if (evs) XT_HIP(ctx, hipEventRecord(evs[1], st));
   const unsigned groups = (unsigned)(ctx->num_cus * ctx->group_blocks);
-#define XT_GROUPS(P, B, R, W)                                                                      \
-  hipLaunchKernelGGL((k_count_groups<P, B, R, W>), dim3(groups), dim3(kGroupThreads), 0, st, ot,     \
+#define XT_GROUPS(C, B, R, W)                                                                      \
+  hipLaunchKernelGGL((k_count_groups<C, B, R, W>), dim3(groups), dim3(kGroupThreads), 0, st, ot,     \
                      (const Item*)s.items.as<Item>(), (const TileDesc*)s.tiledesc.as<TileDesc>(),   \
-                     (const int64_t*)s.tileoff.as<int64_t>() + nv, counters, pvp, gid, gt)
+                     (const int64_t*)s.tileoff.as<int64_t>() + nv, counters, pvp, gid, gt, rlo, rhi)
 #define XT_GROUPS_R(P, B)          \
   if (ctx->group_round == 1)       \
     XT_GROUPS(P, B, 1, 8);         \
@@ -938,14 +987,16 @@ int count_oriented(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne_bound, const 
     XT_GROUPS(P, B, 2, 6);         \
   else                             \
     XT_GROUPS(P, B, 4, 4);
-  if (pv && batch) {
-    XT_GROUPS_R(true, true);
+  if (range) {
+    XT_GROUPS_R(kCreditRange, false);
+  } else if (pv && batch) {
+    XT_GROUPS_R(kCreditVertex, true);
   } else if (pv) {
-    XT_GROUPS_R(true, false);
+    XT_GROUPS_R(kCreditVertex, false);
   } else if (batch) {
-    XT_GROUPS_R(false, true);
+    XT_GROUPS_R(kCreditTotal, true);
   } else {
-    XT_GROUPS_R(false, false);
+    XT_GROUPS_R(kCreditTotal, false);
   }
 #undef XT_GROUPS_R
 #undef XT_GROUPS
@@ -954,9 +1005,11 @@ int count_oriented(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne_bound, const 
 }
 
 template int count_oriented<int32_t>(xt_ctx*, Scratch&, int64_t, int64_t, const int64_t*, const int32_t*,
-                                     uint64_t*, const int32_t*, uint64_t*, hipStream_t, hipEvent_t*);
+                                     uint64_t*, const int32_t*, uint64_t*, hipStream_t, hipEvent_t*,
+                                     const OwnedRange*);
 template int count_oriented<int64_t>(xt_ctx*, Scratch&, int64_t, int64_t, const int64_t*, const int64_t*,
-                                     uint64_t*, const int32_t*, uint64_t*, hipStream_t, hipEvent_t*);
+                                     uint64_t*, const int32_t*, uint64_t*, hipStream_t, hipEvent_t*,
+                                     const OwnedRange*);
 
 int alg_stats(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne_bound, hipStream_t st) {
   if (nv <= 0 || ne_bound <= 0) return XT_OK;

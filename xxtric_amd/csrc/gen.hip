@@ -246,4 +246,100 @@ int gen_er(xt_ctx* ctx, int64_t n, uint64_t seed, uint64_t thr64, xt_graph* g) {
   return XT_OK;
 }
 
+// ---------------------------------------------------------------------------
+// One rank's slice of a 1-D vertex split (graph.hpp:76-80 splits vertices into
+// contiguous blocks): the rows of the owned vertices [lo, hi) and of every
+// neighbour of them (ghosts) kept whole, all other rows emptied; ids stay
+// global.  Every triangle whose minimum vertex is owned has all three edges
+// in it: two in the owned row, the third in a ghost row.
+__global__ __launch_bounds__(256) void k_slice_mark(int64_t lo, int64_t hi, const int64_t* __restrict__ off,
+                                                    const int32_t* __restrict__ tails,
+                                                    uint8_t* __restrict__ mark) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t e0 = off[lo], e1 = off[hi];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (hi - lo) + (e1 - e0); i += stride) {
+    if (i < hi - lo)
+      mark[lo + i] = 1;  // owned
+    else
+      mark[tails[e0 + (i - (hi - lo))]] = 1;  // neighbour of an owned vertex
+  }
+}
+
+__global__ __launch_bounds__(256) void k_slice_degrees(int64_t nv, const int64_t* __restrict__ off,
+                                                       const uint8_t* __restrict__ mark,
+                                                       uint32_t* __restrict__ deg) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < nv; u += stride)
+    deg[u] = mark[u] ? (uint32_t)(off[u + 1] - off[u]) : 0u;
+}
+
+// One wave per row; the lanes copy the row's ids.
+__global__ __launch_bounds__(256) void k_slice_copy(int64_t nv, const int64_t* __restrict__ off,
+                                                    const int32_t* __restrict__ tails,
+                                                    const uint8_t* __restrict__ mark,
+                                                    const int64_t* __restrict__ noff,
+                                                    int32_t* __restrict__ ntails) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t nw = (int64_t)gridDim.x * (blockDim.x / kWave);
+  for (int64_t u = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; u < nv; u += nw) {
+    if (!mark[u]) continue;
+    const int64_t a = off[u], d = off[u + 1] - a, b = noff[u];
+    for (int64_t i = lane; i < d; i += kWave) ntails[b + i] = tails[a + i];
+  }
+}
+
+int graph_slice(xt_ctx* ctx, const xt_graph* g, int64_t lo, int64_t hi, xt_graph* out) {
+  const int64_t nv = g->nv;
+  hipStream_t st = ctx->stream;
+  uint8_t* mark = nullptr;
+  uint32_t* deg = nullptr;
+  int64_t* tmp = nullptr;
+  int rc = XT_OK;
+  auto fail = [&](hipError_t e, const char* what) {
+    rc = hip_err(ctx, e, what);
+    return rc;
+  };
+  hipError_t e = hipMalloc(&mark, (size_t)(nv > 0 ? nv : 1));
+  if (e == hipSuccess) e = hipMalloc(&deg, (size_t)(nv > 0 ? nv : 1) * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc(&tmp, scan_tmp_bytes(nv > 0 ? nv : 1));
+  if (e == hipSuccess) e = hipMalloc(&out->off, (size_t)(nv + 1) * sizeof(int64_t));
+  if (e != hipSuccess) fail(e, "hipMalloc(slice)");
+  if (rc == XT_OK && nv > 0) {
+    e = hipMemsetAsync(mark, 0, (size_t)nv, st);
+    if (e != hipSuccess) fail(e, "hipMemsetAsync(slice)");
+  }
+  if (rc == XT_OK && nv > 0) {
+    hipLaunchKernelGGL(k_slice_mark, dim3(grid_of(nv)), dim3(256), 0, st, lo, hi, (const int64_t*)g->off,
+                       (const int32_t*)g->tails, mark);
+    hipLaunchKernelGGL(k_slice_degrees, dim3(grid_of(nv)), dim3(256), 0, st, nv, (const int64_t*)g->off,
+                       (const uint8_t*)mark, deg);
+    e = hipGetLastError();
+    if (e != hipSuccess) fail(e, "slice kernels");
+  }
+  if (rc == XT_OK) rc = scan_u32_to_i64_tmp(ctx, deg, nv, out->off, tmp, st);
+  int64_t ne = 0;
+  if (rc == XT_OK) {
+    e = hipMemcpyAsync(&ne, out->off + nv, sizeof(int64_t), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) fail(e, "slice size");
+  }
+  if (rc == XT_OK) {
+    e = hipMalloc(&out->tails, (size_t)(ne > 0 ? ne : 1) * sizeof(int32_t));
+    if (e != hipSuccess) fail(e, "hipMalloc(slice tails)");
+  }
+  if (rc == XT_OK && nv > 0) {
+    hipLaunchKernelGGL(k_slice_copy, dim3(grid_of(nv * kWave)), dim3(256), 0, st, nv, (const int64_t*)g->off,
+                       (const int32_t*)g->tails, (const uint8_t*)mark, (const int64_t*)out->off, out->tails);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) fail(e, "slice copy");
+  }
+  out->nv = nv;
+  out->ne = ne;
+  if (mark) (void)hipFree(mark);
+  if (deg) (void)hipFree(deg);
+  if (tmp) (void)hipFree(tmp);
+  return rc;
+}
+
 }  // namespace xt

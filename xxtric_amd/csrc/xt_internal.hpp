@@ -34,6 +34,8 @@ enum Slot : int {
   kStatMinHeavy = 9,   // same sum over heavy edges only
   kStatHeavyEdges = 10,// number of heavy edges
   kHeavyItems = 11,    // heavy edges listed under their owners by the counting pass
+  kOwned = 12,         // owned-range count: triangles whose minimum vertex is owned
+  kLocal = 13,         // ... and whose middle vertex is owned too
   kNumSlots = 16
 };
 
@@ -106,6 +108,9 @@ int ensure(xt_ctx* ctx, DevBuf& b, size_t bytes);
 // Exclusive scan of n uint32 -> int64 out[0..n] (out[n] = total).
 int scan_u32_to_i64(xt_ctx* ctx, Scratch& s, const uint32_t* in, int64_t n, int64_t* out,
                     hipStream_t st);
+// The same with a caller-provided block-sum buffer of scan_tmp_bytes(n) bytes.
+int scan_u32_to_i64_tmp(xt_ctx* ctx, const uint32_t* in, int64_t n, int64_t* out, int64_t* tmp,
+                        hipStream_t st);
 
 // Orientation of a symmetric CSR into the context's ooff/otails.
 // Upper bound of |E+| is ne/2 (one direction per undirected pair).
@@ -119,10 +124,15 @@ int orient(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne, const int64_t* off, 
 // and the intersection kernel drains a device tile queue with a fixed grid.
 // evs (nullable): two events recorded after the light-edge kernel and after
 // the item lists are built.
+// Owned vertex range [lo, hi) of a credit-by-range count.
+struct OwnedRange {
+  int64_t lo, hi;
+};
+
 template <typename T>
 int count_oriented(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne_bound, const int64_t* in_off,
                    const T* in_tails, uint64_t* pv, const int32_t* gid, uint64_t* gtotal,
-                   hipStream_t st, hipEvent_t* evs);
+                   hipStream_t st, hipEvent_t* evs, const OwnedRange* range = nullptr);
 int alg_stats(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne_bound, hipStream_t st);
 // Pre-sizes every scratch buffer for graphs up to (nv, ne).
 int reserve(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne, bool per_vertex);
@@ -142,12 +152,14 @@ int validate(xt_ctx* ctx, int64_t nv, int64_t ne, const int64_t* off, const T* t
 // Whole single-graph count on the device (orientation + intersection).
 template <typename T>
 int count_device_csr(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne, const int64_t* off, const T* tails,
-                     uint32_t flags, hipStream_t st, bool timed);
+                     uint32_t flags, hipStream_t st, bool timed, const OwnedRange* range = nullptr);
 
 // Device generators.
 int gen_rmat(xt_ctx* ctx, int scale, int edgefactor, const uint32_t thr[3], uint64_t seed,
              int permute, xt_graph* g);
 int gen_er(xt_ctx* ctx, int64_t n, uint64_t seed, uint64_t thr64, xt_graph* g);
+// Rows of [lo, hi) and of their neighbours kept, the others emptied (fills out).
+int graph_slice(xt_ctx* ctx, const xt_graph* g, int64_t lo, int64_t hi, xt_graph* out);
 
 // Host mixing function shared by the generators (SplitMix64 output function).
 inline uint64_t mix64(uint64_t z) {
