@@ -519,7 +519,7 @@ __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(W
     unsigned long long* __restrict__ pv, const int32_t* __restrict__ gid,
     unsigned long long* __restrict__ gtotal, int64_t rlo, int64_t rhi) {
   constexpr bool PV = CM == kCreditVertex;
-  constexpr bool MASKS = CM != kCreditTotal;  // hits are credited one by one
+  constexpr bool MASKS = PV;  // hits are credited one by one (per vertex)
   __shared__ int4 s_tab[kTabBuckets];
   // Per item k: {adj (64-bit), end, y}.  Chunk c of item k (16 bytes) starts at
   // byte address adj + 16*c and holds end - 4*c ids of the list (mod 2^32;
@@ -693,6 +693,7 @@ __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(W
     // Hits counted per wave (the same value in every lane): below 2^32, since the
     // waves take the tile's chunks in turns and it streams fewer than 2^33 ids.
     uint32_t hits_w = 0;
+    uint32_t r_owned_w = 0, r_local_w = 0;  // owned-range credit counted per wave
     auto credit = [&](int32_t w, int k) {  // a hit, with its per-vertex or owned-range credit
       hits++;
       if constexpr (PV) {
@@ -753,17 +754,33 @@ __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(W
     // bucket carries the mark, so an id there compares unequal -- like every
     // other miss in a spilled bucket it goes to the continuation, which tests
     // the bucket again with the mark removed and walks on.
-    auto probe_count = [&](const int4 v, int r) {
+    // With owned-range credit the same masks are split further: the triangle
+    // {x, y, w} of a hit has minimum min(a, w) and middle max(a, min(b, w)),
+    // a = min(x, y), b = max(x, y).
+    auto probe_count = [&](const int4 v, int k, int r) {
       const int32_t w[4] = {v.x, v.y, v.z, v.w};
       int4 t[4];
 #pragma unroll
       for (int j = 0; j < 4; j++) t[j] = bucket_at(s_tab, bucket_addr(w[j]));
+      int32_t ya = 0, yb = 0;
+      if constexpr (CM == kCreditRange) {
+        const int32_t y = s_rec[k].w;
+        ya = min((int32_t)x, y);
+        yb = max((int32_t)x, y);
+      }
 #pragma unroll
       for (int j = 0; j < 4; j++) {
         const uint64_t hit = (__builtin_amdgcn_ballot_w64(t[j].x == w[j]) | __builtin_amdgcn_ballot_w64(t[j].y == w[j]) |
                               __builtin_amdgcn_ballot_w64(t[j].z == w[j]) | __builtin_amdgcn_ballot_w64(t[j].w == w[j])) &
                              __builtin_amdgcn_ballot_w64(j < r);
         hits_w += (uint32_t)__popcll(hit);
+        if constexpr (CM == kCreditRange) {
+          const int32_t m = min(ya, w[j]), mid = max(ya, min(yb, w[j]));
+          const uint64_t own =
+              hit & __builtin_amdgcn_ballot_w64((uint32_t)(m - (int32_t)rlo) < (uint32_t)(rhi - rlo));
+          r_owned_w += (uint32_t)__popcll(own);
+          r_local_w += (uint32_t)__popcll(own & __builtin_amdgcn_ballot_w64(mid < (int32_t)rhi));
+        }
       }
       // A lane whose four buckets include a spilled one takes the continuation
       // (only tables with a spilled bucket can have one; uniform test).
@@ -771,10 +788,12 @@ __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(W
 #pragma unroll
         for (int j = 0; j < 4; j++) {
           if (j >= r || t[j].w >= 0) continue;
-          if ((t[j].w & 0x7fffffff) == w[j])
-            hits++;  // in slot 3 of a spilled bucket: the raw compare missed it
-          else if (bucket_miss(t[j], w[j]) != 0 && tab_continue(s_tab, bucket_addr(w[j]) >> 4, w[j]))
+          bool h = (t[j].w & 0x7fffffff) == w[j];  // in slot 3 of a spilled bucket: the raw compare missed it
+          if (!h) h = bucket_miss(t[j], w[j]) != 0 && tab_continue(s_tab, bucket_addr(w[j]) >> 4, w[j]);
+          if (h) {
             hits++;
+            if constexpr (CM == kCreditRange) range_credit(ya, yb, w[j], rlo, rhi, r_owned, r_local);
+          }
         }
       }
     };
@@ -793,7 +812,7 @@ __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(W
       if constexpr (MASKS)
         probe_hash(v, k, r);
       else
-        probe_count(v, r);
+        probe_count(v, k, r);
     };
     // Two register buffers alternate by hand (no loop-carried copies) and the
     // next round is always issued (past the end it re-reads the last chunk,
@@ -829,10 +848,10 @@ __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(W
     }
     if constexpr (CM == kCreditRange) {
       __syncthreads();
-      const uint64_t bo = block_sum_u64<kT>((uint64_t)r_owned, s_red);
+      const uint64_t bo = block_sum_u64<kT>((uint64_t)r_owned + (lane == 0 ? (uint64_t)r_owned_w : 0ull), s_red);
       if (tid == 0 && bo) atomicAdd((unsigned long long*)&counters[kOwned], (unsigned long long)bo);
       __syncthreads();
-      const uint64_t bl = block_sum_u64<kT>((uint64_t)r_local, s_red);
+      const uint64_t bl = block_sum_u64<kT>((uint64_t)r_local + (lane == 0 ? (uint64_t)r_local_w : 0ull), s_red);
       if (tid == 0 && bl) atomicAdd((unsigned long long*)&counters[kLocal], (unsigned long long)bl);
     }
     if (tid == 0 && bh) {
