@@ -105,6 +105,13 @@ __device__ __forceinline__ EdgeLists edge_lists(int64_t a0, int64_t a1, int64_t 
   return L;
 }
 
+// Lane l's value of v, in every lane (l uniform).
+__device__ __forceinline__ int64_t readlane64(int64_t v, int l) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)v >> 32), l);
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
 // Rows of the oriented edges [eb, ee) staged in LDS when they fit.
 struct TileRows {
   int64_t ulo, uhi, span;
@@ -193,7 +200,7 @@ __device__ __forceinline__ void load_ids(const int32_t* __restrict__ ot, int64_t
 // saturated out-degree classifies every entry (both rows see the same
 // saturated pair) and the owned and light entries gather their offsets.
 template <typename T, bool NARROW, int CM, bool BATCH>
-__global__ __launch_bounds__(kBlock) void k_light_items(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_light_items(
     int64_t nv, int64_t ne, const int64_t* __restrict__ off, const T* __restrict__ tails,
     const uint8_t* __restrict__ oflags, const int64_t* __restrict__ ooff, const uint64_t* __restrict__ vrec,
     const uint16_t* __restrict__ dp16, const int32_t* __restrict__ ot, const int64_t* __restrict__ crows,
@@ -321,72 +328,77 @@ __global__ __launch_bounds__(kBlock) void k_light_items(
     if (r.uhi != r.ulo) reserve_row(r.uhi, 1);
   }
   __syncthreads();
-  if (e0 < ee) {
-    // 5. Along the rows again: write the owned items and intersect the light edges.
-    int64_t u = u0;
-    int64_t cur = tile_off(r, s_off, off, u);
-    int64_t nxt = tile_off(r, s_off, off, u + 1);
-    int64_t a0 = ooff[u], a1 = ooff[u + 1];
+  {
+    // 5. Along the rows again: write the owned items and intersect the light
+    //    edges.  A light edge is intersected by the whole wave, one edge at a
+    //    time: the lanes take the ids of the shorter list and search each in
+    //    the longer one (both at most light_cut long), instead of one lane
+    //    walking the list while the others wait.
+    const bool live = e0 < ee;
+    int64_t u = u0, cur = 0, nxt = 0, a0 = 0, a1 = 0;
+    if (live) {
+      cur = tile_off(r, s_off, off, u);
+      nxt = tile_off(r, s_off, off, u + 1);
+      a0 = ooff[u];
+      a1 = ooff[u + 1];
+    }
+    const int lane = threadIdx.x & (kWave - 1);
 #pragma unroll
     for (int j = 0; j < kPerThread; j++) {
       const int64_t e = e0 + j;
-      if (e >= ee) break;
-      while (e >= nxt) {
-        ++u;
-        cur = nxt;
-        nxt = tile_off(r, s_off, off, u + 1);
-        a0 = a1;
-        a1 = ooff[u + 1];
+      const bool act = live && e < ee;
+      if (act) {
+        while (e >= nxt) {
+          ++u;
+          cur = nxt;
+          nxt = tile_off(r, s_off, off, u + 1);
+          a0 = a1;
+          a1 = ooff[u + 1];
+        }
+        const bool inner = cur >= eb && nxt <= ee;
+        if (inner && e == cur) {  // the row's first entry publishes its item count
+          const int cnt = s_rank[nxt - eb] - s_rank[cur - eb];
+          if (cnt) itcnt[u] = (uint32_t)cnt;
+        }
+        if ((own >> j) & 1u) {
+          const int64_t rank = s_rank[e - eb] - s_rank[max(cur, eb) - eb];
+          const int64_t extra = inner ? 0 : (u == r.ulo ? (int64_t)s_bb[0] : (int64_t)s_bb[1]);
+          items[cur + extra + rank] = Item{(int64_t)gb[j], (int32_t)gl[j], ids[j]};
+        }
       }
-      const bool inner = cur >= eb && nxt <= ee;
-      if (inner && e == cur) {  // the row's first entry publishes its item count
-        const int cnt = s_rank[nxt - eb] - s_rank[cur - eb];
-        if (cnt) itcnt[u] = (uint32_t)cnt;
-      }
-      if ((own >> j) & 1u) {
-        const int64_t rank = s_rank[e - eb] - s_rank[max(cur, eb) - eb];
-        const int64_t extra = inner ? 0 : (u == r.ulo ? (int64_t)s_bb[0] : (int64_t)s_bb[1]);
-        items[cur + extra + rank] = Item{(int64_t)gb[j], (int32_t)gl[j], ids[j]};
-      }
-      if (!((light >> j) & 1u)) continue;
-      const int64_t v = ids[j];
-      const EdgeLists L = edge_lists(a0, a1, (int64_t)gb[j], (int64_t)gb[j] + (int64_t)gl[j]);
-      if (L.s1 == L.s0) continue;
-      const int64_t mmin = u < v ? u : v;
-      uint64_t hits = 0, at_min = 0;
-      int64_t pos = L.l0;
-      const int32_t lmax = ot[L.l1 - 1];
-      for (int64_t k = L.s0; k < L.s1; k++) {
-        const int32_t w = ot[k];
-        if (w > lmax) break;
-        pos = lower_bound_dev<int32_t>(ot, pos, L.l1, w);
-        if (ot[pos] == w) {
-          hits++;
-          if constexpr (PV) {
-            if ((int64_t)w < mmin)
-              atomicAdd(&pv[w], 1ull);
-            else
-              at_min++;
-          } else if constexpr (CM == kCreditRange) {
-            range_credit(u, v, w, rlo, rhi, r_owned, r_local);
+      uint64_t pend = __builtin_amdgcn_ballot_w64(act && ((light >> j) & 1u));
+      while (pend) {  // uniform
+        const int src = __builtin_ctzll(pend);
+        pend &= pend - 1;
+        const int64_t su = readlane64(u, src), sv = __builtin_amdgcn_readlane(ids[j], src);
+        const int64_t sb = readlane64((int64_t)gb[j], src);
+        const EdgeLists L = edge_lists(readlane64(a0, src), readlane64(a1, src), sb,
+                                       sb + readlane64((int64_t)gl[j], src));
+        const int64_t mmin = su < sv ? su : sv;
+        uint32_t h = 0;
+        for (int64_t t = lane; t < L.s1 - L.s0; t += kWave) {
+          const int32_t w = ot[L.s0 + t];
+          const int64_t p = lower_bound_dev<int32_t>(ot, L.l0, L.l1, w);
+          if (p < L.l1 && ot[p] == w) {
+            h++;
+            if constexpr (PV) {
+              atomicAdd(&pv[(int64_t)w < mmin ? (int64_t)w : mmin], 1ull);
+            } else if constexpr (CM == kCreditRange) {
+              range_credit(su, sv, w, rlo, rhi, r_owned, r_local);
+            }
           }
-          pos++;
-          if (pos >= L.l1) break;
         }
-      }
-      if constexpr (PV) {
-        if (at_min) atomicAdd(&pv[mmin], (unsigned long long)at_min);
-      }
-      if constexpr (BATCH) {
-        if (hits) {
-          const int32_t slot = gid[u] - g0;
-          if (slot < kGraphSlots)
-            atomicAdd(&s_g[slot], (unsigned long long)hits);
-          else
-            atomicAdd(&gtotal[gid[u]], (unsigned long long)hits);
+        if constexpr (BATCH) {
+          if (h) {
+            const int32_t slot = gid[su] - g0;
+            if (slot < kGraphSlots)
+              atomicAdd(&s_g[slot], (unsigned long long)h);
+            else
+              atomicAdd(&gtotal[gid[su]], (unsigned long long)h);
+          }
+        } else {
+          hits_all += h;
         }
-      } else {
-        hits_all += hits;
       }
     }
   }
