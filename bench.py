@@ -159,7 +159,7 @@ def batch_workload(ctx, n_graphs=4096, runs=5):
     return out
 
 
-def serve_workload(ctx, rate, n_req=4000, pool=400, seed=7):
+def serve_workload(ctx, rate, n_req=4000, pool=400, seed=7, borrow=True):
     import xxtric_amd
     rng = np.random.default_rng(seed)
     graphs = [xxtric_amd.batch_graph_csr(2026, g) for g in range(pool)]
@@ -181,7 +181,7 @@ def serve_workload(ctx, rate, n_req=4000, pool=400, seed=7):
         while srv.now_us() < base + a:
             pass
         o, t = graphs[k]
-        tickets.append((srv.submit(o, t, base + a), k))
+        tickets.append((srv.submit(o, t, base + a, borrow=borrow), k))
     srv.drain(120.0)
     solo = {}
     ok = True

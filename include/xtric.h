@@ -241,6 +241,13 @@ int xt_serve_create(xt_ctx* ctx, const xt_serve_config* cfg, xt_server** out);
  * arrival_us < 0 stamps the arrival with the server clock. */
 int xt_serve_submit(xt_server* s, int64_t nv, int64_t ne, const int64_t* offsets,
                     const int32_t* tails, double arrival_us, uint64_t* ticket);
+/* The same without the copy: the server reads offsets/tails in place when it
+ * stages the request (checked and copied into pinned memory in one pass, in
+ * the server's thread), so the caller keeps them valid and unchanged until
+ * xt_serve_poll returns for the ticket or the server is destroyed.  A request
+ * that fails the check is reported by xt_serve_poll (XT_ERR_INPUT). */
+int xt_serve_submit_borrowed(xt_server* s, int64_t nv, int64_t ne, const int64_t* offsets,
+                             const int32_t* tails, double arrival_us, uint64_t* ticket);
 /* XT_OK with the count when done, XT_PENDING while queued or running. */
 int xt_serve_poll(xt_server* s, uint64_t ticket, uint64_t* total, double* latency_us);
 int xt_serve_drain(xt_server* s, double timeout_s);

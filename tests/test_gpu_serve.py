@@ -97,3 +97,29 @@ def test_admission_budget(ctx):
     st = srv.stats()
     assert st["rejected"] == 1 and 0 < st["peak_resident_bytes"] <= budget
     srv.close()
+
+
+def test_borrowed_submissions(ctx):
+    """Borrowed requests (read in place, checked in the server's thread) give
+    the same counts as copied ones, mixed in the same batches; a bad borrowed
+    request is reported by poll and does not disturb the others."""
+    gs = _graphs(800, 80)
+    exp = [O.count_degree_ordered(o, t) for o, t in gs]
+    bad_off = np.array([0, 2, 3, 4], np.int64)
+    bad_tails = np.array([1, 7, 0, 0], np.int32)  # id 7 outside [0, 3)
+    srv = xxtric_amd.Server(ctx, hbm_budget_bytes=1 << 30, max_batch_graphs=16)
+    _say("serve borrowed + copied")
+    tickets = []
+    for i, (o, t) in enumerate(gs):
+        tickets.append(srv.submit(o, t, borrow=(i % 2 == 0)))
+        if i == 40:
+            bad = srv.submit(bad_off, bad_tails, borrow=True)
+    srv.drain(120.0)
+    with pytest.raises(xxtric_amd.XtError) as e:
+        srv.poll(bad)
+    assert e.value.code == -4
+    res = {t: srv.poll(t) for t in tickets}
+    assert [res[t][0] for t in tickets] == exp
+    st = srv.stats()
+    assert st["completed"] == 80 and st["rejected"] == 1
+    srv.close()

@@ -77,15 +77,20 @@ class Server:
         h = ctypes.c_void_p()
         ctx._check(self._L.xt_serve_create(ctx.h, ctypes.byref(cfg), ctypes.byref(h)))
         self.h = h
+        self._borrowed = {}  # ticket -> arrays the server reads in place
 
-    def submit(self, off, tails, arrival_us=-1.0):
+    def submit(self, off, tails, arrival_us=-1.0, borrow=False):
+        """Queue a graph.  borrow=True skips the copy: the server reads the arrays
+        in place, and they are kept referenced here until the ticket is polled."""
         off = np.ascontiguousarray(off, np.int64)
         tails = np.ascontiguousarray(tails, np.int32)
         t = ctypes.c_uint64()
-        rc = self._L.xt_serve_submit(self.h, len(off) - 1, len(tails), capi.ptr(off), capi.ptr(tails),
-                                     arrival_us, ctypes.byref(t))
+        fn = self._L.xt_serve_submit_borrowed if borrow else self._L.xt_serve_submit
+        rc = fn(self.h, len(off) - 1, len(tails), capi.ptr(off), capi.ptr(tails), arrival_us, ctypes.byref(t))
         if rc != capi.XT_OK:
             raise XtError(rc, self._L.xt_last_error(self.ctx.h).decode())
+        if borrow:
+            self._borrowed[t.value] = (off, tails)
         return t.value
 
     def poll(self, ticket):
@@ -94,6 +99,7 @@ class Server:
         rc = self._L.xt_serve_poll(self.h, ticket, ctypes.byref(total), ctypes.byref(lat))
         if rc == capi.XT_PENDING:
             return None
+        self._borrowed.pop(ticket, None)
         if rc != capi.XT_OK:
             raise XtError(rc, "xt_serve_poll")
         return total.value, lat.value
@@ -117,6 +123,7 @@ class Server:
         if self.h:
             self._L.xt_serve_destroy(self.h)
             self.h = None
+        self._borrowed.clear()
 
     def __del__(self):
         try:

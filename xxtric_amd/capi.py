@@ -119,6 +119,7 @@ _SIGS = {
     "xt_batch_graph_csr": (_int, [_u64, _i64, _i64, _i64, _p, _p, _p, _p]),
     "xt_serve_create": (_int, [_p, ctypes.POINTER(XtServeConfig), _pp]),
     "xt_serve_submit": (_int, [_p, _i64, _i64, _p, _p, _dbl, _p]),
+    "xt_serve_submit_borrowed": (_int, [_p, _i64, _i64, _p, _p, _dbl, _p]),
     "xt_serve_poll": (_int, [_p, _u64, _p, _p]),
     "xt_serve_drain": (_int, [_p, _dbl]),
     "xt_serve_stats_get": (_int, [_p, ctypes.POINTER(XtServeStats)]),

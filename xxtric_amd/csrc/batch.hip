@@ -393,6 +393,39 @@ int batch_stage_host(xt_batch* b, int64_t nv, int64_t ne, const int64_t* off, co
   return XT_OK;
 }
 
+// Reserves one graph's place in the pinned mirror (offsets at *off_dst, ids at
+// *tails_dst) without copying it; the caller fills both before the next launch.
+int batch_stage_layout(xt_batch* b, int64_t nv, int64_t ne, int64_t** off_dst, int32_t** tails_dst) {
+  XT_TRY(batch_check_room(b, nv, ne));
+  XT_TRY(batch_stage_reserve(b));
+  XT_TRY(batch_stage_idle(b));
+  *off_dst = b->h_loff + b->LO;
+  *tails_dst = b->h_ltails + b->E;
+  b->graphs.push_back(BatchGraph{nv, ne, b->V, b->E, b->LO});
+  b->V += nv;
+  b->E += ne;
+  b->LO += nv + 1;
+  return XT_OK;
+}
+
+// Checks a host graph and copies it into the pinned mirror in the same pass
+// (the serving path's borrowed requests).  Returns the check's message (the
+// graph is then not staged) or nullptr; *rc carries a staging error.
+const char* batch_stage_host_checked(xt_batch* b, int64_t nv, int64_t ne, const int64_t* off,
+                                     const int32_t* tails, int* rc) {
+  *rc = batch_check_room(b, nv, ne);
+  if (*rc == XT_OK) *rc = batch_stage_reserve(b);
+  if (*rc == XT_OK) *rc = batch_stage_idle(b);
+  if (*rc != XT_OK) return nullptr;
+  const char* bad = check_host_graph(nv, ne, off, tails, b->h_loff + b->LO, b->h_ltails + b->E);
+  if (bad) return bad;
+  b->graphs.push_back(BatchGraph{nv, ne, b->V, b->E, b->LO});
+  b->V += nv;
+  b->E += ne;
+  b->LO += nv + 1;
+  return nullptr;
+}
+
 int batch_add_host(xt_batch* b, int64_t nv, int64_t ne, const int64_t* off, const int32_t* tails) {
   if (nv >= (1LL << 31) - 1) return set_err(b->ctx, XT_ERR_ARG, "nv must be below 2^31-1");
   const char* bad = check_host_graph(nv, ne, off, tails, nullptr, nullptr);

@@ -9,7 +9,9 @@
 // read back, their requests completed (latency = completion - arrival) and
 // their footprint released, which admits the next requests.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <cstring>
 #include <cmath>
 #include <condition_variable>
 #include <deque>
@@ -29,6 +31,9 @@ const char* check_host_graph(int64_t nv, int64_t ne, const int64_t* off, const i
                              int64_t* off_out, int32_t* tails_out);
 int batch_stage_reserve(xt_batch* b);
 int batch_stage_host(xt_batch* b, int64_t nv, int64_t ne, const int64_t* off, const int32_t* tails);
+const char* batch_stage_host_checked(xt_batch* b, int64_t nv, int64_t ne, const int64_t* off,
+                                     const int32_t* tails, int* rc);
+int batch_stage_layout(xt_batch* b, int64_t nv, int64_t ne, int64_t** off_dst, int32_t** tails_dst);
 }  // namespace xt
 
 using namespace xt;
@@ -47,10 +52,122 @@ struct Req {
   int64_t nv = 0, ne = 0;
   std::unique_ptr<int64_t[]> off;  // the request's copy (not zero-filled first)
   std::unique_ptr<int32_t[]> tails;
+  const int64_t* b_off = nullptr;  // borrowed caller arrays (xt_serve_submit_borrowed)
+  const int32_t* b_tails = nullptr;
   double arrival_us = 0, done_us = 0;
   uint64_t total = 0;
   int64_t footprint = 0;
   int state = kQueued;
+  int err = XT_OK;  // a borrowed request that failed its check
+};
+
+// Host threads that copy admitted requests into a slot's pinned staging
+// mirror in parallel, ranges of at most kCopyRange elements, checking borrowed
+// requests in the same pass (offsets monotone, ids in [0, nv)).
+constexpr int64_t kCopyRange = 1 << 22;
+
+struct CopyTask {
+  const int64_t* off = nullptr;  // offsets [lo, hi) (off != nullptr) or ids [lo, hi)
+  const int32_t* tails = nullptr;
+  int64_t* off_dst = nullptr;
+  int32_t* tails_dst = nullptr;
+  int64_t nv = 0, lo = 0, hi = 0;
+  bool check = false;
+  std::atomic<int>* bad = nullptr;
+};
+
+void run_copy(const CopyTask& t) {
+  if (t.off) {
+    int64_t down = 0;
+    for (int64_t i = t.lo; i < t.hi; i++) {
+      t.off_dst[i] = t.off[i];
+      if (t.check && i < t.nv) down |= (int64_t)(t.off[i + 1] < t.off[i]);
+    }
+    if (down) t.bad->store(1, std::memory_order_relaxed);
+  } else if (t.check) {
+    uint32_t out = 0;
+    const uint32_t n = (uint32_t)t.nv;
+    for (int64_t e = t.lo; e < t.hi; e++) {
+      const int32_t v = t.tails[e];
+      t.tails_dst[e] = v;
+      out |= (uint32_t)((uint32_t)v >= n);
+    }
+    if (out) t.bad->store(1, std::memory_order_relaxed);
+  } else {
+    std::memcpy(t.tails_dst + t.lo, t.tails + t.lo, (size_t)(t.hi - t.lo) * sizeof(int32_t));
+  }
+}
+
+// A run hands out task indices from one counter; a thread takes part in a run
+// only after registering (under the lock, while the run's task list is
+// current), and run() returns once every task is done and every registered
+// thread has left, so no thread keeps a run's task list past it.
+class CopyPool {
+ public:
+  explicit CopyPool(int n) {
+    for (int i = 0; i < n; i++) th_.emplace_back([this] { loop(); });
+  }
+  ~CopyPool() {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    for (std::thread& t : th_) t.join();
+  }
+  // Runs every task (the calling thread helps) and returns when all are done.
+  void run(const std::vector<CopyTask>& tasks) {
+    if (tasks.empty()) return;
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      tasks_ = &tasks;
+      next_.store(0);
+      left_ = tasks.size();
+      active_ = 1;  // this thread
+      gen_++;
+    }
+    cv_.notify_all();
+    work();
+    std::unique_lock<std::mutex> lk(mu_);
+    done_cv_.wait(lk, [&] { return left_ == 0 && active_ == 0; });
+    tasks_ = nullptr;
+  }
+
+ private:
+  void work() {  // the caller is registered in active_
+    const std::vector<CopyTask>& ts = *tasks_;
+    size_t finished = 0;
+    for (size_t i = next_.fetch_add(1); i < ts.size(); i = next_.fetch_add(1)) {
+      run_copy(ts[i]);
+      finished++;
+    }
+    std::lock_guard<std::mutex> lk(mu_);
+    left_ -= finished;
+    active_--;
+    if (left_ == 0 && active_ == 0) done_cv_.notify_all();
+  }
+  void loop() {
+    uint64_t seen = 0;
+    while (true) {
+      {
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_.wait(lk, [&] { return stop_ || (gen_ != seen && tasks_ != nullptr); });
+        if (stop_) return;
+        seen = gen_;
+        active_++;
+      }
+      work();
+    }
+  }
+  std::vector<std::thread> th_;
+  std::mutex mu_;
+  std::condition_variable cv_, done_cv_;
+  const std::vector<CopyTask>* tasks_ = nullptr;
+  std::atomic<size_t> next_{0};
+  size_t left_ = 0;
+  int active_ = 0;
+  uint64_t gen_ = 0;
+  bool stop_ = false;
 };
 
 struct ServeSlot {
@@ -78,6 +195,7 @@ struct xt_server {
   int64_t submitted = 0, completed = 0, rejected = 0, batches = 0;
   int worker_err = XT_OK;
   bool stop = false;
+  std::unique_ptr<CopyPool> pool;
   std::thread worker;
   std::chrono::steady_clock::time_point t0;
 
@@ -100,6 +218,7 @@ void complete_slot(xt_server* s, ServeSlot& sl) {
     r->done_us = t;
     r->state = kDone;
     s->resident -= r->footprint;
+    if (r->err != XT_OK) continue;  // failed its check: counted as rejected, no latency
     s->latencies.push_back(t - r->arrival_us);
     s->completed++;
     s->last_done = t;
@@ -156,16 +275,49 @@ void worker_main(xt_server* s) {
           G++;
         }
       }
-      for (Req* r : take) {
-        int rc = batch_stage_host(free_slot->b, r->nv, r->ne, r->off.get(), r->tails.get());
+      // Lay the admitted requests out in the slot's pinned mirror, then copy
+      // (and check the borrowed ones) with the copy pool.
+      std::vector<CopyTask> tasks;
+      std::vector<int64_t*> odst(take.size());
+      std::vector<int32_t*> tdst(take.size());
+      std::unique_ptr<std::atomic<int>[]> bad(new std::atomic<int>[take.size()]);
+      for (size_t i = 0; i < take.size(); i++) {
+        Req* r = take[i];
+        bad[i].store(0);
+        int rc = batch_stage_layout(free_slot->b, r->nv, r->ne, &odst[i], &tdst[i]);
         if (rc != XT_OK) {
           std::lock_guard<std::mutex> lk(s->mu);
           s->worker_err = rc;
+          break;
         }
         free_slot->reqs.push_back(r);
         free_slot->V += r->nv;
         free_slot->E += r->ne;
-        r->off.reset();  // host copy no longer needed
+        const bool borrowed = r->b_off != nullptr;
+        const int64_t* off = borrowed ? r->b_off : r->off.get();
+        const int32_t* tails = borrowed ? r->b_tails : r->tails.get();
+        if (borrowed && (off[0] != 0 || off[r->nv] != r->ne)) bad[i].store(1);
+        for (int64_t lo = 0; lo <= r->nv; lo += kCopyRange)
+          tasks.push_back(CopyTask{off, nullptr, odst[i], nullptr, r->nv, lo, std::min(r->nv + 1, lo + kCopyRange),
+                                   borrowed, &bad[i]});
+        for (int64_t lo = 0; lo < r->ne; lo += kCopyRange)
+          tasks.push_back(CopyTask{nullptr, tails, nullptr, tdst[i], r->nv, lo, std::min(r->ne, lo + kCopyRange),
+                                   borrowed, &bad[i]});
+      }
+      s->pool->run(tasks);
+      for (size_t i = 0; i < free_slot->reqs.size() && i < take.size(); i++) {
+        Req* r = take[i];
+        if (bad[i].load()) {
+          // Keep the slot's layout: the graph becomes rows of vertex 0 ids
+          // (valid, never reported) and the request fails at poll.
+          for (int64_t v = 0; v < r->nv; v++) odst[i][v] = 0;
+          odst[i][r->nv] = r->ne;
+          if (r->ne > 0) std::memset(tdst[i], 0, (size_t)r->ne * sizeof(int32_t));
+          std::lock_guard<std::mutex> lk(s->mu);
+          r->err = XT_ERR_INPUT;
+          s->rejected++;
+        }
+        r->off.reset();  // host copies no longer needed
         r->tails.reset();
       }
       if (!free_slot->reqs.empty()) {
@@ -218,6 +370,8 @@ int xt_serve_create(xt_ctx* ctx, const xt_serve_config* cfg, xt_server** out) {
       return rc;
     }
   }
+  const unsigned hc = std::thread::hardware_concurrency();
+  s->pool.reset(new CopyPool((int)std::min(7u, std::max(1u, hc / 2))));
   s->t0 = std::chrono::steady_clock::now();
   s->worker = std::thread(worker_main, s);
   *out = s;
@@ -266,6 +420,35 @@ int xt_serve_submit(xt_server* s, int64_t nv, int64_t ne, const int64_t* offsets
   return XT_OK;
 }
 
+int xt_serve_submit_borrowed(xt_server* s, int64_t nv, int64_t ne, const int64_t* offsets,
+                             const int32_t* tails, double arrival_us, uint64_t* ticket) {
+  if (!s || !ticket || nv < 1 || ne < 0 || !offsets || (ne > 0 && !tails)) return XT_ERR_ARG;
+  const int64_t fp = kBytesPerVertex * nv + kBytesPerEntry * ne;
+  std::unique_lock<std::mutex> lk(s->mu);
+  s->submitted++;
+  if (nv > s->slot_v || ne > s->slot_e || fp > s->cfg.hbm_budget_bytes ||
+      (int64_t)s->queue.size() >= s->cfg.max_queue) {
+    s->rejected++;
+    lk.unlock();
+    return set_err(s->ctx, XT_ERR_CAPACITY,
+                   "request rejected: larger than a batch slot, the HBM budget or the queue");
+  }
+  Req* r = new Req();
+  r->nv = nv;
+  r->ne = ne;
+  r->b_off = offsets;
+  r->b_tails = tails;
+  r->ticket = s->next_ticket++;
+  r->arrival_us = arrival_us >= 0 ? arrival_us : s->now();
+  r->footprint = fp;
+  if (s->first_arrival < 0 || r->arrival_us < s->first_arrival) s->first_arrival = r->arrival_us;
+  s->live[r->ticket] = r;
+  s->queue.push_back(r);
+  *ticket = r->ticket;
+  s->cv.notify_all();
+  return XT_OK;
+}
+
 int xt_serve_poll(xt_server* s, uint64_t ticket, uint64_t* total, double* latency_us) {
   if (!s) return XT_ERR_ARG;
   std::lock_guard<std::mutex> lk(s->mu);
@@ -274,6 +457,12 @@ int xt_serve_poll(xt_server* s, uint64_t ticket, uint64_t* total, double* latenc
   if (it == s->live.end()) return XT_ERR_NOTFOUND;
   Req* r = it->second;
   if (r->state != kDone) return XT_PENDING;
+  if (r->err != XT_OK) {  // a borrowed request that failed its check
+    const int err = r->err;
+    s->live.erase(it);
+    delete r;
+    return set_err(s->ctx, err, "request rejected: adjacency out of range or offsets not monotone");
+  }
   if (total) *total = r->total;
   if (latency_us) *latency_us = r->done_us - r->arrival_us;
   s->live.erase(it);
