@@ -105,6 +105,10 @@ __device__ __forceinline__ EdgeLists edge_lists(int64_t a0, int64_t a1, int64_t 
   return L;
 }
 
+// Light edges of one entry slot in a wave beyond which each lane intersects its
+// own instead of the whole wave taking them one by one.
+constexpr int kWaveWideLight = 4;
+
 // Lane l's value of v, in every lane (l uniform).
 __device__ __forceinline__ int64_t readlane64(int64_t v, int l) {
   const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
@@ -200,7 +204,7 @@ __device__ __forceinline__ void load_ids(const int32_t* __restrict__ ot, int64_t
 // saturated out-degree classifies every entry (both rows see the same
 // saturated pair) and the owned and light entries gather their offsets.
 template <typename T, bool NARROW, int CM, bool BATCH>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_light_items(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_light_items(
     int64_t nv, int64_t ne, const int64_t* __restrict__ off, const T* __restrict__ tails,
     const uint8_t* __restrict__ oflags, const int64_t* __restrict__ ooff, const uint64_t* __restrict__ vrec,
     const uint16_t* __restrict__ dp16, const int32_t* __restrict__ ot, const int64_t* __restrict__ crows,
@@ -330,10 +334,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8))) 
   __syncthreads();
   {
     // 5. Along the rows again: write the owned items and intersect the light
-    //    edges.  A light edge is intersected by the whole wave, one edge at a
-    //    time: the lanes take the ids of the shorter list and search each in
-    //    the longer one (both at most light_cut long), instead of one lane
-    //    walking the list while the others wait.
+    //    edges.  Where few lanes hold one (R-MAT: one light edge per few
+    //    hundred entries), a light edge is intersected by the whole wave, one
+    //    edge at a time: the lanes take the ids of the shorter list and search
+    //    each in the longer one (both at most light_cut long), instead of one
+    //    lane walking the list while the others wait.  Where many do (small
+    //    sparse graphs), each lane walks its own edge.
     const bool live = e0 < ee;
     int64_t u = u0, cur = 0, nxt = 0, a0 = 0, a1 = 0;
     if (live) {
@@ -367,6 +373,40 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8))) 
         }
       }
       uint64_t pend = __builtin_amdgcn_ballot_w64(act && ((light >> j) & 1u));
+      if (__popcll(pend) > kWaveWideLight) {  // uniform: many light edges, one per lane
+        if (act && ((light >> j) & 1u)) {
+          const int64_t v = ids[j];
+          const EdgeLists L = edge_lists(a0, a1, (int64_t)gb[j], (int64_t)gb[j] + (int64_t)gl[j]);
+          const int64_t mmin = u < v ? u : v;
+          uint32_t h = 0;
+          int64_t pos = L.l0;
+          for (int64_t k = L.s0; k < L.s1 && pos < L.l1; k++) {
+            const int32_t w = ot[k];
+            pos = lower_bound_dev<int32_t>(ot, pos, L.l1, w);  // restarted after the previous hit
+            if (pos < L.l1 && ot[pos] == w) {
+              h++;
+              pos++;
+              if constexpr (PV) {
+                atomicAdd(&pv[(int64_t)w < mmin ? (int64_t)w : mmin], 1ull);
+              } else if constexpr (CM == kCreditRange) {
+                range_credit(u, v, w, rlo, rhi, r_owned, r_local);
+              }
+            }
+          }
+          if constexpr (BATCH) {
+            if (h) {
+              const int32_t slot = gid[u] - g0;
+              if (slot < kGraphSlots)
+                atomicAdd(&s_g[slot], (unsigned long long)h);
+              else
+                atomicAdd(&gtotal[gid[u]], (unsigned long long)h);
+            }
+          } else {
+            hits_all += h;
+          }
+        }
+        pend = 0;
+      }
       while (pend) {  // uniform
         const int src = __builtin_ctzll(pend);
         pend &= pend - 1;
