@@ -50,8 +50,8 @@ typedef struct xt_timing {
   double orient_ms;   /* degree keys + out-degree count + scans + oriented fill */
   double count_ms;    /* intersection + reduction kernels (the three phases below) */
   double total_ms;    /* first to last event of the count */
-  double light_ms;    /* one pass over the input rows: light edges counted one per
-                         thread, heavy edges listed under their owners */
+  double light_ms;    /* one pass over the input rows: light edges intersected,
+                         heavy edges listed under their owners */
   double items_ms;    /* tiles of the owners' item lists (counts, scan, descriptors) */
   double group_ms;    /* persistent group kernel: staged list x streamed lists */
   int64_t ne_plus;    /* oriented edges |E+| */

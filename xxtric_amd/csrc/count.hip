@@ -5,9 +5,10 @@
 // Out-lists hold ascending ids.  d+(x) = |N+(x)|.
 //
 // Edges are binned by the longer of their two out-lists:
-//  * light edges (both lists <= light_cut): one thread per edge walks the
-//    shorter list and searches each id in the longer one (lower_bound restarted
-//    after the previous hit);
+//  * light edges (both lists <= light_cut): intersected directly, each id of
+//    the shorter list searched in the longer one -- by the whole wave, one
+//    edge at a time, where light edges are sparse, or one lane per edge where
+//    they are dense;
 //  * heavy edges: the edge belongs to its endpoint x with the longer list (ties
 //    to the source).  Each such x gets an item list -- one 16-byte record
 //    {offset, length, id} per other endpoint y of its heavy edges.
@@ -20,10 +21,11 @@
 // buckets, and streams the items' shorter lists N+(y) through it: the tile's
 // lists are concatenated into one run of 16-byte chunks that all 512 threads
 // read with coalesced loads, the next round of loads in flight while the
-// current one is probed.  Each oriented edge reads only its shorter list; the
-// longer one is read once per tile.
-// Reductions: per-thread uint64 -> wave64 shuffle tree -> LDS -> one 64-bit
-// atomic per workgroup (per tile for the group kernel).
+// current one is probed; compare results are lane masks, counted per wave on
+// the scalar unit.  Each oriented edge reads only its shorter list; the longer
+// one is read once per tile.
+// Reductions: per-thread (and per-wave scalar) counters -> wave64 shuffle tree
+// -> LDS -> one 64-bit atomic per workgroup (per tile for the group kernel).
 //
 // Per-vertex credit (XT_PER_VERTEX): the triangle {a, b, w} belongs to
 // min(a, b, w) (xxtric.hpp:165); hits credited to the edge's endpoints are
