@@ -74,6 +74,11 @@ int xt_last_timing(const xt_ctx* ctx, xt_timing* out);
  * in flight per streaming round (1, 2 or 4, default 2).  None of them changes
  * a result. */
 int xt_ctx_tune(xt_ctx* ctx, int light_cut, int group_blocks, int group_round);
+/* on != 0: count every graph with the layout used for graphs of 2^32 or more
+ * entries (64-bit list offsets gathered by the light pass, 16-bit saturated
+ * out-degrees for classification) -- for tests and comparisons on graphs of
+ * any size.  Changes no result. */
+int xt_ctx_force_wide(xt_ctx* ctx, int on);
 
 /* ---- device memory plumbing ------------------------------------------------
  * The CudaMalloc / CudaMemcpyHtoD / CudaMemcpyDtoH / CudaFree /

@@ -298,3 +298,41 @@ def test_resident_graph_and_alg_bytes(ctx):
     s = np.minimum(dplus[rows[keep]], dplus[tails[keep]]).sum()
     assert sm == s and b == 8 * (n + 1) + 4 * nep + 4 * s
     g.close()
+
+
+@pytest.mark.parametrize("small_cut", [1, 32, 1 << 30])
+def test_wide_layout_gives_the_same_counts(ctx, small_cut):
+    """The layout of graphs with 2^32 or more entries (64-bit list offsets, saturated
+    16-bit out-degrees), forced on graphs small enough for the oracle: global,
+    per-vertex, owned-range and batch counts are unchanged."""
+    off, tails = O.rmat_csr(13, 16, 2, True)
+    off = np.asarray(off, np.int64)
+    tails = np.asarray(tails, np.int64)
+    n = len(off) - 1
+    exp, exp_pv = O.count_reference(off, tails, per_vertex=True)
+    hub_o, hub_t = wheel(70000)  # one vertex of degree 69999: its saturated key ties and breaks by id
+    hub_exp = O.count_degree_ordered(hub_o, hub_t)
+    hub_exp = hub_exp[0] if isinstance(hub_exp, tuple) else hub_exp
+    ctx.tune(small_cut=small_cut)
+    ctx.force_wide(True)
+    try:
+        _say("wide layout, cut", small_cut)
+        for orient in ("degree", "id"):
+            tot, pv = ctx.count_csr(off, tails, per_vertex=True, orient=orient)
+            assert tot == exp and (pv == exp_pv).all(), orient
+            assert ctx.count_csr(off, tails, orient=orient) == exp
+        g = ctx.graph(off, tails)
+        owned, _, tot = g.count_owned(0, n // 3)
+        assert tot == exp and owned == int(exp_pv[: n // 3].sum())
+        g.close()
+        assert ctx.count_csr(hub_o, hub_t) == hub_exp
+        b = xxtric_amd.Batch(ctx, 2_000_000, 40_000_000, 64)
+        gs = [xxtric_amd.batch_graph_csr(2026, i) for i in range(40)]
+        for o, t in gs:
+            b.add(o, t)
+        b.run()
+        assert list(b.results()) == [O.count_degree_ordered(o, t) for o, t in gs]
+        b.close()
+    finally:
+        ctx.force_wide(False)
+        ctx.tune(small_cut=xxtric_amd.DEFAULT_LIGHT_CUT, group_blocks=4, group_round=2)

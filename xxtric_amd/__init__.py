@@ -75,6 +75,10 @@ class Context:
         chunks in flight per thread (0 = keep).  No setting changes a result."""
         self._check(self._L.xt_ctx_tune(self.h, small_cut, group_blocks, group_round))
 
+    def force_wide(self, on=True):
+        """Count with the layout of graphs of 2^32 or more entries (tests, comparisons)."""
+        self._check(self._L.xt_ctx_force_wide(self.h, 1 if on else 0))
+
     def timing(self):
         t = capi.XtTiming()
         self._check(self._L.xt_last_timing(self.h, ctypes.byref(t)))

@@ -238,6 +238,13 @@ int xt_ctx_tune(xt_ctx* ctx, int light_cut, int group_blocks, int group_round) {
   return XT_OK;
 }
 
+int xt_ctx_force_wide(xt_ctx* ctx, int on) {
+  if (!ctx) return XT_ERR_ARG;
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  ctx->force_wide = on != 0;
+  return XT_OK;
+}
+
 int xt_dev_alloc(xt_ctx* ctx, size_t bytes, void** out) {
   if (!ctx || !out) return XT_ERR_ARG;
   std::lock_guard<std::recursive_mutex> lk(ctx->mu);

@@ -1000,7 +1000,8 @@ int count_oriented(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne_bound, const 
   const int cut = ctx->small_cut;
   int64_t g = (nv + 1 + 255) / 256;
   if (g > 65536) g = 65536;
-  const bool narrow = ne_bound < (1ll << 32);  // every oriented offset and out-degree fits 32 bits
+  // every oriented offset and out-degree fits 32 bits (unless the wide layout is forced)
+  const bool narrow = ne_bound < (1ll << 32) && !ctx->force_wide;
   uint64_t* vrec = nullptr;
   uint16_t* dp16 = nullptr;
   if (narrow) {

@@ -88,6 +88,7 @@ struct xt_ctx {
   int small_cut = 32;     // light edges: both out-lists at most this long
   int group_blocks = 4;   // persistent group-kernel workgroups per CU
   int group_round = 2;    // 16-byte chunks per thread per group-kernel streaming round (1, 2, 4)
+  bool force_wide = false;  // use the 64-bit offset layout of graphs with >= 2^32 entries everywhere
 };
 
 struct xt_graph {
