@@ -139,6 +139,16 @@ int xt_graph_rmat(xt_ctx* ctx, int scale, int edgefactor, double a, double b, do
                   uint64_t seed, int permute, xt_graph** out);
 /* Seeded G(n,p): edge {i<j} iff (mix64(mix64(seed) + i*n + j) >> 32) < thr64. */
 int xt_graph_er(xt_ctx* ctx, int64_t n, uint64_t seed, uint64_t thr64, xt_graph** out);
+/* Edge list -> resident symmetric CSR, built on the device: pairs (src[i],
+ * dst[i]), i < m, in any order and direction, ids in [0, nv) (else
+ * XT_ERR_INPUT); self-loops dropped, duplicates merged, both directions kept,
+ * rows ascending (a radix sort of (u, v) keys).  The reference builds its CSR
+ * on the host (graph.hpp:1289-1341).  1 <= nv < 2^31-1. */
+int xt_graph_from_edges(xt_ctx* ctx, int64_t nv, int64_t m, const int64_t* src, const int64_t* dst,
+                        xt_graph** out);
+/* xt_graph_from_edges, xt_graph_count and xt_graph_destroy in one call. */
+int xt_count_edges(xt_ctx* ctx, int64_t nv, int64_t m, const int64_t* src, const int64_t* dst, uint32_t flags,
+                   uint64_t* total, uint64_t* per_vertex);
 int xt_graph_info(const xt_graph* g, int64_t* nv, int64_t* ne);
 /* Copy the CSR back to host arrays (offsets nv+1, tails ne). */
 int xt_graph_download(xt_ctx* ctx, const xt_graph* g, int64_t* offsets, int32_t* tails);

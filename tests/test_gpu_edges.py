@@ -1,0 +1,78 @@
+"""GPU: edge list -> symmetric CSR built on the device (xt_graph_from_edges,
+xt_count_edges).  The device CSR must equal the host-built one exactly
+(duplicates merged, both directions, self-loops dropped, rows ascending), and
+counts must equal the oracle's."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_py as O
+import xxtric_amd
+from helpers import csr_from_edges
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def _messy_edges(n, m, seed):
+    rng = np.random.default_rng(seed)
+    src = rng.integers(0, n, m)
+    dst = rng.integers(0, n, m)
+    # repeat some pairs, in both directions, and add self-loops
+    k = m // 5
+    src = np.concatenate([src, dst[:k], src[:k], np.arange(10) % n])
+    dst = np.concatenate([dst, src[:k], dst[:k], np.arange(10) % n])
+    return src.astype(np.int64), dst.astype(np.int64)
+
+
+@pytest.mark.parametrize("n,m,seed", [(2, 3, 1), (1000, 5000, 2), (5000, 80000, 3), (1 << 14, 200000, 4)])
+def test_device_csr_equals_host_csr(ctx, n, m, seed):
+    src, dst = _messy_edges(n, m, seed)
+    pairs = [(int(a), int(b)) for a, b in zip(src, dst) if a != b]
+    exp_off, exp_tails = csr_from_edges(n, pairs)
+    _say("from_edges", n, m)
+    g = ctx.graph_from_edges(n, src, dst)
+    off, tails = g.download()
+    assert (off == exp_off).all() and (tails.astype(np.int64) == exp_tails).all()
+    exp, exp_pv = O.count_reference(exp_off, exp_tails, per_vertex=True)
+    tot, pv = g.count(per_vertex=True)
+    assert tot == exp and (pv == exp_pv).all()
+    assert ctx.count_edges(n, src, dst) == exp
+    tot2, pv2 = ctx.count_edges(n, src, dst, per_vertex=True, orient="id")
+    assert tot2 == exp and (pv2 == exp_pv).all()
+    g.close()
+
+
+def test_rmat_directed_list_round_trip(ctx):
+    off, tails = O.rmat_csr(14, 16, 6, True)
+    off = np.asarray(off, np.int64)
+    tails = np.asarray(tails, np.int64)
+    n = len(off) - 1
+    rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(off))
+    up = rows < tails  # one direction of each edge, shuffled
+    src, dst = rows[up], tails[up]
+    perm = np.random.default_rng(9).permutation(len(src))
+    _say("rmat14 round trip")
+    g = ctx.graph_from_edges(n, dst[perm], src[perm])
+    o2, t2 = g.download()
+    assert (o2 == off).all() and (t2.astype(np.int64) == tails).all()
+    assert g.count() == O.count_degree_ordered(off, tails)
+    g.close()
+
+
+def test_empty_and_bad_edge_lists(ctx):
+    g = ctx.graph_from_edges(5, np.zeros(0, np.int64), np.zeros(0, np.int64))
+    assert g.ne == 0 and g.count() == 0
+    g.close()
+    assert ctx.count_edges(3, [0, 1, 2, 2], [0, 1, 2, 2]) == 0  # self-loops only
+    with pytest.raises(xxtric_amd.XtError) as e:
+        ctx.graph_from_edges(4, [0, 1], [1, 4])  # id 4 outside [0, 4)
+    assert e.value.code == -4
+    with pytest.raises(xxtric_amd.XtError) as e:
+        ctx.graph_from_edges(0, [0], [0])
+    assert e.value.code == -1

@@ -107,6 +107,27 @@ class Context:
                                               ctypes.byref(h)))
         return DeviceGraph(self, h)
 
+    def graph_from_edges(self, nv, src, dst):
+        """Symmetric CSR built on the device from an edge list (any order or direction,
+        duplicates merged, self-loops dropped)."""
+        src = np.ascontiguousarray(src, np.int64)
+        dst = np.ascontiguousarray(dst, np.int64)
+        if len(src) != len(dst):
+            raise ValueError("src and dst differ in length")
+        h = ctypes.c_void_p()
+        self._check(self._L.xt_graph_from_edges(self.h, int(nv), len(src), capi.ptr(src), capi.ptr(dst),
+                                                ctypes.byref(h)))
+        return DeviceGraph(self, h)
+
+    def count_edges(self, nv, src, dst, per_vertex=False, orient="degree"):
+        src = np.ascontiguousarray(src, np.int64)
+        dst = np.ascontiguousarray(dst, np.int64)
+        total = ctypes.c_uint64()
+        pv = np.zeros(int(nv), np.uint64) if per_vertex else None
+        self._check(self._L.xt_count_edges(self.h, int(nv), len(src), capi.ptr(src), capi.ptr(dst),
+                                           _flags(orient, per_vertex), ctypes.byref(total), capi.ptr(pv)))
+        return (total.value, pv) if per_vertex else total.value
+
     def rmat(self, scale, edgefactor=16, seed=1, permute=True, abc=(0.57, 0.19, 0.19)):
         h = ctypes.c_void_p()
         self._check(self._L.xt_graph_rmat(self.h, scale, edgefactor, abc[0], abc[1], abc[2], seed,

@@ -433,6 +433,37 @@ int xt_graph_er(xt_ctx* ctx, int64_t n, uint64_t seed, uint64_t thr64, xt_graph*
   return XT_OK;
 }
 
+int xt_graph_from_edges(xt_ctx* ctx, int64_t nv, int64_t m, const int64_t* src, const int64_t* dst,
+                        xt_graph** out) {
+  if (!ctx || !out) return XT_ERR_ARG;
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  XT_HIP(ctx, hipSetDevice(ctx->device));
+  *out = nullptr;
+  if (nv < 1 || nv >= (1LL << 31) - 1 || m < 0 || (m > 0 && (!src || !dst)))
+    return set_err(ctx, XT_ERR_ARG, "edge list: need 1 <= nv < 2^31-1, m >= 0 and both id arrays");
+  xt_graph* g = new xt_graph();
+  const int rc = graph_from_edges(ctx, nv, m, src, dst, g);
+  if (rc != XT_OK) {
+    if (g->off) (void)hipFree(g->off);
+    if (g->tails) (void)hipFree(g->tails);
+    delete g;
+    return rc;
+  }
+  *out = g;
+  return XT_OK;
+}
+
+int xt_count_edges(xt_ctx* ctx, int64_t nv, int64_t m, const int64_t* src, const int64_t* dst, uint32_t flags,
+                   uint64_t* total, uint64_t* per_vertex) {
+  if (!ctx || !total) return XT_ERR_ARG;
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  xt_graph* g = nullptr;
+  XT_TRY(xt_graph_from_edges(ctx, nv, m, src, dst, &g));
+  const int rc = xt_graph_count(ctx, g, flags, total, per_vertex);
+  xt_graph_destroy(ctx, g);
+  return rc;
+}
+
 int xt_graph_info(const xt_graph* g, int64_t* nv, int64_t* ne) {
   if (!g) return XT_ERR_ARG;
   if (nv) *nv = g->nv;

@@ -94,23 +94,20 @@ static unsigned grid_of(int64_t n) {
   return (unsigned)(g > 65536 ? 65536 : g);
 }
 
-int gen_rmat(xt_ctx* ctx, int scale, int edgefactor, const uint32_t thr[3], uint64_t seed,
-             int permute, xt_graph* g) {
-  if (scale < 1 || scale > 31 || edgefactor < 1) return set_err(ctx, XT_ERR_ARG, "R-MAT scale must be in [1,31]");
+// Sorted, merged (u << bits | v) keys -> CSR: radix sort over key_bits bits,
+// first of each run kept (the sentinel dropped), ids scattered to their rank,
+// row lengths counted and scanned.  ka/kb hold nk+1 keys each (caller-owned;
+// kb doubles as the rank buffer).
+static int build_csr_from_keys(xt_ctx* ctx, uint64_t* ka, uint64_t* kb, int64_t nk, int key_bits, int bits,
+                               int64_t nv, uint64_t sentinel, xt_graph* g) {
   hipStream_t st = ctx->stream;
-  const int64_t nv = 1LL << scale;
-  const int64_t nt = (int64_t)edgefactor * nv;
-  const int64_t nk = 2 * nt;
-  const uint64_t sentinel = (1ULL << (2 * scale)) - 1;  // (2^s-1, 2^s-1): a self-loop, never a real edge
-  const uint64_t key = mix64(seed);
-  uint64_t *ka = nullptr, *kb = nullptr;
   uint32_t* flags = nullptr;
+  uint32_t* deg = nullptr;
   void* tmp = nullptr;
   int rc = XT_OK;
   auto cleanup = [&]() {
-    if (ka) (void)hipFree(ka);
-    if (kb) (void)hipFree(kb);
     if (flags) (void)hipFree(flags);
+    if (deg) (void)hipFree(deg);
     if (tmp) (void)hipFree(tmp);
   };
 #define GEN_HIP(expr)                                   \
@@ -122,23 +119,20 @@ int gen_rmat(xt_ctx* ctx, int scale, int edgefactor, const uint32_t thr[3], uint
       return rc;                                        \
     }                                                   \
   } while (0)
-  GEN_HIP(hipMalloc(&ka, (size_t)(nk + 1) * sizeof(uint64_t)));
-  GEN_HIP(hipMalloc(&kb, (size_t)(nk + 1) * sizeof(uint64_t)));
-  hipLaunchKernelGGL(k_rmat_keys, dim3(grid_of(nt)), dim3(256), 0, st, scale, nt, key, thr[0], thr[1],
-                     thr[2], permute, sentinel, ka);
-  GEN_HIP(hipGetLastError());
   rocprim::double_buffer<uint64_t> db(ka, kb);
   size_t tmp_bytes = 0;
-  GEN_HIP(rocprim::radix_sort_keys(nullptr, tmp_bytes, db, (size_t)nk, 0u, (unsigned)(2 * scale), st));
+  GEN_HIP(rocprim::radix_sort_keys(nullptr, tmp_bytes, db, (size_t)nk, 0u, (unsigned)key_bits, st));
   GEN_HIP(hipMalloc(&tmp, tmp_bytes > 0 ? tmp_bytes : 1));
-  GEN_HIP(rocprim::radix_sort_keys(tmp, tmp_bytes, db, (size_t)nk, 0u, (unsigned)(2 * scale), st));
+  GEN_HIP(rocprim::radix_sort_keys(tmp, tmp_bytes, db, (size_t)nk, 0u, (unsigned)key_bits, st));
   (void)hipFree(tmp);
   tmp = nullptr;
   uint64_t* sorted = db.current();
   int64_t* pos = reinterpret_cast<int64_t*>(db.alternate());  // nk+1 entries, free after the sort
-  GEN_HIP(hipMalloc(&flags, (size_t)nk * sizeof(uint32_t)));
-  hipLaunchKernelGGL(k_mark_unique, dim3(grid_of(nk)), dim3(256), 0, st, sorted, nk, sentinel, flags);
-  GEN_HIP(hipGetLastError());
+  GEN_HIP(hipMalloc(&flags, (size_t)(nk > 0 ? nk : 1) * sizeof(uint32_t)));
+  if (nk > 0) {
+    hipLaunchKernelGGL(k_mark_unique, dim3(grid_of(nk)), dim3(256), 0, st, sorted, nk, sentinel, flags);
+    GEN_HIP(hipGetLastError());
+  }
   rc = scan_u32_to_i64(ctx, ctx->s, flags, nk, pos, st);
   if (rc != XT_OK) {
     cleanup();
@@ -151,14 +145,11 @@ int gen_rmat(xt_ctx* ctx, int scale, int edgefactor, const uint32_t thr[3], uint
   g->ne = ne;
   GEN_HIP(hipMalloc(&g->off, (size_t)(nv + 1) * sizeof(int64_t)));
   GEN_HIP(hipMalloc(&g->tails, (size_t)(ne > 0 ? ne : 1) * sizeof(int32_t)));
-  // deg counts live in the flags buffer's head once flags are consumed: use a
-  // separate small buffer instead to keep the scatter independent.
-  uint32_t* deg = nullptr;
   GEN_HIP(hipMalloc(&deg, (size_t)nv * sizeof(uint32_t)));
   rc = fill_u32(ctx, deg, nv, 0u, st);
-  if (rc == XT_OK) {
+  if (rc == XT_OK && nk > 0) {
     hipLaunchKernelGGL(k_scatter_unique, dim3(grid_of(nk)), dim3(256), 0, st, (const uint64_t*)sorted,
-                       nk, (const uint32_t*)flags, (const int64_t*)pos, scale, g->tails, deg);
+                       nk, (const uint32_t*)flags, (const int64_t*)pos, bits, g->tails, deg);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) rc = hip_err(ctx, e, "k_scatter_unique");
   }
@@ -167,9 +158,96 @@ int gen_rmat(xt_ctx* ctx, int scale, int edgefactor, const uint32_t thr[3], uint
     hipError_t e = hipStreamSynchronize(st);
     if (e != hipSuccess) rc = hip_err(ctx, e, "hipStreamSynchronize");
   }
-  (void)hipFree(deg);
   cleanup();
 #undef GEN_HIP
+  return rc;
+}
+
+int gen_rmat(xt_ctx* ctx, int scale, int edgefactor, const uint32_t thr[3], uint64_t seed,
+             int permute, xt_graph* g) {
+  if (scale < 1 || scale > 31 || edgefactor < 1) return set_err(ctx, XT_ERR_ARG, "R-MAT scale must be in [1,31]");
+  hipStream_t st = ctx->stream;
+  const int64_t nv = 1LL << scale;
+  const int64_t nt = (int64_t)edgefactor * nv;
+  const int64_t nk = 2 * nt;
+  const uint64_t sentinel = (1ULL << (2 * scale)) - 1;  // (2^s-1, 2^s-1): a self-loop, never a real edge
+  const uint64_t key = mix64(seed);
+  uint64_t *ka = nullptr, *kb = nullptr;
+  hipError_t e = hipMalloc(&ka, (size_t)(nk + 1) * sizeof(uint64_t));
+  if (e == hipSuccess) e = hipMalloc(&kb, (size_t)(nk + 1) * sizeof(uint64_t));
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_rmat_keys, dim3(grid_of(nt)), dim3(256), 0, st, scale, nt, key, thr[0], thr[1],
+                       thr[2], permute, sentinel, ka);
+    e = hipGetLastError();
+  }
+  const int rc = e == hipSuccess ? build_csr_from_keys(ctx, ka, kb, nk, 2 * scale, scale, nv, sentinel, g)
+                                 : hip_err(ctx, e, "R-MAT keys");
+  if (ka) (void)hipFree(ka);
+  if (kb) (void)hipFree(kb);
+  return rc;
+}
+
+// ---------------------------------------------------------------------------
+// Edge list -> symmetric CSR on the device: each pair (u, v) becomes the keys
+// (u << bits | v) and (v << bits | u); self-loops become the sentinel (the
+// self-loop of the largest representable id); ids outside [0, nv) are counted.
+__global__ __launch_bounds__(256) void k_edge_keys(int64_t m, const int64_t* __restrict__ src,
+                                                   const int64_t* __restrict__ dst, int64_t nv, int bits,
+                                                   uint64_t sentinel, uint64_t* __restrict__ keys,
+                                                   unsigned long long* __restrict__ bad) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  uint32_t nbad = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride) {
+    const int64_t u = src[i], v = dst[i];
+    const bool ok = u >= 0 && u < nv && v >= 0 && v < nv;
+    nbad += ok ? 0u : 1u;
+    if (!ok || u == v) {
+      keys[2 * i] = sentinel;
+      keys[2 * i + 1] = sentinel;
+    } else {
+      keys[2 * i] = ((uint64_t)u << bits) | (uint64_t)v;
+      keys[2 * i + 1] = ((uint64_t)v << bits) | (uint64_t)u;
+    }
+  }
+  if (nbad) atomicAdd(bad, (unsigned long long)nbad);
+}
+
+int graph_from_edges(xt_ctx* ctx, int64_t nv, int64_t m, const int64_t* src, const int64_t* dst, xt_graph* g) {
+  hipStream_t st = ctx->stream;
+  int bits = 1;
+  while ((1ll << bits) < nv) bits++;
+  const uint64_t sentinel = (1ULL << (2 * bits)) - 1;
+  const int64_t nk = 2 * m;
+  int64_t *ds = nullptr, *dd = nullptr;
+  uint64_t *ka = nullptr, *kb = nullptr;
+  unsigned long long* bad = nullptr;
+  int rc = XT_OK;
+  hipError_t e = hipMalloc(&ka, (size_t)(nk + 1) * sizeof(uint64_t));
+  if (e == hipSuccess) e = hipMalloc(&kb, (size_t)(nk + 1) * sizeof(uint64_t));
+  if (e == hipSuccess) e = hipMalloc(&bad, sizeof(unsigned long long));
+  if (e == hipSuccess && m > 0) e = hipMalloc(&ds, (size_t)m * sizeof(int64_t));
+  if (e == hipSuccess && m > 0) e = hipMalloc(&dd, (size_t)m * sizeof(int64_t));
+  if (e == hipSuccess) e = hipMemsetAsync(bad, 0, sizeof(unsigned long long), st);
+  if (e == hipSuccess && m > 0) e = hipMemcpyAsync(ds, src, (size_t)m * sizeof(int64_t), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess && m > 0) e = hipMemcpyAsync(dd, dst, (size_t)m * sizeof(int64_t), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess && m > 0) {
+    hipLaunchKernelGGL(k_edge_keys, dim3(grid_of(m)), dim3(256), 0, st, m, (const int64_t*)ds,
+                       (const int64_t*)dd, nv, bits, sentinel, ka, bad);
+    e = hipGetLastError();
+  }
+  unsigned long long nbad = 0;
+  if (e == hipSuccess) e = hipMemcpyAsync(&nbad, bad, sizeof(nbad), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess)
+    rc = hip_err(ctx, e, "edge list upload");
+  else if (nbad)
+    rc = set_err(ctx, XT_ERR_INPUT, "edge list id out of range [0, nv)");
+  if (ds) (void)hipFree(ds);
+  if (dd) (void)hipFree(dd);
+  if (bad) (void)hipFree(bad);
+  if (rc == XT_OK) rc = build_csr_from_keys(ctx, ka, kb, nk, 2 * bits, bits, nv, sentinel, g);
+  if (ka) (void)hipFree(ka);
+  if (kb) (void)hipFree(kb);
   return rc;
 }
 

@@ -159,6 +159,8 @@ int count_device_csr(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne, const int6
 int gen_rmat(xt_ctx* ctx, int scale, int edgefactor, const uint32_t thr[3], uint64_t seed,
              int permute, xt_graph* g);
 int gen_er(xt_ctx* ctx, int64_t n, uint64_t seed, uint64_t thr64, xt_graph* g);
+// Edge list (host pairs) -> symmetric, merged, self-loop-free CSR on the device.
+int graph_from_edges(xt_ctx* ctx, int64_t nv, int64_t m, const int64_t* src, const int64_t* dst, xt_graph* g);
 // Rows of [lo, hi) and of their neighbours kept, the others emptied (fills out).
 int graph_slice(xt_ctx* ctx, const xt_graph* g, int64_t lo, int64_t hi, xt_graph* out);
 
