@@ -146,6 +146,12 @@ int xt_graph_er(xt_ctx* ctx, int64_t n, uint64_t seed, uint64_t thr64, xt_graph*
  * on the host (graph.hpp:1289-1341).  1 <= nv < 2^31-1. */
 int xt_graph_from_edges(xt_ctx* ctx, int64_t nv, int64_t m, const int64_t* src, const int64_t* dst,
                         xt_graph** out);
+/* vite binary file (graph.hpp:294-395; layout32 selects USE_32_BIT_GRAPH)
+ * streamed into a resident graph: offsets rebased to 0 (graph.hpp:386-388)
+ * and checked monotone, ids checked in [0, nv), read in 4 M-element blocks
+ * through two pinned buffers (host memory stays bounded for s26-sized
+ * files).  flags: XT_VALIDATE / XT_CHECK_SYMMETRY as for xt_graph_from_csr. */
+int xt_graph_from_vite(xt_ctx* ctx, const char* path, int layout32, uint32_t flags, xt_graph** out);
 /* xt_graph_from_edges, xt_graph_count and xt_graph_destroy in one call. */
 int xt_count_edges(xt_ctx* ctx, int64_t nv, int64_t m, const int64_t* src, const int64_t* dst, uint32_t flags,
                    uint64_t* total, uint64_t* per_vertex);

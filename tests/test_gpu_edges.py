@@ -76,3 +76,43 @@ def test_empty_and_bad_edge_lists(ctx):
     with pytest.raises(xxtric_amd.XtError) as e:
         ctx.graph_from_edges(0, [0], [0])
     assert e.value.code == -1
+
+
+@pytest.mark.parametrize("layout32", [False, True])
+def test_vite_streamed_to_device(ctx, tmp_path, layout32):
+    """A vite file streamed into HBM (blocks of 4 M elements through pinned
+    buffers) equals the host reader's CSR; counts equal the oracle's.  A graph
+    of 5 M ids spans two id blocks."""
+    off, tails = O.rmat_csr(18, 10, 8, True)
+    off = np.asarray(off, np.int64)
+    tails = np.asarray(tails, np.int64)
+    assert len(tails) > (1 << 22)
+    path = str(tmp_path / ("g32.bin" if layout32 else "g64.bin"))
+    xxtric_amd.vite_write(path, off, tails, layout32=layout32)
+    _say("vite stream", layout32, len(tails))
+    g = ctx.graph_from_vite(path, layout32=layout32, validate=True, symmetry=True)
+    o2, t2 = g.download()
+    assert (o2 == off).all() and (t2.astype(np.int64) == tails).all()
+    assert g.count() == O.count_degree_ordered(off, tails)
+    g.close()
+
+
+def test_vite_stream_rejects_bad_files(ctx, tmp_path):
+    off = np.array([0, 2, 3, 4], np.int64)
+    tails = np.array([1, 2, 0, 0], np.int64)
+    good = str(tmp_path / "ok.bin")
+    xxtric_amd.vite_write(good, off, tails)
+    assert ctx.graph_from_vite(good).ne == 4
+    raw = open(good, "rb").read()
+    short = str(tmp_path / "short.bin")
+    open(short, "wb").write(raw[:-8])  # last record cut
+    with pytest.raises(xxtric_amd.XtError) as e:
+        ctx.graph_from_vite(short)
+    assert e.value.code == -4
+    bad = str(tmp_path / "badid.bin")
+    xxtric_amd.vite_write(bad, off, np.array([1, 2, 0, 9], np.int64))  # id 9 outside [0, 3)
+    with pytest.raises(xxtric_amd.XtError) as e:
+        ctx.graph_from_vite(bad)
+    assert e.value.code == -4
+    with pytest.raises(xxtric_amd.XtError):
+        ctx.graph_from_vite(str(tmp_path / "missing.bin"))

@@ -119,6 +119,13 @@ class Context:
                                                 ctypes.byref(h)))
         return DeviceGraph(self, h)
 
+    def graph_from_vite(self, path, layout32=False, validate=False, symmetry=False):
+        """A vite binary file streamed into a resident graph (bounded host memory)."""
+        h = ctypes.c_void_p()
+        self._check(self._L.xt_graph_from_vite(self.h, str(path).encode(), 1 if layout32 else 0,
+                                               _flags(validate=validate, symmetry=symmetry), ctypes.byref(h)))
+        return DeviceGraph(self, h)
+
     def count_edges(self, nv, src, dst, per_vertex=False, orient="degree"):
         src = np.ascontiguousarray(src, np.int64)
         dst = np.ascontiguousarray(dst, np.int64)

@@ -96,6 +96,7 @@ _SIGS = {
     "xt_graph_rmat": (_int, [_p, _int, _int, _dbl, _dbl, _dbl, _u64, _int, _pp]),
     "xt_graph_er": (_int, [_p, _i64, _u64, _u64, _pp]),
     "xt_graph_from_edges": (_int, [_p, _i64, _i64, _p, _p, _pp]),
+    "xt_graph_from_vite": (_int, [_p, ctypes.c_char_p, _int, _u32, _pp]),
     "xt_count_edges": (_int, [_p, _i64, _i64, _p, _p, _u32, _p, _p]),
     "xt_graph_info": (_int, [_p, _p, _p]),
     "xt_graph_download": (_int, [_p, _p, _p, _p]),

@@ -1,6 +1,7 @@
 // api.hip -- extern "C" entry points of libxtric (context, single-graph counts,
 // reference lower-FFI shapes, resident graphs).
 #include <algorithm>
+#include <cstdio>
 #include <cstring>
 #include <vector>
 
@@ -443,6 +444,128 @@ int xt_graph_from_edges(xt_ctx* ctx, int64_t nv, int64_t m, const int64_t* src, 
     return set_err(ctx, XT_ERR_ARG, "edge list: need 1 <= nv < 2^31-1, m >= 0 and both id arrays");
   xt_graph* g = new xt_graph();
   const int rc = graph_from_edges(ctx, nv, m, src, dst, g);
+  if (rc != XT_OK) {
+    if (g->off) (void)hipFree(g->off);
+    if (g->tails) (void)hipFree(g->tails);
+    delete g;
+    return rc;
+  }
+  *out = g;
+  return XT_OK;
+}
+
+}  // extern "C"
+
+// vite file (graph.hpp:294-395 layout: E nv, E ne, (nv+1) E offsets rebased to
+// 0 as graph.hpp:386-388 does, ne records {E tail, W weight}) streamed into a
+// resident graph through two pinned buffers: the next block is read and
+// converted to int32 ids while the previous one uploads.
+template <typename E, typename W>
+static int vite_to_device(xt_ctx* ctx, std::FILE* f, xt_graph* g) {
+  hipStream_t st = ctx->stream;
+  E hdr[2];
+  if (std::fread(hdr, sizeof(E), 2, f) != 2) return set_err(ctx, XT_ERR_INPUT, "vite: short header");
+  const int64_t nv = (int64_t)hdr[0], ne = (int64_t)hdr[1];
+  if (nv < 1 || nv >= (1LL << 31) - 1 || ne < 0)
+    return set_err(ctx, XT_ERR_INPUT, "vite: need 1 <= nv < 2^31-1 and ne >= 0");
+  g->nv = nv;
+  g->ne = ne;
+  XT_HIP(ctx, hipMalloc(&g->off, (size_t)(nv + 1) * sizeof(int64_t)));
+  XT_HIP(ctx, hipMalloc(&g->tails, (size_t)(ne > 0 ? ne : 1) * sizeof(int32_t)));
+  constexpr int64_t kBlock = 1 << 22;  // elements per pinned block
+  struct Rec {
+    E tail;
+    W w;
+  };
+  void* pin[2] = {nullptr, nullptr};
+  hipEvent_t done[2] = {nullptr, nullptr};
+  std::vector<unsigned char> raw((size_t)kBlock * std::max(sizeof(E), sizeof(Rec)));
+  int rc = XT_OK;
+  auto fail = [&](int code, const char* msg) {
+    if (rc == XT_OK) rc = set_err(ctx, code, msg);
+  };
+  hipError_t e = hipSuccess;
+  for (int i = 0; i < 2 && e == hipSuccess; i++) {
+    e = hipHostMalloc(&pin[i], (size_t)kBlock * sizeof(int64_t));
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&done[i], hipEventDisableTiming);
+  }
+  if (e != hipSuccess) rc = hip_err(ctx, e, "vite staging buffers");
+  // Offsets: rebased, checked monotone and ending at ne.
+  int64_t base = 0, prev = 0, k = 0;
+  for (int64_t i = 0; rc == XT_OK && i <= nv; i += kBlock, k++) {
+    const int64_t n = std::min(kBlock, nv + 1 - i);
+    const int b = (int)(k & 1);
+    if (std::fread(raw.data(), sizeof(E), (size_t)n, f) != (size_t)n) {
+      fail(XT_ERR_INPUT, "vite: short offsets");
+      break;
+    }
+    const E* src = reinterpret_cast<const E*>(raw.data());
+    if (i == 0) base = prev = (int64_t)src[0];
+    if (k >= 2) {
+      e = hipEventSynchronize(done[b]);  // this pinned block's previous upload is done
+      if (e != hipSuccess) break;
+    }
+    int64_t* dst = static_cast<int64_t*>(pin[b]);
+    int64_t down = 0;
+    for (int64_t t = 0; t < n; t++) {
+      const int64_t v = (int64_t)src[t];
+      down |= (int64_t)(v < prev);
+      prev = v;
+      dst[t] = v - base;
+    }
+    if (down) fail(XT_ERR_INPUT, "vite: offsets are not monotone");
+    if (i + n == nv + 1 && prev - base != ne) fail(XT_ERR_INPUT, "vite: last offset does not match ne");
+    e = hipMemcpyAsync(g->off + i, dst, (size_t)n * sizeof(int64_t), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipEventRecord(done[b], st);
+    if (e != hipSuccess) break;
+  }
+  // Ids: int32, checked in [0, nv).
+  for (int64_t i = 0; rc == XT_OK && e == hipSuccess && i < ne; i += kBlock, k++) {
+    const int64_t n = std::min(kBlock, ne - i);
+    const int b = (int)(k & 1);
+    if (std::fread(raw.data(), sizeof(Rec), (size_t)n, f) != (size_t)n) {
+      fail(XT_ERR_INPUT, "vite: short edge records");
+      break;
+    }
+    const Rec* src = reinterpret_cast<const Rec*>(raw.data());
+    if (k >= 2) {
+      e = hipEventSynchronize(done[b]);
+      if (e != hipSuccess) break;
+    }
+    int32_t* dst = static_cast<int32_t*>(pin[b]);
+    uint32_t out = 0;
+    for (int64_t t = 0; t < n; t++) {
+      const int64_t v = (int64_t)src[t].tail;
+      out |= (uint32_t)(v < 0 || v >= nv);
+      dst[t] = (int32_t)v;
+    }
+    if (out) fail(XT_ERR_INPUT, "vite: neighbour id out of range [0, nv)");
+    e = hipMemcpyAsync(g->tails + i, dst, (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipEventRecord(done[b], st);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess && rc == XT_OK) rc = hip_err(ctx, e, "vite upload");
+  for (int i = 0; i < 2; i++) {
+    if (done[i]) (void)hipEventDestroy(done[i]);
+    if (pin[i]) (void)hipHostFree(pin[i]);
+  }
+  return rc;
+}
+
+extern "C" {
+
+int xt_graph_from_vite(xt_ctx* ctx, const char* path, int layout32, uint32_t flags, xt_graph** out) {
+  if (!ctx || !path || !out) return XT_ERR_ARG;
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  XT_HIP(ctx, hipSetDevice(ctx->device));
+  *out = nullptr;
+  std::FILE* f = std::fopen(path, "rb");
+  if (!f) return set_err(ctx, XT_ERR_ARG, std::string("vite: cannot open ") + path);
+  xt_graph* g = new xt_graph();
+  int rc = layout32 ? vite_to_device<int32_t, float>(ctx, f, g) : vite_to_device<int64_t, double>(ctx, f, g);
+  std::fclose(f);
+  if (rc == XT_OK && (flags & XT_VALIDATE))
+    rc = validate<int32_t>(ctx, g->nv, g->ne, g->off, g->tails, (flags & XT_CHECK_SYMMETRY) != 0, ctx->stream);
   if (rc != XT_OK) {
     if (g->off) (void)hipFree(g->off);
     if (g->tails) (void)hipFree(g->tails);
