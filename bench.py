@@ -120,12 +120,25 @@ def cpu_baseline(g, seconds):
     # extrapolate by the fraction of the adjacency the completed vertices hold
     frac = edges / max(1, len(tails))
     est_s = el / frac if frac > 0 else float("inf")
+    # the same on one thread, a shorter budget
+    t1 = time.time()
+    _, _, e1 = O.count_reference_budget(off, tails, max(1.0, seconds / 4), 1)
+    f1 = e1 / max(1, len(tails))
+    one = m / ((time.time() - t1) / f1) if f1 > 0 else 0.0
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
     return {"value": round(m / est_s, 1), "unit": "edges/s", "cores": cores, "kind": "port",
+            "single_thread_value": round(one, 1), "nproc": os.cpu_count(), "cpu_model": model,
             "sample": "reference algorithm (xxtric.hpp:60-208, oracle restatement, OpenMP) on "
                       "vertices 0..%d of the same graph (random order: labels are permuted) "
                       "in a %.0f s budget: %d of %d vertices holding %.4f%% of the adjacency; "
                       "time extrapolated by adjacency fraction (hub vertices abandoned at the "
-                      "deadline, so this overstates the CPU rate)"
+                      "deadline, so this overstates the CPU rate); single_thread_value: the same on "
+                      "one thread in a quarter of the budget, over its own vertex prefix"
                       % (done, seconds, done, nv, 100 * frac)}
 
 
