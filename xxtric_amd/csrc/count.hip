@@ -16,9 +16,9 @@
 // (x, w) of row x is an out-edge or an in-edge of x, so every oriented edge is
 // seen from both of its rows and is counted or listed from exactly one of them.
 // The item lists are cut into tiles of up to 256 items with one descriptor each
-// (k_tile_desc).  k_count_groups, a persistent kernel, draws tiles from a
-// device work queue, stages N+(x) once per tile in an LDS hash table of 4-slot
-// buckets, and streams the items' shorter lists N+(y) through it: the tile's
+// (k_tile_desc).  k_count_groups, a persistent kernel, draws runs of 16
+// consecutive tiles from a device work queue, stages N+(x) in an LDS hash
+// table of 4-slot buckets (kept for the next tile of the same owner), and streams the items' shorter lists N+(y) through it: the tile's
 // lists are concatenated into one run of 16-byte chunks that all 512 threads
 // read with coalesced loads, the next round of loads in flight while the
 // current one is probed; compare results are lane masks, counted per wave on
@@ -592,17 +592,29 @@ __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(W
   const int tid = threadIdx.x;
   const int lane = tid & (kWave - 1);
   const int wave = __builtin_amdgcn_readfirstlane(tid / kWave);  // uniform per wave
-  // Tile indices fit 32 bits: there are fewer than nv + ne/256 + 1 < 2^32 tiles.
+  // Tile indices fit 32 bits: there are fewer than nv + ne/256 + 1 tiles, with
+  // nv < 2^31 and ne < 2^33 (checked on the host), so the queue, which overshoots
+  // ntiles by at most 3 runs per workgroup, cannot wrap.
   const uint32_t ntiles = (uint32_t)*ntiles_p;
   unsigned int* queue = reinterpret_cast<unsigned int*>(&counters[kTileQueue]);
 
   // Work queue, fetched ahead by wave 0: while tile T runs, the descriptor of
   // T+1 is loaded and the index of T+2 is drawn.
+  // Tiles are drawn in runs of kTileRun consecutive indices: an owner's tiles
+  // are consecutive, so a workgroup often keeps the owner's table for the
+  // next tile.
+  constexpr uint32_t kTileRun = 16;
+  uint32_t run_next = 0, run_end = 0;  // wave 0: tiles drawn, not yet taken
   auto draw = [&]() -> uint32_t {
-    uint32_t v = 0;
-    if (lane == 0) v = atomicAdd(queue, 1u);
-    return __builtin_amdgcn_readfirstlane(v);
+    if (run_next >= run_end) {
+      uint32_t v = 0;
+      if (lane == 0) v = atomicAdd(queue, kTileRun);
+      run_next = __builtin_amdgcn_readfirstlane(v);
+      run_end = run_next + kTileRun;
+    }
+    return run_next++;
   };
+  int64_t table_a0 = -1;  // owner list staged in s_tab (its oriented offset), -1 none
   // Descriptors are copied global -> LDS by wave 0's lanes 0-1 with one 16-byte
   // LDS-DMA load each (no registers held across the tile).
   const int4* __restrict__ tdesc4 = reinterpret_cast<const int4*>(tdesc);
@@ -642,6 +654,7 @@ __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(W
     const int nitems = d.nitems;
     const int64_t i0 = d.i0, a0 = d.a0, dx = d.dx;
     const bool hashed = dx <= 2 * kTabBuckets;  // at most two ids per bucket on average
+    const bool reuse = hashed && a0 == table_a0;  // the previous tile's owner: table in place
 
     // Items k = tid*kJ + j: each list is streamed as ceil(len/4) 16-byte chunks
     // from its first id (dword-aligned loads; the last chunk is masked).
@@ -658,9 +671,10 @@ __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(W
         nchunks += nc[j];
       }
     }
-    if (hashed)
+    if (hashed && !reuse) {
       for (int b = tid; b < kTabBuckets; b += kT) s_tab[b] = make_int4(kEmpty, kEmpty, kEmpty, kEmpty);
-    if (tid == 0) s_spilled = 0;
+      if (tid == 0) s_spilled = 0;
+    }
     int64_t nch_all = 0;
     int64_t pre = block_excl_scan_i64<kT>(nchunks, s_scan, &nch_all);
 #pragma unroll
@@ -737,9 +751,10 @@ __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(W
     int kA[RC], rA[RC], kB[RC], rB[RC];
     int4 vA[RC], vB[RC];
     if (nchunk > 0) locate(0, kA, vA, rA);  // first loads fly while the table is filled
-    if (hashed)
+    if (hashed && !reuse)
       for (int64_t k = tid; k < dx; k += kT) tab_insert(s_tab, ot[a0 + k], &s_spilled);
     __syncthreads();  // table complete
+    table_a0 = hashed ? a0 : -1;
 
     const bool spl = s_spilled != 0;  // uniform: read after the table barrier
     uint32_t hits = 0, at_x = 0;  // below 2^32: a tile streams fewer than 2^33 ids over 512 threads
