@@ -202,7 +202,8 @@ int xt_rgg_generate(int64_t nv, uint32_t seed, double radius, int64_t* offsets,
                     int64_t* tails, int64_t cap, int64_t* ne);
 double xt_rgg_radius(int64_t nv);
 /* vite binary file (graph.hpp:294-395); layout32 selects the USE_32_BIT_GRAPH
- * layout.  Header only when offsets == NULL. */
+ * layout.  Header only when offsets == NULL; header and offsets (rebased to
+ * start at 0, graph.hpp:386-388) when tails == NULL. */
 int xt_vite_read(const char* path, int layout32, int64_t* nv, int64_t* ne,
                  int64_t* offsets, int64_t* tails);
 int xt_vite_write(const char* path, int layout32, int64_t nv, const int64_t* offsets,
@@ -211,6 +212,26 @@ int xt_vite_write(const char* path, int layout32, int64_t nv, const int64_t* off
  * (cugraph/example-inputs format).  Two calls: tails == NULL sizes it. */
 int xt_edgelist_to_csr(const char* path, int64_t* nv, int64_t* ne, int64_t* offsets,
                        int64_t* tails);
+
+/* ---- vertex partitions for a split over ranks (host code; no device work) ---
+ * Replaces BinaryEdgeList::find_balanced_num_edges (graph.hpp:556-602, flag -b)
+ * and linear_partitioning_num_edges (graph.hpp:397-419 with linear_partition,
+ * utils.hpp:46-135, flag -d), which read_balanced (graph.hpp:608-640) calls
+ * on rank 0 with the file's offsets.  Writes bins[0..nparts]: rank r owns
+ * vertices [bins[r], bins[r+1]); the same bins as the reference's.
+ *   XT_PARTITION_GREEDY: a vertex joins the current rank while the rank holds
+ *     at most (offsets[nv] - offsets[0]) / nparts entries; the last rank takes
+ *     the rest.
+ *   XT_PARTITION_LINEAR: the smallest possible largest part over the sequence
+ *     offsets[0..nv) (the running counts, as the reference reads them), first
+ *     split point on ties; O(nv * nparts) time, 4 * nv * nparts bytes.
+ *     XT_ERR_ARG when nparts > nv - 1 or where the reference's table walk
+ *     would leave its table (its result is undefined there).
+ * XT_ERR_INPUT for decreasing or negative offsets; nv < 2^31. */
+#define XT_PARTITION_GREEDY 0
+#define XT_PARTITION_LINEAR 1
+int xt_partition_vertices(int64_t nv, const int64_t* offsets, int64_t nparts, int method,
+                          int64_t* bins);
 
 /* ---- batching: many small graphs counted by one persistent launch ------- */
 typedef struct xt_batch xt_batch;

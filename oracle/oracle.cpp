@@ -553,3 +553,75 @@ extern "C" void or_batch_params(uint64_t base_seed, int64_t g, int64_t nmin, int
   const double p = nn > 1 ? deg / (double)(nn - 1) : 0.0;
   *er_thr64 = p >= 1.0 ? (1ULL << 32) : (uint64_t)std::floor(p * 4294967296.0);
 }
+
+// ---- vertex partitions (BinaryEdgeList::read_balanced, graph.hpp:608-640) ----
+
+// graph.hpp:556-602 (find_balanced_num_edges, the default build's -b): walk the
+// vertices in order; a vertex joins the current rank while the rank stays at
+// or under ne / nparts entries (the last rank takes everything left), else it
+// opens the next rank.  bins are counts per rank, then a prefix sum.
+extern "C" int or_partition_greedy(int64_t nv, const int64_t* off, int64_t nparts,
+                                   int64_t* bins) {
+  if (nv < 0 || nparts < 1) return -1;
+  for (int64_t r = 0; r <= nparts; r++) bins[r] = 0;
+  std::vector<int64_t> held(nparts, 0);
+  const int64_t cap = (off[nv] - off[0]) / nparts;
+  int64_t r = 0;
+  for (int64_t v = 0; v < nv; v++) {
+    const int64_t d = off[v + 1] - off[v];
+    if (held[r] + d > cap && r != nparts - 1) r++;
+    held[r] += d;
+    bins[r + 1]++;
+  }
+  for (int64_t q = 1; q <= nparts; q++) bins[q] += bins[q - 1];
+  return 0;
+}
+
+// graph.hpp:397-419 + utils.hpp:46-135 (-d).  The sequence is the first nv
+// offsets.  best[i][j]: smallest possible largest part when seq[0..i] is cut
+// into j+1 consecutive parts, with best[i][0] the prefix sum and best[0][j] =
+// seq[0]; cut[i][j] is the first split x (the last part is seq[x+1..i]) that
+// attains best[i][j], scanning x = 0..i-1 and keeping only strict improvements.
+// The split points are then walked back from the last part.
+extern "C" int or_partition_linear(int64_t nv, const int64_t* off, int64_t nparts,
+                                   int64_t* bins) {
+  const int64_t n = nv, k = nparts;
+  if (n < 1 || k < 1) return -1;
+  if (k > n - 1) return -1;  // the reference fills bins with 1s and goes on: not a partition
+  for (int64_t r = 0; r <= k; r++) bins[r] = 0;
+  std::vector<int64_t> best((size_t)(n * k)), cut((size_t)((n - 1) * (k - 1)));
+  auto B = [&](int64_t i, int64_t j) -> int64_t& { return best[(size_t)(i * k + j)]; };
+  int64_t run = 0;
+  for (int64_t i = 0; i < n; i++) {
+    run += off[i];
+    B(i, 0) = run;
+  }
+  for (int64_t j = 0; j < k; j++) B(0, j) = off[0];
+  for (int64_t i = 1; i < n; i++) {
+    for (int64_t j = 1; j < k; j++) {
+      int64_t lo = 0, at = 0;
+      for (int64_t x = 0; x < i; x++) {
+        const int64_t last = B(i, 0) - B(x, 0);
+        const int64_t worst = B(x, j - 1) > last ? B(x, j - 1) : last;
+        if (x == 0 || worst < lo) {
+          lo = worst;
+          at = x;
+        }
+      }
+      B(i, j) = lo;
+      cut[(size_t)((i - 1) * (k - 1) + (j - 1))] = at;
+    }
+  }
+  // Walk back: rank p takes seq[cut+1 .. i]; the first rank takes seq[0 .. i].
+  int64_t i = n - 1, p = k;
+  for (int64_t j = k - 2; j >= 0; j--, p--) {
+    const int64_t idx = (i - 1) * (k - 1) + j;
+    if (i < 1 || idx < 0 || idx >= (n - 1) * (k - 1)) return -1;  // outside the reference's table
+    const int64_t x = cut[(size_t)idx];
+    bins[p] += i - x;
+    i = x;
+  }
+  bins[p] += i + 1;
+  for (int64_t q = 1; q <= k; q++) bins[q] += bins[q - 1];
+  return 0;
+}

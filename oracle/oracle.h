@@ -17,6 +17,10 @@
  *                           and with the closed forms above.
  *   or_rgg_lcg              golden edge and triangle counts from the reference binary.
  *   or_vite_read/write      byte-layout round trip and reading by the reference binary.
+ *   or_partition_greedy /   bins of the reference's own routines (graph.hpp:397-602,
+ *   or_partition_linear     utils.hpp:46-135) compiled from its headers (oracle/_ref/
+ *                           ref_partition) on fixture offsets; the dynamic program's
+ *                           largest part equals a brute-force minimum over all splits.
  *   or_rmat_csr / or_er_csr the build's own generators: pinned by invariants
  *                           (symmetry, no self-loops, sorted unique rows, edge-count
  *                           bounds) and by golden counts of the reference binary
@@ -98,6 +102,19 @@ int64_t or_er_csr(int64_t n, uint64_t seed, uint64_t thr64, int64_t* off,
 /* Batch graph parameters of graph g (log-uniform n, RGG even / ER odd). */
 void or_batch_params(uint64_t base_seed, int64_t g, int64_t nmin, int64_t nmax,
                      int64_t* n, int* kind, uint64_t* gseed, uint64_t* er_thr64);
+
+/* ---- vertex partitions for a split over ranks (BinaryEdgeList::read_balanced) ----
+ * bins[0..nparts] (zeroed here): rank r owns vertices [bins[r], bins[r+1]).
+ * Return 0, or -1 where the reference's table walk would read outside its
+ * table (it is undefined there) or nparts > nv - 1 for the dynamic program
+ * (the reference's bins are not a partition there). */
+/* -b: greedy fill up to ne / nparts entries per rank, graph.hpp:556-602
+ * (default build; ne = off[nv] - off[0], the vite header's ne). */
+int or_partition_greedy(int64_t nv, const int64_t* off, int64_t nparts, int64_t* bins);
+/* -d: linear_partitioning_num_edges (graph.hpp:397-419): Skiena's dynamic
+ * program of utils.hpp:46-135 on seq[i] = off[i], i < nv (the file's first nv
+ * offsets, as the reference reads them), literally O(nv^2 * nparts). */
+int or_partition_linear(int64_t nv, const int64_t* off, int64_t nparts, int64_t* bins);
 
 #ifdef __cplusplus
 }

@@ -48,6 +48,8 @@ def lib():
             "or_rmat_csr": (_i64, [_int, _int, _u64, _p, _int, _p, _p, _i64]),
             "or_er_csr": (_i64, [_i64, _u64, _u64, _p, _p, _i64]),
             "or_batch_params": (None, [_u64, _i64, _i64, _i64, _p, _p, _p, _p]),
+            "or_partition_greedy": (_int, [_i64, _p, _i64, _p]),
+            "or_partition_linear": (_int, [_i64, _p, _i64, _p]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -211,3 +213,16 @@ def batch_params(base_seed, g, nmin=200, nmax=20000):
     lib().or_batch_params(base_seed, g, nmin, nmax, ctypes.addressof(n), ctypes.addressof(kind),
                           ctypes.addressof(gseed), ctypes.addressof(thr))
     return n.value, kind.value, gseed.value, thr.value
+
+
+REF_PARTITION_BIN = os.path.join(_HERE, "_ref", "ref_partition")
+
+
+def partition(off, nparts, method="greedy"):
+    """bins int64[nparts+1] of or_partition_greedy / or_partition_linear, or None
+    where the restatement reports the reference's undefined cases."""
+    off = np.ascontiguousarray(off, np.int64)
+    bins = np.zeros(nparts + 1, np.int64)
+    f = lib().or_partition_greedy if method == "greedy" else lib().or_partition_linear
+    rc = f(len(off) - 1, _ptr(off), nparts, _ptr(bins))
+    return bins if rc == 0 else None

@@ -9,6 +9,10 @@ What it records (every count comes from the reference binary oracle/_ref/tric_re
 built from main.cpp/xxtric.hpp/graph.hpp where they lie):
   * rgg_lcg: `tric_ref -n N -l` -> "Number of edges" and "Number of triangles"
     (the reference's seeded LCG random geometric graph, graph.hpp:745-1374).
+  * partitions: the vertex bins of the reference's balanced reads (-b, -d),
+    from oracle/_ref/ref_partition (our driver over the reference's
+    BinaryEdgeList routines, graph.hpp:397-602) on fixture offsets.
+    `python tests/golden/make_golden.py --partitions` refreshes only these.
   * vite inputs read with `tric_ref -f FILE`: R-MAT and Erdos-Renyi graphs of
     this build's own generators and amazon0302 (cugraph/example-inputs), each
     with a sha256 of its CSR so a test can tell that the same graph is rebuilt.
@@ -72,9 +76,66 @@ def amazon_csr():
     return off, dst.astype(np.int64)
 
 
+def ref_partition(off, nparts, method, tmp):
+    """Bins of the reference's own routines (oracle/_ref/ref_partition) for
+    the offsets `off` written as a vite file (the tails do not matter)."""
+    path = os.path.join(tmp, "part.bin")
+    O.vite_write(path, off, np.zeros(int(off[-1]), np.int64))
+    out = subprocess.run([O.REF_PARTITION_BIN, path, str(nparts), str(method)], capture_output=True,
+                         text=True, check=True, timeout=3600).stdout
+    os.unlink(path)
+    return [int(t) for t in out.split()]
+
+
+def off_sha256(off):
+    return hashlib.sha256(np.ascontiguousarray(off, np.int64).tobytes()).hexdigest()
+
+
+def partitions():
+    """Vertex bins of the reference's balanced reads (-b greedy, -d linear
+    partitioning) on fixture offsets; the dynamic program is O(nv^2 nparts) in
+    the reference, so it runs on the smaller fixtures only."""
+    cases = [("rmat", dict(scale=10, seed=1, permute=1), True),
+             ("rmat", dict(scale=12, seed=3, permute=0), True),
+             ("rmat", dict(scale=16, seed=1, permute=0), False),
+             ("er", dict(n=2000, seed=12, p=0.01), True),
+             ("rgg_lcg", dict(n=4096), True)]
+    if os.path.exists(AMAZON_TSV):
+        cases.append(("amazon0302", {}, False))
+    out = []
+    with tempfile.TemporaryDirectory() as tmp:
+        for kind, prm, linear in cases:
+            if kind == "rmat":
+                off, _ = O.rmat_csr(prm["scale"], 16, prm["seed"], bool(prm["permute"]))
+            elif kind == "er":
+                off, _ = O.er_csr(prm["n"], prm["seed"], O.er_threshold(prm["p"]))
+            elif kind == "rgg_lcg":
+                off, _ = O.rgg_lcg(prm["n"])
+            else:
+                off, _ = amazon_csr()
+            for nparts in (2, 3, 4, 8, 64):
+                for method, name in ((0, "greedy"), (1, "linear")):
+                    if method == 1 and (not linear or nparts > 8):
+                        continue
+                    bins = ref_partition(off, nparts, method, tmp)
+                    out.append({"graph": kind, "params": prm, "nv": len(off) - 1,
+                                "off_sha256": off_sha256(off), "nparts": nparts,
+                                "method": name, "bins": bins})
+                    print("partition", kind, prm, nparts, name, bins[:4], "...", flush=True)
+    return out
+
+
 def main():
     if not os.path.exists(REF):
         sys.exit("build the reference first: make -C oracle ref")
+    if sys.argv[1:] == ["--partitions"]:  # refresh only the partition bins
+        with open(os.path.join(HERE, "golden.json")) as f:
+            g = json.load(f)
+        g["partitions"] = partitions()
+        g["reference_partition_binary"] = "oracle/_ref/ref_partition"
+        with open(os.path.join(HERE, "golden.json"), "w") as f:
+            json.dump(g, f, indent=1)
+        return
     g = {"generator": "tests/golden/make_golden.py", "reference_binary": "oracle/_ref/tric_ref",
          "rgg_lcg": [], "rmat": [], "er": [], "batch": [], "amazon0302": None}
     for n in (256, 1024, 4096, 65536, 100000):
@@ -116,6 +177,8 @@ def main():
             g["amazon0302"] = {"nv": len(off) - 1, "ne": int(off[-1]), "triangles": tri,
                                "csr_sha256": csr_sha256(off, tails)}
             print("amazon", len(off) - 1, off[-1], tri, flush=True)
+    g["partitions"] = partitions()
+    g["reference_partition_binary"] = "oracle/_ref/ref_partition"
     with open(os.path.join(HERE, "golden.json"), "w") as f:
         json.dump(g, f, indent=1)
 

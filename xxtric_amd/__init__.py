@@ -362,6 +362,23 @@ def edgelist_to_csr(path):
     return off, tails
 
 
+PARTITION_METHODS = {"greedy": 0, "linear": 1}
+
+
+def partition_vertices(off, nparts, method="greedy"):
+    """Owned vertex ranges of a split over nparts ranks, as the reference's
+    balanced read computes them (graph.hpp:397-640): "greedy" is flag -b,
+    "linear" flag -d.  Returns bins int64[nparts + 1]; rank r owns
+    [bins[r], bins[r + 1])."""
+    off = np.ascontiguousarray(off, np.int64)
+    bins = np.zeros(nparts + 1, np.int64)
+    rc = capi.lib().xt_partition_vertices(len(off) - 1, capi.ptr(off), nparts,
+                                          PARTITION_METHODS[method], capi.ptr(bins))
+    if rc != capi.XT_OK:
+        raise XtError(rc, "xt_partition_vertices(%s, %d parts)" % (method, nparts))
+    return bins
+
+
 def batch_params(base_seed, g, nmin=200, nmax=20000):
     n = ctypes.c_int64()
     kind = ctypes.c_int()

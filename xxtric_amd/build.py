@@ -27,7 +27,7 @@ CXX = os.environ.get("CXX", "g++")
 ARCH = "gfx950"
 
 HIP_SOURCES = ["api.hip", "orient.hip", "count.hip", "gen.hip", "batch.hip", "serve.hip"]
-HOST_SOURCES = ["ingest.cpp"]
+HOST_SOURCES = ["ingest.cpp", "partition.cpp"]
 
 HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-Wall",
              "-Wno-unused-result", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -111,6 +111,7 @@ _SIGS = {
     "xt_vite_read": (_int, [ctypes.c_char_p, _int, _p, _p, _p, _p]),
     "xt_vite_write": (_int, [ctypes.c_char_p, _int, _i64, _p, _p]),
     "xt_edgelist_to_csr": (_int, [ctypes.c_char_p, _p, _p, _p, _p]),
+    "xt_partition_vertices": (_int, [_i64, _p, _i64, _int, _p]),
     "xt_batch_create": (_int, [_p, _i64, _i64, _i64, _pp]),
     "xt_batch_add": (_int, [_p, _i64, _i64, _p, _p, _p]),
     "xt_batch_add_generated": (_int, [_p, _u64, _i64, _i64, _i64, _i64]),

@@ -182,6 +182,10 @@ static int vite_read_t(const char* path, int64_t* nv, int64_t* ne, int64_t* off,
   }
   // graph.hpp:386-388 rebases the offsets to 0.
   for (int64_t i = 0; i <= *nv; i++) off[i] = (int64_t)o[i] - (int64_t)o[0];
+  if (!tails) {  // offsets only
+    std::fclose(f);
+    return XT_OK;
+  }
   struct Rec {
     E tail;
     W w;

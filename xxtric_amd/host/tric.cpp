@@ -4,7 +4,8 @@
 // Reference flags (main.cpp:204-236):
 //   -f FILE   read a vite binary edge list          -n N   generate an RGG of N vertices
 //   -l        use the seeded LCG stream (reference)  -p P   add P% random edges (seeded here)
-//   -b / -d   balanced reads (one process: same as a plain read)
+//   -b / -d   balanced reads: greedy / linear-partitioning vertex bins (one process:
+//             the read itself is a plain read)
 //   -r R      ranks per node (ignored: one process)  -s B   buffer size (ignored)
 // Additional flags of this build:
 //   -R S      R-MAT scale S generated on the device (with -e edge factor, -S seed, -P no relabel)
@@ -13,6 +14,8 @@
 //   -w FILE   write the input graph as a vite file  -x     generate/read only, no count
 //   -L        read/write the 32-bit vite layout      -j     also print a JSON summary line
 //   -t K      time K counts (the reference times one)
+//   -B K      with -f: print the vertex bins of a balanced read over K ranks
+//             (greedy, or linear partitioning with -d)
 #include <unistd.h>
 
 #include <chrono>
@@ -33,6 +36,8 @@ struct Opts {
   bool lcg = false, layout32 = false, json = false, no_count = false, permute = true;
   double pct = 0.0;
   int rmat = 0, ef = 16, device = 0, reps = 1;
+  int balanced = 0;   // 1: -b (greedy bins), 2: -d (linear partitioning)
+  int bin_ranks = 0;  // -B K
   uint64_t seed = 1;
   bool seed_set = false;
   uint32_t flags = XT_ORIENT_DEGREE;
@@ -46,10 +51,13 @@ struct Opts {
 Opts parse(int argc, char* argv[]) {
   Opts o;
   int c;
-  while ((c = getopt(argc, argv, "f:r:n:p:olbds:R:e:S:Pm:g:v:w:xLjt:")) != -1) {
+  while ((c = getopt(argc, argv, "f:r:n:p:olbds:R:e:S:Pm:g:v:w:xLjt:B:")) != -1) {
     switch (c) {
       case 'f': o.file = optarg; break;
-      case 'b': case 'd': case 'o': break;   // balanced reads: one process reads everything
+      case 'b': o.balanced = 1; break;
+      case 'd': o.balanced = 2; break;
+      case 'o': break;
+      case 'B': o.bin_ranks = atoi(optarg); break;
       case 'r': case 's': break;             // ranks per node / buffer size: no meaning at one process
       case 'n': o.nv = atol(optarg); break;
       case 'l': o.lcg = true; break;
@@ -100,7 +108,29 @@ int main(int argc, char* argv[]) {
       g = gr.generate(o.lcg, true, o.pct, o.seed_set ? (uint32_t)o.seed : 0);
     } else {
       BinaryEdgeList rm;
-      g = rm.read(0, 1, 1, o.file, o.layout32);
+      if (o.balanced) {  // main.cpp:95-103 and graph.hpp:620-636, rank 0 of one process
+        std::cout << std::endl;
+        std::cout << "Trying to balance the edge distribution while reading: " << std::endl;
+        std::cout << o.file << std::endl;
+        if (o.balanced == 1)
+          std::cout << "Trying to achieve equal edge distribution across processes." << std::endl;
+        else
+          std::cout << "Trying to achieve equal edge distribution across processes via dynamic "
+                       "programming (linear partitioning)." << std::endl;
+        g = rm.read_balanced(0, 1, 1, o.file, (short)(o.balanced - 1), o.layout32);
+      } else {
+        g = rm.read(0, 1, 1, o.file, o.layout32);
+      }
+      if (o.bin_ranks > 0) {
+        std::vector<GraphElem> bins;
+        if (o.balanced == 2)
+          rm.linear_partitioning_num_edges(o.bin_ranks, o.file, bins, o.layout32);
+        else
+          rm.find_balanced_num_edges(o.bin_ranks, o.file, bins, o.layout32);
+        std::cout << "Vertex bins for " << o.bin_ranks << " ranks:";
+        for (GraphElem b : bins) std::cout << " " << b;
+        std::cout << std::endl;
+      }
     }
   } catch (const std::exception& e) {
     die(e.what());

@@ -104,6 +104,42 @@ class BinaryEdgeList {         // graph.hpp:279-395 (read), one process
       throw std::runtime_error("Error reading file! " + file);
     return graph_from_csr(off, tails);
   }
+
+  // Owned vertex bins of a balanced read over nprocs ranks (mbins: nprocs + 1
+  // entries; rank r owns [mbins[r], mbins[r+1])): the greedy fill of flag -b
+  // (graph.hpp:556-602) and the linear partitioning of flag -d
+  // (graph.hpp:397-419).  Offsets are read rebased to 0, as every file the
+  // reference writes has them.
+  void find_balanced_num_edges(int nprocs, std::string file, std::vector<GraphElem>& mbins,
+                               bool layout32 = false) {
+    bins_of(nprocs, file, mbins, XT_PARTITION_GREEDY, layout32);
+  }
+  void linear_partitioning_num_edges(int nprocs, std::string file, std::vector<GraphElem>& mbins,
+                                     bool layout32 = false) {
+    bins_of(nprocs, file, mbins, XT_PARTITION_LINEAR, layout32);
+  }
+
+  // graph.hpp:608-733 at one process (nprocs = 1): rank 0 owns every vertex,
+  // so the balanced read is the plain read.
+  Graph* read_balanced(int me, int nprocs, int ranks_per_node, std::string file,
+                       short /*static_or_dyn*/ = 0, bool layout32 = false) {
+    if (nprocs != 1) throw std::runtime_error("read_balanced: one process (nprocs = 1)");
+    return read(me, nprocs, ranks_per_node, file, layout32);
+  }
+
+ private:
+  void bins_of(int nprocs, const std::string& file, std::vector<GraphElem>& mbins, int method,
+               bool layout32) {
+    int64_t nv = 0, ne = 0;
+    if (xt_vite_read(file.c_str(), layout32, &nv, &ne, nullptr, nullptr) != XT_OK)
+      throw std::runtime_error("Error opening file! " + file);
+    std::vector<int64_t> off(nv + 1);
+    if (xt_vite_read(file.c_str(), layout32, &nv, &ne, off.data(), nullptr) != XT_OK)
+      throw std::runtime_error("Error reading file! " + file);
+    mbins.assign(nprocs + 1, 0);
+    const int rc = xt_partition_vertices(nv, off.data(), nprocs, method, mbins.data());
+    if (rc != XT_OK) throw std::runtime_error("vertex partition failed (" + std::to_string(rc) + ")");
+  }
 };
 
 class GenerateRGG {            // graph.hpp:745-1374, one process
