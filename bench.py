@@ -88,6 +88,22 @@ def rank0_slice(ctx, g, triangles, parts=8, steps=3):
     return out
 
 
+def per_vertex_count(ctx, g, triangles, steps=3):
+    """The same count with per-vertex output (each triangle credited to its
+    smallest vertex, uint64 per vertex in HBM, then copied to the host)."""
+    tot, pv = g.count(per_vertex=True)  # warm-up
+    ev = []
+    for _ in range(steps):
+        t, pv = g.count(per_vertex=True)
+        assert t == tot
+        ev.append(ctx.timing())
+    ms = float(np.median([e["total_ms"] for e in ev]))
+    return {"ms": round(ms, 3), "edges_per_s": round((g.ne // 2) / (ms * 1e-3), 1),
+            "phases_ms": {k: round(float(np.median([e[k + "_ms"] for e in ev])), 3)
+                          for k in ("orient", "light", "group")},
+            "sum_equals_total": int(pv.sum(dtype=np.uint64)) == tot == triangles}
+
+
 def roofline(r):
     # Dominant kernel: k_count_groups (heavy edges).  Its algorithmic bytes per
     # launch: 4 B per id of each heavy edge's shorter out-list plus 4 B per item
@@ -283,6 +299,10 @@ def main():
     }
     line["cpu_baseline"] = cpu_baseline(g, args.cpu_seconds)
     if not args.no_extras:
+        try:
+            line["per_vertex"] = per_vertex_count(ctx, g, r["triangles"])
+        except Exception as e:
+            line["per_vertex"] = {"error": str(e)}
         try:
             line["rank0_of_8"] = rank0_slice(ctx, g, r["triangles"])
         except Exception as e:

@@ -566,6 +566,23 @@ __device__ __forceinline__ bool tab_continue(const int4* tab, uint32_t b, int32_
   }
 }
 
+// Slot index 4*bucket + s of w in bucket b (the four slots t), or -1.
+__device__ __forceinline__ int bucket_slot(const int4 t, uint32_t b, int32_t w) {
+  const int s = t.x == w ? 0 : t.y == w ? 1 : t.z == w ? 2 : (t.w & 0x7fffffff) == w ? 3 : -1;
+  return s < 0 ? -1 : (int)(4 * b) + s;
+}
+
+// tab_continue returning the slot index of w, or -1.
+__device__ __forceinline__ int tab_continue_slot(const int4* tab, uint32_t b, int32_t w) {
+  for (;;) {
+    b = (b + 1) & (kTabBuckets - 1);
+    const int4 t = tab[b];
+    const int s = bucket_slot(t, b, w);
+    if (s >= 0) return s;
+    if (!bucket_spilled(t)) return -1;
+  }
+}
+
 template <int CM, bool BATCH, int RC, int WPE>
 __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(WPE, 8))) void k_count_groups(
     const int32_t* __restrict__ ot, const Item* __restrict__ items, const TileDesc* __restrict__ tdesc,
@@ -573,7 +590,6 @@ __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(W
     unsigned long long* __restrict__ pv, const int32_t* __restrict__ gid,
     unsigned long long* __restrict__ gtotal, int64_t rlo, int64_t rhi) {
   constexpr bool PV = CM == kCreditVertex;
-  constexpr bool MASKS = PV;  // hits are credited one by one (per vertex)
   __shared__ int4 s_tab[kTabBuckets];
   // Per item k: {adj (64-bit), end, y}.  Chunk c of item k (16 bytes) starts at
   // byte address adj + 16*c and holds end - 4*c ids of the list (mod 2^32;
@@ -587,6 +603,15 @@ __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(W
   __shared__ int4 s_desc[2][2];  // TileDesc of the current and of the next tile (ping-pong)
   __shared__ uint32_t s_tile;
   __shared__ int s_spilled;  // some bucket of the current table spilled
+  // Per-vertex credit, gathered in LDS before it goes to HBM: hits whose
+  // smallest vertex is the table id in slot i add to the 16-bit half i & 1 of
+  // s_wcnt[i >> 1] (kept for up to kFlushTiles tiles of one table, so below
+  // kFlushTiles * kItemsPerTile, as an item hits each id at most once); hits
+  // whose smallest vertex is item k's partner add to s_ycnt[k] (one tile).
+  constexpr int kFlushTiles = 16;
+  static_assert(kFlushTiles * kItemsPerTile < 65536, "16-bit slot counters");
+  __shared__ uint32_t s_wcnt[PV ? 2 * kTabBuckets : 1];
+  __shared__ uint32_t s_ycnt[PV ? kItemsPerTile : 1];
   constexpr int kT = kGroupThreads;
   constexpr int kJ = (kItemsPerTile + kT - 1) / kT;  // items per thread in the tile set-up
   const int tid = threadIdx.x;
@@ -623,6 +648,28 @@ __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(W
   };
   uint32_t t_nx = 0, t_nx2 = 0;
   int par = 0;
+  int table_tiles = 0;  // tiles counted into s_wcnt since its last flush
+  if constexpr (PV) {
+    for (int i = tid; i < 2 * kTabBuckets; i += kT) s_wcnt[i] = 0;
+    for (int i = tid; i < kItemsPerTile; i += kT) s_ycnt[i] = 0;
+  }
+  // Adds the slot counters of the table in s_tab to pv and clears them.  Each
+  // thread takes whole buckets, the same ones it clears and fills in the set-up.
+  auto flush_slots = [&]() {
+    if constexpr (PV) {
+      for (int b = tid; b < kTabBuckets; b += kT) {
+        const uint32_t c01 = s_wcnt[2 * b], c23 = s_wcnt[2 * b + 1];
+        if ((c01 | c23) == 0) continue;
+        const int4 t = s_tab[b];
+        if (c01 & 0xffffu) atomicAdd(&pv[t.x], (unsigned long long)(c01 & 0xffffu));
+        if (c01 >> 16) atomicAdd(&pv[t.y], (unsigned long long)(c01 >> 16));
+        if (c23 & 0xffffu) atomicAdd(&pv[t.z], (unsigned long long)(c23 & 0xffffu));
+        if (c23 >> 16) atomicAdd(&pv[t.w & 0x7fffffff], (unsigned long long)(c23 >> 16));
+        s_wcnt[2 * b] = 0;
+        s_wcnt[2 * b + 1] = 0;
+      }
+    }
+  };
   if (wave == 0) {
     const uint32_t t = draw();
     if (lane == 0) s_tile = t;
@@ -670,6 +717,15 @@ __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(W
         nc[j] = (it[j].len + 3) >> 2;
         nchunks += nc[j];
       }
+    }
+    if constexpr (PV) {
+      // The previous table's counters go out before it is replaced, and at
+      // least every kFlushTiles tiles (16-bit counters).
+      if (hashed && (!reuse || table_tiles == kFlushTiles)) {
+        flush_slots();
+        table_tiles = 0;
+      }
+      if (hashed) table_tiles++;
     }
     if (hashed && !reuse) {
       for (int b = tid; b < kTabBuckets; b += kT) s_tab[b] = make_int4(kEmpty, kEmpty, kEmpty, kEmpty);
@@ -763,59 +819,68 @@ __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(W
     // waves take the tile's chunks in turns and it streams fewer than 2^33 ids.
     uint32_t hits_w = 0;
     uint32_t r_owned_w = 0, r_local_w = 0;  // owned-range credit counted per wave
-    auto credit = [&](int32_t w, int k) {  // a hit, with its per-vertex or owned-range credit
+    // A hit, with its per-vertex or owned-range credit.  slot: w's slot in the
+    // LDS table, -1 when the owner list is searched in HBM instead.
+    auto credit = [&](int32_t w, int k, int slot) {
       hits++;
       if constexpr (PV) {
-        const int64_t yy = s_rec[k].w;
-        int64_t m = x < yy ? x : yy;
-        if ((int64_t)w < m) m = w;
-        if (m == x)
+        const int64_t yy = s_rec[k].w;  // x, y and w are three distinct vertices
+        if (x < yy && x < (int64_t)w)
           at_x++;
+        else if (yy < (int64_t)w)
+          atomicAdd(&s_ycnt[k], 1u);
+        else if (slot >= 0)
+          atomicAdd(&s_wcnt[slot >> 1], 1u << (16 * (slot & 1)));
         else
-          atomicAdd(&pv[m], 1ull);
+          atomicAdd(&pv[w], 1ull);
       } else if constexpr (CM == kCreditRange) {
         range_credit(x, (int64_t)s_rec[k].w, (int64_t)w, rlo, rhi, r_owned, r_local);
       }
     };
-    // Membership in the LDS hash of the ids of one chunk of item k (r of its
-    // four ids lie inside the list): four bucket reads, an xor/min test and a
-    // carry-add of the hits, branch-free.  Only tables with a spilled bucket
-    // take the (uniform) branch to the rare continuations.
-    auto probe_hash = [&](const int4 v, int k, int r) {
+    // Per-vertex probe: membership of the ids of one chunk of item k (r of its
+    // four ids lie inside the list) in the LDS hash, four bucket reads and the
+    // slot of each hit.  Only tables with a spilled bucket take the (uniform)
+    // branch to the rare continuations.  Each triangle {x, y, w} is credited to
+    // its smallest vertex without branches: x (a register count), y (the
+    // chunk's partner count, one LDS add per chunk) or w (its table slot's
+    // counter, one LDS add per such hit).
+    auto probe_pv = [&](const int4 v, int k, int r) {
       const int32_t w[4] = {r > 0 ? v.x : -1, r > 1 ? v.y : -1, r > 2 ? v.z : -1, r > 3 ? v.w : -1};
       int4 t[4];
 #pragma unroll
       for (int j = 0; j < 4; j++) t[j] = bucket_at(s_tab, bucket_addr(w[j]));
-      uint32_t hitm = 0;
+      int slot[4];
       bool need = false;
 #pragma unroll
       for (int j = 0; j < 4; j++) {
-        const bool h = bucket_miss(t[j], w[j]) == 0;
-        if constexpr (MASKS)
-          hitm |= (uint32_t)h << j;
-        else
-          hits += h;
-        need |= !h && bucket_spilled(t[j]) && w[j] >= 0;
+        slot[j] = bucket_slot(t[j], bucket_addr(w[j]) >> 4, w[j]);
+        need |= slot[j] < 0 && bucket_spilled(t[j]) && w[j] >= 0;
       }
       if (spl) {  // uniform: only tables with a spilled bucket can need this
         if (need) {
 #pragma unroll
           for (int j = 0; j < 4; j++) {
-            if (w[j] < 0 || bucket_miss(t[j], w[j]) == 0 || !bucket_spilled(t[j])) continue;
-            if (tab_continue(s_tab, bucket_addr(w[j]) >> 4, w[j])) {
-              if constexpr (MASKS)
-                hitm |= 1u << j;
-              else
-                hits++;
-            }
+            if (w[j] < 0 || slot[j] >= 0 || !bucket_spilled(t[j])) continue;
+            slot[j] = tab_continue_slot(s_tab, bucket_addr(w[j]) >> 4, w[j]);
           }
         }
       }
-      if constexpr (MASKS) {
+      const int32_t x32 = (int32_t)x;  // ids are below 2^31
+      const int32_t y = s_rec[k].w;
+      const int32_t lo = min(x32, y);
+      uint32_t nlo = 0;  // hits whose smallest vertex is min(x, y)
 #pragma unroll
-        for (int j = 0; j < 4; j++)
-          if ((hitm >> j) & 1u) credit(w[j], k);
+      for (int j = 0; j < 4; j++) {
+        const bool h = slot[j] >= 0;
+        const bool to_w = h && w[j] < lo;
+        hits += h;
+        nlo += h && !to_w;
+        if (to_w) atomicAdd(&s_wcnt[slot[j] >> 1], 1u << (16 * (slot[j] & 1)));
       }
+      if (x32 < y)
+        at_x += nlo;
+      else if (nlo)
+        atomicAdd(&s_ycnt[k], nlo);
     };
     // Counting without per-vertex credit: each id is compared with the four
     // slots of its home bucket as raw words; the compare results are lane
@@ -873,13 +938,13 @@ __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(W
       for (int j = 0; j < 4; j++) {
         if (w[j] < 0) continue;
         const int64_t p = lower_bound_dev<int32_t>(ot, a0, a0 + dx, w[j]);
-        if (p < a0 + dx && ot[p] == w[j]) credit(w[j], k);
+        if (p < a0 + dx && ot[p] == w[j]) credit(w[j], k, -1);
       }
     };
 
     auto probe = [&](const int4 v, int k, int r) {
-      if constexpr (MASKS)
-        probe_hash(v, k, r);
+      if constexpr (PV)
+        probe_pv(v, k, r);
       else
         probe_count(v, k, r);
     };
@@ -914,6 +979,14 @@ __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(W
       __syncthreads();
       const uint64_t bx = block_sum_u64<kT>((uint64_t)at_x, s_red);
       if (tid == 0 && bx) atomicAdd(&pv[x], (unsigned long long)bx);
+      // Partner credit of the tile's items (the block sums above ended every probe).
+      for (int k = tid; k < nitems; k += kT) {
+        const uint32_t c = s_ycnt[k];
+        if (c) {
+          atomicAdd(&pv[s_rec[k].w], (unsigned long long)c);
+          s_ycnt[k] = 0;
+        }
+      }
     }
     if constexpr (CM == kCreditRange) {
       __syncthreads();
@@ -937,6 +1010,7 @@ __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(W
     par ^= 1;
     __syncthreads();  // next tile published; LDS free for reuse
   }
+  flush_slots();  // the last table's counters (the loop ends on a barrier)
 }
 
 // Sums over oriented edges of min(d+(u), d+(v)) (all edges, and the heavy ones)
