@@ -5,10 +5,11 @@
 // Out-lists hold ascending ids.  d+(x) = |N+(x)|.
 //
 // Edges are binned by the longer of their two out-lists:
-//  * light edges (both lists <= light_cut): intersected directly, each id of
-//    the shorter list searched in the longer one -- by the whole wave, one
-//    edge at a time, where light edges are sparse, or one lane per edge where
-//    they are dense;
+//  * light edges (both lists <= light_cut): intersected directly -- by the
+//    whole wave, one edge at a time, each id of the shorter list searched in
+//    the longer one, where light edges are sparse, or one lane per edge where
+//    they are dense (blocks of eight ids in registers against 16-byte pieces
+//    of the longer list);
 //  * heavy edges: the edge belongs to its endpoint x with the longer list (ties
 //    to the source).  Each such x gets an item list -- one 16-byte record
 //    {offset, length, id} per other endpoint y of its heavy edges.
@@ -110,6 +111,41 @@ __device__ __forceinline__ EdgeLists edge_lists(int64_t a0, int64_t a1, int64_t 
 // Light edges of one entry slot in a wave beyond which each lane intersects its
 // own instead of the whole wave taking them one by one.
 constexpr int kWaveWideLight = 4;
+
+// |shorter ∩ longer| for two sorted lists of the oriented ids (both at most
+// the light cut long), for plain counts.  Blocks of eight
+// ids of the shorter list are held in registers and the longer list is walked
+// in 16-byte pieces, each compared with the whole block (32 compares); a piece
+// reaching past the block's largest id is kept for the next block.  About a
+// dozen loads per edge, none dependent on a search.  Loads stay inside the ids
+// plus their kOtailsPad padding: a piece starts before its list's end, and a
+// block's second piece is read only where it does.
+__device__ __forceinline__ uint32_t light_intersect(const int32_t* __restrict__ ot, const EdgeLists& L) {
+  auto piece = [&](int64_t i) -> v4i_a4 {
+    return *reinterpret_cast<const gv4i_a4*>(reinterpret_cast<uint64_t>(ot + i));
+  };
+  uint32_t h = 0;
+  int64_t lc = L.l0;
+  for (int64_t sc = L.s0; sc < L.s1; sc += 8) {
+    const int64_t ns = L.s1 - sc;
+    const v4i_a4 p = piece(sc);
+    v4i_a4 q = {-1, -1, -1, -1};
+    if (ns > 4) q = piece(sc + 4);
+    const int32_t a[8] = {p.x,               ns > 1 ? p.y : -1, ns > 2 ? p.z : -1, ns > 3 ? p.w : -1,
+                          ns > 4 ? q.x : -1, ns > 5 ? q.y : -1, ns > 6 ? q.z : -1, ns > 7 ? q.w : -1};
+    const int32_t amax = max(max(max(a[0], a[1]), max(a[2], a[3])), max(max(a[4], a[5]), max(a[6], a[7])));
+    while (lc < L.l1) {
+      const int64_t nl = L.l1 - lc;
+      const v4i_a4 t = piece(lc);
+      const int32_t b[4] = {t.x, nl > 1 ? t.y : -2, nl > 2 ? t.z : -2, nl > 3 ? t.w : -2};
+#pragma unroll
+      for (int i = 0; i < 8; i++) h += (a[i] == b[0]) | (a[i] == b[1]) | (a[i] == b[2]) | (a[i] == b[3]);
+      if (max(max(b[0], b[1]), max(b[2], b[3])) > amax) break;  // may meet the next block
+      lc += 4;
+    }
+  }
+  return h;
+}
 
 // Lane l's value of v, in every lane (l uniform).
 __device__ __forceinline__ int64_t readlane64(int64_t v, int l) {
@@ -341,7 +377,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8))) 
     //    edge at a time: the lanes take the ids of the shorter list and search
     //    each in the longer one (both at most light_cut long), instead of one
     //    lane walking the list while the others wait.  Where many do (small
-    //    sparse graphs), each lane walks its own edge.
+    //    sparse graphs), each lane intersects its own edge (light_intersect
+    //    for plain counts).
     const bool live = e0 < ee;
     int64_t u = u0, cur = 0, nxt = 0, a0 = 0, a1 = 0;
     if (live) {
@@ -379,19 +416,26 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8))) 
         if (act && ((light >> j) & 1u)) {
           const int64_t v = ids[j];
           const EdgeLists L = edge_lists(a0, a1, (int64_t)gb[j], (int64_t)gb[j] + (int64_t)gl[j]);
-          const int64_t mmin = u < v ? u : v;
           uint32_t h = 0;
-          int64_t pos = L.l0;
-          for (int64_t k = L.s0; k < L.s1 && pos < L.l1; k++) {
-            const int32_t w = ot[k];
-            pos = lower_bound_dev<int32_t>(ot, pos, L.l1, w);  // restarted after the previous hit
-            if (pos < L.l1 && ot[pos] == w) {
-              h++;
-              pos++;
-              if constexpr (PV) {
-                atomicAdd(&pv[(int64_t)w < mmin ? (int64_t)w : mmin], 1ull);
-              } else if constexpr (CM == kCreditRange) {
-                range_credit(u, v, w, rlo, rhi, r_owned, r_local);
+          if constexpr (CM == kCreditTotal) {
+            h = light_intersect(ot, L);
+          } else {
+            // Credit modes need each common id: each id of the shorter list is
+            // searched in the rest of the longer one (fewer registers than the
+            // blocks, with the credit inside).
+            const int64_t mmin = u < v ? u : v;
+            int64_t pos = L.l0;
+            for (int64_t k = L.s0; k < L.s1 && pos < L.l1; k++) {
+              const int32_t w = ot[k];
+              pos = lower_bound_dev<int32_t>(ot, pos, L.l1, w);  // restarted after the previous hit
+              if (pos < L.l1 && ot[pos] == w) {
+                h++;
+                pos++;
+                if constexpr (PV) {
+                  atomicAdd(&pv[(int64_t)w < mmin ? (int64_t)w : mmin], 1ull);
+                } else if constexpr (CM == kCreditRange) {
+                  range_credit(u, v, w, rlo, rhi, r_owned, r_local);
+                }
               }
             }
           }
