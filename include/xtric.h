@@ -69,7 +69,8 @@ int xt_last_timing(const xt_ctx* ctx, xt_timing* out);
 /* Tuning knobs (set by xxtric_amd.autotune from measurements; 0 keeps the
  * current value): light_cut -- edges whose two out-lists are both at most this
  * long are intersected one per thread, the rest by the group kernel (default
- * 32); group_blocks -- persistent group-kernel workgroups per CU (1..4,
+ * 32 for one graph, 64 for batches and the server; a value set here applies
+ * to both, -1 restores the defaults); group_blocks -- persistent group-kernel workgroups per CU (1..4,
  * default 4); group_round -- 16-byte list chunks each group-kernel thread has
  * in flight per streaming round (1, 2 or 4, default 2).  None of them changes
  * a result. */

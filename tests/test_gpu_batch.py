@@ -66,7 +66,7 @@ def test_permuted_order_and_split(ctx):
     graphs = [xxtric_amd.batch_graph_csr(SEED, g) for g in gs]
     exp = _expected(gs)
     perm = np.random.default_rng(5).permutation(len(gs))
-    for cut in (1, 16, 64, 1 << 30):
+    for cut in (1, 16, 64, 1 << 30, xxtric_amd.DEFAULT_LIGHT_CUT):  # the last: back to the defaults
         ctx.tune(small_cut=cut)
         try:
             b = xxtric_amd.Batch(ctx, 2_000_000, 20_000_000, 1024)

@@ -16,7 +16,7 @@ from .capi import (XT_BATCH_HIPGRAPH, XT_BATCH_PER_GRAPH, XT_BATCH_PERSISTENT,  
                    XT_CHECK_SYMMETRY, XT_ORIENT_DEGREE, XT_ORIENT_ID, XT_PER_VERTEX,
                    XT_VALIDATE, XtError)
 
-DEFAULT_LIGHT_CUT = 32  # xt_ctx default: edges with both out-lists <= 32 are "light"
+DEFAULT_LIGHT_CUT = -1  # tune() value that restores the defaults: light = both out-lists <= 32 (64 in batches)
 
 __all__ = ["Context", "DeviceGraph", "TriangleCountingSingle", "rgg_generate", "rgg_radius",
            "vite_read", "vite_write", "edgelist_to_csr", "batch_params", "batch_graph_csr",
@@ -72,7 +72,8 @@ class Context:
 
     def tune(self, small_cut=0, group_blocks=0, group_round=0):
         """Set the light-edge cut, the group kernel's workgroups per CU and/or its
-        chunks in flight per thread (0 = keep).  No setting changes a result."""
+        chunks in flight per thread (0 = keep; small_cut=-1 restores the default
+        cuts).  No setting changes a result."""
         self._check(self._L.xt_ctx_tune(self.h, small_cut, group_blocks, group_round))
 
     def force_wide(self, on=True):

@@ -230,10 +230,11 @@ int xt_last_timing(const xt_ctx* ctx, xt_timing* out) {
 }
 
 int xt_ctx_tune(xt_ctx* ctx, int light_cut, int group_blocks, int group_round) {
-  if (!ctx || group_blocks > 4 || light_cut < 0 || group_blocks < 0) return XT_ERR_ARG;
+  if (!ctx || group_blocks > 4 || light_cut < -1 || group_blocks < 0) return XT_ERR_ARG;
   if (group_round != 0 && group_round != 1 && group_round != 2 && group_round != 4) return XT_ERR_ARG;
   std::lock_guard<std::recursive_mutex> lk(ctx->mu);
   if (light_cut > 0) ctx->small_cut = light_cut;
+  if (light_cut == -1) ctx->small_cut = 0;  // back to the defaults (one graph / batches)
   if (group_blocks > 0) ctx->group_blocks = group_blocks;
   if (group_round > 0) ctx->group_round = group_round;
   return XT_OK;

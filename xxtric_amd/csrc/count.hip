@@ -1130,7 +1130,7 @@ int count_oriented(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne_bound, const 
   const uint8_t* oflags = s.oflags.as<uint8_t>();
   unsigned long long* pvp = (unsigned long long*)pv;
   unsigned long long* gt = (unsigned long long*)gtotal;
-  const int cut = ctx->small_cut;
+  const int cut = ctx->small_cut > 0 ? ctx->small_cut : (gid ? kBatchLightCut : kLightCut);
   int64_t g = (nv + 1 + 255) / 256;
   if (g > 65536) g = 65536;
   // every oriented offset and out-degree fits 32 bits (unless the wide layout is forced)
@@ -1226,7 +1226,7 @@ int alg_stats(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne_bound, hipStream_t
   XT_TRY(fill_u64(ctx, counters + kStatMinHeavy, 2, 0, st));
   hipLaunchKernelGGL(k_alg_stats, dim3((unsigned)nchunks), dim3(kBlock), 0, st, nv,
                      (const int64_t*)s.ooff.as<int64_t>(), (const int32_t*)s.otails.as<int32_t>(),
-                     ctx->small_cut, counters);
+                     ctx->small_cut > 0 ? ctx->small_cut : kLightCut, counters);
   XT_HIP(ctx, hipGetLastError());
   return XT_OK;
 }

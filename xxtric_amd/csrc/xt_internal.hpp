@@ -20,6 +20,8 @@ constexpr int kChunk = kBlock * kPerThread;  // 2048
 // The oriented ids are read in 16-byte chunks from each list's first id; the
 // last chunk of the last list may read up to three ids past |E+|.
 constexpr int64_t kOtailsPad = 4;
+constexpr int kLightCut = 32;       // default light cut, one graph
+constexpr int kBatchLightCut = 64;  // default light cut, batches and the server
 constexpr int kOffCap = 2048;
 
 // Counter slots in Scratch::counters (uint64).
@@ -85,7 +87,10 @@ struct xt_ctx {
   xt::Scratch s;
   hipEvent_t ev[xt::kNumEv] = {};
   xt_timing timing = {};
-  int small_cut = 32;     // light edges: both out-lists at most this long
+  // Light edges: both out-lists at most this long; 0 picks kLightCut for one
+  // graph and kBatchLightCut for batches (lists of small sparse graphs, where
+  // per-thread intersections beat the group kernel up to ~64 ids).
+  int small_cut = 0;
   int group_blocks = 4;   // persistent group-kernel workgroups per CU
   int group_round = 2;    // 16-byte chunks per thread per group-kernel streaming round (1, 2, 4)
   bool force_wide = false;  // use the 64-bit offset layout of graphs with >= 2^32 entries everywhere
