@@ -166,6 +166,36 @@ def test_complete_graph_fills_the_lds_hash(ctx, n):
     assert ctx.count_csr(off, tails) == tot
 
 
+def test_per_vertex_owner_with_many_tiles(ctx):
+    """Id orientation gives vertex 2000, joined to every other vertex, an
+    out-list of 4000 ids (inside the LDS hash) and 6000 items: 24 group tiles of
+    one owner, more than the 16 after which the per-vertex slot counters
+    (16-bit, in LDS) are added to HBM while the owner's table stays.  Random
+    edges among all vertices close triangles credited to the owner, to the
+    item's partner and to the table id."""
+    n, x = 6001, 2000
+    rng = np.random.default_rng(11)
+    a = rng.integers(0, n, 180000)
+    b = rng.integers(0, n, 180000)
+    a = np.concatenate([a, np.full(n, x)])
+    b = np.concatenate([b, np.arange(n)])
+    keep = a != b
+    lo, hi = np.minimum(a, b)[keep], np.maximum(a, b)[keep]
+    key = np.unique(np.concatenate([lo * n + hi, hi * n + lo]))
+    src, dst = key // n, key % n
+    off = np.zeros(n + 1, np.int64)
+    np.add.at(off, src + 1, 1)
+    off = np.cumsum(off)
+    tails = dst.astype(np.int64)
+    exp, exp_pv = O.count_reference(off, tails, per_vertex=True)
+    _say("owner with many tiles", n, len(tails))
+    tot, pv = ctx.count_csr(off, tails, per_vertex=True, orient="id")
+    assert ctx.timing()["n_tiles"] >= 24
+    assert tot == exp and (pv == exp_pv).all()
+    tot, pv = ctx.count_csr(off, tails, per_vertex=True)  # degree orientation too
+    assert tot == exp and (pv == exp_pv).all()
+
+
 def test_invalid_inputs_are_rejected(ctx):
     off, tails = csr_from_edges(4, [(0, 1), (1, 2), (0, 2), (2, 3)])
     bad = tails.copy()
