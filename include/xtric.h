@@ -32,6 +32,7 @@ extern "C" {
 #define XT_ERR_CAPACITY (-5)  /* batch / server capacity exceeded */
 #define XT_ERR_NOTFOUND (-6)  /* unknown ticket / slot */
 #define XT_ERR_STATE (-7)     /* call not valid in the object's current state */
+#define XT_ERR_INTERNAL (-8)  /* unexpected host-side failure (e.g. a thread could not start) */
 #define XT_PENDING 1          /* xt_serve_poll: request not finished yet */
 
 /* ---- counting flags ----------------------------------------------------- */
@@ -228,7 +229,8 @@ int xt_edgelist_to_csr(const char* path, int64_t* nv, int64_t* ne, int64_t* offs
  *     split point on ties; O(nv * nparts) time, 4 * nv * nparts bytes.
  *     XT_ERR_ARG when nparts > nv - 1 or where the reference's table walk
  *     would leave its table (its result is undefined there).
- * XT_ERR_INPUT for decreasing or negative offsets; nv < 2^31. */
+ * XT_ERR_INPUT for decreasing or negative offsets; XT_ERR_NOMEM when the
+ * linear program's host tables cannot be allocated; nv < 2^31. */
 #define XT_PARTITION_GREEDY 0
 #define XT_PARTITION_LINEAR 1
 int xt_partition_vertices(int64_t nv, const int64_t* offsets, int64_t nparts, int method,

@@ -23,6 +23,7 @@ XT_ERR_INPUT = -4
 XT_ERR_CAPACITY = -5
 XT_ERR_NOTFOUND = -6
 XT_ERR_STATE = -7
+XT_ERR_INTERNAL = -8
 XT_PENDING = 1
 
 XT_ORIENT_DEGREE = 0x0

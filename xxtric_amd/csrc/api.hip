@@ -181,278 +181,312 @@ extern "C" {
 const char* xt_version(void) { return "xtric 0.1 (gfx950)"; }
 
 int xt_ctx_create(int device, xt_ctx** out) {
-  if (!out) return XT_ERR_ARG;
-  *out = nullptr;
-  int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return XT_ERR_HIP;
-  if (device < 0 || device >= ndev) return XT_ERR_ARG;
-  if (hipSetDevice(device) != hipSuccess) return XT_ERR_HIP;
-  xt_ctx* ctx = new xt_ctx();
-  ctx->device = device;
-  int ncu = 0;
-  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu > 0)
-    ctx->num_cus = ncu;
-  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
-    delete ctx;
-    return XT_ERR_HIP;
-  }
-  for (int i = 0; i < kNumEv; i++)
-    if (hipEventCreate(&ctx->ev[i]) != hipSuccess) {
+  return xt_guarded([&]() -> int {
+    if (!out) return XT_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return XT_ERR_HIP;
+    if (device < 0 || device >= ndev) return XT_ERR_ARG;
+    if (hipSetDevice(device) != hipSuccess) return XT_ERR_HIP;
+    xt_ctx* ctx = new xt_ctx();
+    ctx->device = device;
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu > 0)
+      ctx->num_cus = ncu;
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
       delete ctx;
       return XT_ERR_HIP;
     }
-  if (ensure(ctx, ctx->s.counters, kNumSlots * sizeof(uint64_t)) != XT_OK) {
-    delete ctx;
-    return XT_ERR_NOMEM;
-  }
-  *out = ctx;
-  return XT_OK;
+    for (int i = 0; i < kNumEv; i++)
+      if (hipEventCreate(&ctx->ev[i]) != hipSuccess) {
+        delete ctx;
+        return XT_ERR_HIP;
+      }
+    if (ensure(ctx, ctx->s.counters, kNumSlots * sizeof(uint64_t)) != XT_OK) {
+      delete ctx;
+      return XT_ERR_NOMEM;
+    }
+    *out = ctx;
+    return XT_OK;
+  });
 }
 
 int xt_ctx_destroy(xt_ctx* ctx) {
-  if (!ctx) return XT_ERR_ARG;
-  (void)hipSetDevice(ctx->device);
-  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-  release(ctx->s);
-  for (int i = 0; i < kNumEv; i++)
-    if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
-  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
-  delete ctx;
-  return XT_OK;
+  return xt_guarded([&]() -> int {
+    if (!ctx) return XT_ERR_ARG;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    release(ctx->s);
+    for (int i = 0; i < kNumEv; i++)
+      if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return XT_OK;
+  });
 }
 
 const char* xt_last_error(const xt_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
 int xt_last_timing(const xt_ctx* ctx, xt_timing* out) {
-  if (!ctx || !out) return XT_ERR_ARG;
-  *out = ctx->timing;
-  return XT_OK;
+  return xt_guarded([&]() -> int {
+    if (!ctx || !out) return XT_ERR_ARG;
+    *out = ctx->timing;
+    return XT_OK;
+  });
 }
 
 int xt_ctx_tune(xt_ctx* ctx, int light_cut, int group_blocks, int group_round) {
-  if (!ctx || group_blocks > 4 || light_cut < -1 || group_blocks < 0) return XT_ERR_ARG;
-  if (group_round != 0 && group_round != 1 && group_round != 2 && group_round != 4) return XT_ERR_ARG;
-  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
-  if (light_cut > 0) ctx->small_cut = light_cut;
-  if (light_cut == -1) ctx->small_cut = 0;  // back to the defaults (one graph / batches)
-  if (group_blocks > 0) ctx->group_blocks = group_blocks;
-  if (group_round > 0) ctx->group_round = group_round;
-  return XT_OK;
+  return xt_guarded([&]() -> int {
+    if (!ctx || group_blocks > 4 || light_cut < -1 || group_blocks < 0) return XT_ERR_ARG;
+    if (group_round != 0 && group_round != 1 && group_round != 2 && group_round != 4) return XT_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    if (light_cut > 0) ctx->small_cut = light_cut;
+    if (light_cut == -1) ctx->small_cut = 0;  // back to the defaults (one graph / batches)
+    if (group_blocks > 0) ctx->group_blocks = group_blocks;
+    if (group_round > 0) ctx->group_round = group_round;
+    return XT_OK;
+  });
 }
 
 int xt_ctx_force_wide(xt_ctx* ctx, int on) {
-  if (!ctx) return XT_ERR_ARG;
-  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
-  ctx->force_wide = on != 0;
-  return XT_OK;
+  return xt_guarded([&]() -> int {
+    if (!ctx) return XT_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    ctx->force_wide = on != 0;
+    return XT_OK;
+  });
 }
 
 int xt_dev_alloc(xt_ctx* ctx, size_t bytes, void** out) {
-  if (!ctx || !out) return XT_ERR_ARG;
-  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
-  XT_HIP(ctx, hipSetDevice(ctx->device));
-  *out = nullptr;
-  XT_HIP(ctx, hipMalloc(out, bytes > 0 ? bytes : 1));
-  return XT_OK;
+  return xt_guarded([&]() -> int {
+    if (!ctx || !out) return XT_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    XT_HIP(ctx, hipSetDevice(ctx->device));
+    *out = nullptr;
+    XT_HIP(ctx, hipMalloc(out, bytes > 0 ? bytes : 1));
+    return XT_OK;
+  });
 }
 
 int xt_dev_free(xt_ctx* ctx, void* p) {
-  if (!ctx) return XT_ERR_ARG;
-  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
-  XT_HIP(ctx, hipSetDevice(ctx->device));
-  if (p) {
-    XT_HIP(ctx, hipDeviceSynchronize());
-    XT_HIP(ctx, hipFree(p));
-  }
-  return XT_OK;
+  return xt_guarded([&]() -> int {
+    if (!ctx) return XT_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    XT_HIP(ctx, hipSetDevice(ctx->device));
+    if (p) {
+      XT_HIP(ctx, hipDeviceSynchronize());
+      XT_HIP(ctx, hipFree(p));
+    }
+    return XT_OK;
+  });
 }
 
 int xt_memcpy(xt_ctx* ctx, void* dst, const void* src, size_t bytes, int kind) {
-  if (!ctx || (bytes > 0 && (!dst || !src))) return XT_ERR_ARG;
-  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
-  XT_HIP(ctx, hipSetDevice(ctx->device));
-  if (kind == XT_D2D_KERNEL) {
-    if (bytes % 16 != 0 || ((uintptr_t)dst | (uintptr_t)src) % 16 != 0)
-      return set_err(ctx, XT_ERR_ARG, "kernel copy needs 16-byte aligned sizes and pointers");
-    XT_TRY(launch_copy16(ctx, dst, src, bytes, ctx->stream));
-    XT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return xt_guarded([&]() -> int {
+    if (!ctx || (bytes > 0 && (!dst || !src))) return XT_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    XT_HIP(ctx, hipSetDevice(ctx->device));
+    if (kind == XT_D2D_KERNEL) {
+      if (bytes % 16 != 0 || ((uintptr_t)dst | (uintptr_t)src) % 16 != 0)
+        return set_err(ctx, XT_ERR_ARG, "kernel copy needs 16-byte aligned sizes and pointers");
+      XT_TRY(launch_copy16(ctx, dst, src, bytes, ctx->stream));
+      XT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      return XT_OK;
+    }
+    hipMemcpyKind k = kind == XT_H2D ? hipMemcpyHostToDevice
+                      : kind == XT_D2H ? hipMemcpyDeviceToHost
+                      : kind == XT_D2D ? hipMemcpyDeviceToDevice
+                                       : hipMemcpyDefault;
+    if (kind < XT_H2D || kind > XT_D2D_KERNEL) return set_err(ctx, XT_ERR_ARG, "bad copy kind");
+    if (bytes == 0) return XT_OK;
+    XT_HIP(ctx, hipDeviceSynchronize());
+    XT_HIP(ctx, hipMemcpy(dst, src, bytes, k));
     return XT_OK;
-  }
-  hipMemcpyKind k = kind == XT_H2D ? hipMemcpyHostToDevice
-                    : kind == XT_D2H ? hipMemcpyDeviceToHost
-                    : kind == XT_D2D ? hipMemcpyDeviceToDevice
-                                     : hipMemcpyDefault;
-  if (kind < XT_H2D || kind > XT_D2D_KERNEL) return set_err(ctx, XT_ERR_ARG, "bad copy kind");
-  if (bytes == 0) return XT_OK;
-  XT_HIP(ctx, hipDeviceSynchronize());
-  XT_HIP(ctx, hipMemcpy(dst, src, bytes, k));
-  return XT_OK;
+  });
 }
 
 int xt_dev_sync(xt_ctx* ctx) {
-  if (!ctx) return XT_ERR_ARG;
-  XT_HIP(ctx, hipSetDevice(ctx->device));
-  XT_HIP(ctx, hipDeviceSynchronize());
-  return XT_OK;
+  return xt_guarded([&]() -> int {
+    if (!ctx) return XT_ERR_ARG;
+    XT_HIP(ctx, hipSetDevice(ctx->device));
+    XT_HIP(ctx, hipDeviceSynchronize());
+    return XT_OK;
+  });
 }
 
 int xt_count_csr(xt_ctx* ctx, int64_t nv, int64_t ne, const int64_t* offsets, const int64_t* tails,
                  uint32_t flags, uint64_t* total, uint64_t* per_vertex) {
-  if (!ctx) return XT_ERR_ARG;
-  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
-  XT_HIP(ctx, hipSetDevice(ctx->device));
-  if (!total || (ne > 0 && !tails)) return set_err(ctx, XT_ERR_ARG, "null output or tails");
-  if (per_vertex && !(flags & XT_PER_VERTEX)) return set_err(ctx, XT_ERR_ARG, "per_vertex needs XT_PER_VERTEX");
-  XT_TRY(check_host_offsets(ctx, nv, ne, offsets));
-  hipStream_t st = ctx->stream;
-  XT_TRY(ensure(ctx, ctx->s.in_off, (size_t)(nv + 1) * sizeof(int64_t)));
-  XT_TRY(ensure(ctx, ctx->s.in_tails, (size_t)(ne > 0 ? ne : 1) * sizeof(int32_t)));
-  int64_t* d_off = ctx->s.in_off.as<int64_t>();
-  int32_t* d_tails = ctx->s.in_tails.as<int32_t>();
-  XT_TRY(upload_csr(ctx, nv, ne, offsets, tails, 8, d_off, d_tails, st));
-  if (flags & XT_VALIDATE) XT_TRY(validate<int32_t>(ctx, nv, ne, d_off, d_tails, (flags & XT_CHECK_SYMMETRY) != 0, st));
-  XT_TRY(count_device_csr<int32_t>(ctx, ctx->s, nv, ne, d_off, d_tails, flags, st, true));
-  return finish_count(ctx, nv, flags, st, true, total, per_vertex);
+  return xt_guarded([&]() -> int {
+    if (!ctx) return XT_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    XT_HIP(ctx, hipSetDevice(ctx->device));
+    if (!total || (ne > 0 && !tails)) return set_err(ctx, XT_ERR_ARG, "null output or tails");
+    if (per_vertex && !(flags & XT_PER_VERTEX)) return set_err(ctx, XT_ERR_ARG, "per_vertex needs XT_PER_VERTEX");
+    XT_TRY(check_host_offsets(ctx, nv, ne, offsets));
+    hipStream_t st = ctx->stream;
+    XT_TRY(ensure(ctx, ctx->s.in_off, (size_t)(nv + 1) * sizeof(int64_t)));
+    XT_TRY(ensure(ctx, ctx->s.in_tails, (size_t)(ne > 0 ? ne : 1) * sizeof(int32_t)));
+    int64_t* d_off = ctx->s.in_off.as<int64_t>();
+    int32_t* d_tails = ctx->s.in_tails.as<int32_t>();
+    XT_TRY(upload_csr(ctx, nv, ne, offsets, tails, 8, d_off, d_tails, st));
+    if (flags & XT_VALIDATE) XT_TRY(validate<int32_t>(ctx, nv, ne, d_off, d_tails, (flags & XT_CHECK_SYMMETRY) != 0, st));
+    XT_TRY(count_device_csr<int32_t>(ctx, ctx->s, nv, ne, d_off, d_tails, flags, st, true));
+    return finish_count(ctx, nv, flags, st, true, total, per_vertex);
+  });
 }
 
 int xt_single_gpu_count(xt_ctx* ctx, int64_t nv, const int64_t* d_indices, const int64_t* d_edge_list,
                         int64_t* d_local_count, void* stream) {
-  if (!ctx) return XT_ERR_ARG;
-  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
-  XT_HIP(ctx, hipSetDevice(ctx->device));
-  if (nv < 0 || !d_indices || !d_local_count) return set_err(ctx, XT_ERR_ARG, "bad arguments");
-  if (nv == 0) return XT_OK;
-  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
-  int64_t ne = 0;
-  XT_HIP(ctx, hipMemcpyAsync(&ne, d_indices + nv, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-  XT_HIP(ctx, hipStreamSynchronize(st));
-  if (ne < 0) return set_err(ctx, XT_ERR_INPUT, "negative edge count in d_indices[nv]");
-  if (ne > 0 && !d_edge_list) return set_err(ctx, XT_ERR_ARG, "null d_edge_list");
-  // Offsets must be a valid index (and rows sorted) before the chunked
-  // kernels search them; the reference launcher assumes the same.
-  XT_TRY(validate<int64_t>(ctx, nv, ne, d_indices, d_edge_list, false, st));
-  XT_TRY(count_device_csr<int64_t>(ctx, ctx->s, nv, ne, d_indices, d_edge_list, XT_ORIENT_DEGREE, st, false));
-  uint64_t bad = 0;
-  XT_HIP(ctx, hipMemcpyAsync(&bad, ctx->s.counters.as<uint64_t>() + kBadIds, sizeof(bad),
-                             hipMemcpyDeviceToHost, st));
-  XT_HIP(ctx, hipStreamSynchronize(st));
-  if (bad) return set_err(ctx, XT_ERR_INPUT, "neighbour id out of range [0, nv)");
-  return launch_add_total(ctx, d_local_count, st);
+  return xt_guarded([&]() -> int {
+    if (!ctx) return XT_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    XT_HIP(ctx, hipSetDevice(ctx->device));
+    if (nv < 0 || !d_indices || !d_local_count) return set_err(ctx, XT_ERR_ARG, "bad arguments");
+    if (nv == 0) return XT_OK;
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+    int64_t ne = 0;
+    XT_HIP(ctx, hipMemcpyAsync(&ne, d_indices + nv, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    XT_HIP(ctx, hipStreamSynchronize(st));
+    if (ne < 0) return set_err(ctx, XT_ERR_INPUT, "negative edge count in d_indices[nv]");
+    if (ne > 0 && !d_edge_list) return set_err(ctx, XT_ERR_ARG, "null d_edge_list");
+    // Offsets must be a valid index (and rows sorted) before the chunked
+    // kernels search them; the reference launcher assumes the same.
+    XT_TRY(validate<int64_t>(ctx, nv, ne, d_indices, d_edge_list, false, st));
+    XT_TRY(count_device_csr<int64_t>(ctx, ctx->s, nv, ne, d_indices, d_edge_list, XT_ORIENT_DEGREE, st, false));
+    uint64_t bad = 0;
+    XT_HIP(ctx, hipMemcpyAsync(&bad, ctx->s.counters.as<uint64_t>() + kBadIds, sizeof(bad),
+                               hipMemcpyDeviceToHost, st));
+    XT_HIP(ctx, hipStreamSynchronize(st));
+    if (bad) return set_err(ctx, XT_ERR_INPUT, "neighbour id out of range [0, nv)");
+    return launch_add_total(ctx, d_local_count, st);
+  });
 }
 
 int xt_single_gpu_count_masked(xt_ctx* ctx, int64_t nedges, const int64_t* d_edge_mask,
                                const int64_t* d_indices, const int64_t* d_edge_list,
                                int64_t* d_local_count, void* stream) {
-  if (!ctx) return XT_ERR_ARG;
-  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
-  XT_HIP(ctx, hipSetDevice(ctx->device));
-  if (nedges < 0 || (nedges > 0 && (!d_edge_mask || !d_indices || !d_edge_list)) || !d_local_count)
-    return set_err(ctx, XT_ERR_ARG, "bad arguments");
-  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
-  return launch_masked(ctx, nedges, d_edge_mask, d_indices, d_edge_list, d_local_count, st);
+  return xt_guarded([&]() -> int {
+    if (!ctx) return XT_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    XT_HIP(ctx, hipSetDevice(ctx->device));
+    if (nedges < 0 || (nedges > 0 && (!d_edge_mask || !d_indices || !d_edge_list)) || !d_local_count)
+      return set_err(ctx, XT_ERR_ARG, "bad arguments");
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+    return launch_masked(ctx, nedges, d_edge_mask, d_indices, d_edge_list, d_local_count, st);
+  });
 }
 
 int xt_local_count_gpu_inner(xt_ctx* ctx, int64_t start, int64_t end, int64_t jstart, int64_t jend,
                              const int64_t* d_edge_list, int64_t* d_local_count, void* stream) {
-  if (!ctx) return XT_ERR_ARG;
-  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
-  XT_HIP(ctx, hipSetDevice(ctx->device));
-  if (!d_edge_list || !d_local_count || start < 0 || jstart < 0) return set_err(ctx, XT_ERR_ARG, "bad arguments");
-  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
-  return launch_inner(ctx, start, end, jstart, jend, d_edge_list, d_local_count, st);
+  return xt_guarded([&]() -> int {
+    if (!ctx) return XT_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    XT_HIP(ctx, hipSetDevice(ctx->device));
+    if (!d_edge_list || !d_local_count || start < 0 || jstart < 0) return set_err(ctx, XT_ERR_ARG, "bad arguments");
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+    return launch_inner(ctx, start, end, jstart, jend, d_edge_list, d_local_count, st);
+  });
 }
 
 // ---- resident graphs ----------------------------------------------------
 int xt_graph_from_csr(xt_ctx* ctx, int64_t nv, int64_t ne, const int64_t* offsets, const void* tails,
                       int tails_bytes, uint32_t flags, xt_graph** out) {
-  if (!ctx || !out) return XT_ERR_ARG;
-  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
-  XT_HIP(ctx, hipSetDevice(ctx->device));
-  *out = nullptr;
-  if (tails_bytes != 4 && tails_bytes != 8) return set_err(ctx, XT_ERR_ARG, "tails_bytes must be 4 or 8");
-  if (ne > 0 && !tails) return set_err(ctx, XT_ERR_ARG, "null tails");
-  XT_TRY(check_host_offsets(ctx, nv, ne, offsets));
-  xt_graph* g = new xt_graph();
-  g->nv = nv;
-  g->ne = ne;
-  hipError_t e = hipMalloc(&g->off, (size_t)(nv + 1) * sizeof(int64_t));
-  if (e == hipSuccess) e = hipMalloc(&g->tails, (size_t)(ne > 0 ? ne : 1) * sizeof(int32_t));
-  if (e != hipSuccess) {
-    if (g->off) (void)hipFree(g->off);
-    delete g;
-    return hip_err(ctx, e, "hipMalloc(graph)");
-  }
-  int rc = upload_csr(ctx, nv, ne, offsets, tails, tails_bytes, g->off, g->tails, ctx->stream);
-  if (rc == XT_OK && (flags & XT_VALIDATE))
-    rc = validate<int32_t>(ctx, nv, ne, g->off, g->tails, (flags & XT_CHECK_SYMMETRY) != 0, ctx->stream);
-  if (rc != XT_OK) {
-    (void)hipFree(g->off);
-    (void)hipFree(g->tails);
-    delete g;
-    return rc;
-  }
-  *out = g;
-  return XT_OK;
+  return xt_guarded([&]() -> int {
+    if (!ctx || !out) return XT_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    XT_HIP(ctx, hipSetDevice(ctx->device));
+    *out = nullptr;
+    if (tails_bytes != 4 && tails_bytes != 8) return set_err(ctx, XT_ERR_ARG, "tails_bytes must be 4 or 8");
+    if (ne > 0 && !tails) return set_err(ctx, XT_ERR_ARG, "null tails");
+    XT_TRY(check_host_offsets(ctx, nv, ne, offsets));
+    xt_graph* g = new xt_graph();
+    g->nv = nv;
+    g->ne = ne;
+    hipError_t e = hipMalloc(&g->off, (size_t)(nv + 1) * sizeof(int64_t));
+    if (e == hipSuccess) e = hipMalloc(&g->tails, (size_t)(ne > 0 ? ne : 1) * sizeof(int32_t));
+    if (e != hipSuccess) {
+      if (g->off) (void)hipFree(g->off);
+      delete g;
+      return hip_err(ctx, e, "hipMalloc(graph)");
+    }
+    int rc = upload_csr(ctx, nv, ne, offsets, tails, tails_bytes, g->off, g->tails, ctx->stream);
+    if (rc == XT_OK && (flags & XT_VALIDATE))
+      rc = validate<int32_t>(ctx, nv, ne, g->off, g->tails, (flags & XT_CHECK_SYMMETRY) != 0, ctx->stream);
+    if (rc != XT_OK) {
+      (void)hipFree(g->off);
+      (void)hipFree(g->tails);
+      delete g;
+      return rc;
+    }
+    *out = g;
+    return XT_OK;
+  });
 }
 
 int xt_graph_rmat(xt_ctx* ctx, int scale, int edgefactor, double a, double b, double c, uint64_t seed,
                   int permute, xt_graph** out) {
-  if (!ctx || !out) return XT_ERR_ARG;
-  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
-  XT_HIP(ctx, hipSetDevice(ctx->device));
-  *out = nullptr;
-  if (!(a >= 0 && b >= 0 && c >= 0 && a + b + c <= 1.0)) return set_err(ctx, XT_ERR_ARG, "bad R-MAT probabilities");
-  const double s = 4294967296.0;
-  const double t[3] = {a * s, (a + b) * s, (a + b + c) * s};
-  uint32_t thr[3];
-  for (int i = 0; i < 3; i++) thr[i] = t[i] >= s ? 0xFFFFFFFFu : (uint32_t)t[i];
-  xt_graph* g = new xt_graph();
-  int rc = gen_rmat(ctx, scale, edgefactor, thr, seed, permute, g);
-  if (rc != XT_OK) {
-    if (g->off) (void)hipFree(g->off);
-    if (g->tails) (void)hipFree(g->tails);
-    delete g;
-    return rc;
-  }
-  *out = g;
-  return XT_OK;
+  return xt_guarded([&]() -> int {
+    if (!ctx || !out) return XT_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    XT_HIP(ctx, hipSetDevice(ctx->device));
+    *out = nullptr;
+    if (!(a >= 0 && b >= 0 && c >= 0 && a + b + c <= 1.0)) return set_err(ctx, XT_ERR_ARG, "bad R-MAT probabilities");
+    const double s = 4294967296.0;
+    const double t[3] = {a * s, (a + b) * s, (a + b + c) * s};
+    uint32_t thr[3];
+    for (int i = 0; i < 3; i++) thr[i] = t[i] >= s ? 0xFFFFFFFFu : (uint32_t)t[i];
+    xt_graph* g = new xt_graph();
+    int rc = gen_rmat(ctx, scale, edgefactor, thr, seed, permute, g);
+    if (rc != XT_OK) {
+      if (g->off) (void)hipFree(g->off);
+      if (g->tails) (void)hipFree(g->tails);
+      delete g;
+      return rc;
+    }
+    *out = g;
+    return XT_OK;
+  });
 }
 
 int xt_graph_er(xt_ctx* ctx, int64_t n, uint64_t seed, uint64_t thr64, xt_graph** out) {
-  if (!ctx || !out) return XT_ERR_ARG;
-  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
-  XT_HIP(ctx, hipSetDevice(ctx->device));
-  *out = nullptr;
-  xt_graph* g = new xt_graph();
-  int rc = gen_er(ctx, n, seed, thr64, g);
-  if (rc != XT_OK) {
-    if (g->off) (void)hipFree(g->off);
-    if (g->tails) (void)hipFree(g->tails);
-    delete g;
-    return rc;
-  }
-  *out = g;
-  return XT_OK;
+  return xt_guarded([&]() -> int {
+    if (!ctx || !out) return XT_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    XT_HIP(ctx, hipSetDevice(ctx->device));
+    *out = nullptr;
+    xt_graph* g = new xt_graph();
+    int rc = gen_er(ctx, n, seed, thr64, g);
+    if (rc != XT_OK) {
+      if (g->off) (void)hipFree(g->off);
+      if (g->tails) (void)hipFree(g->tails);
+      delete g;
+      return rc;
+    }
+    *out = g;
+    return XT_OK;
+  });
 }
 
 int xt_graph_from_edges(xt_ctx* ctx, int64_t nv, int64_t m, const int64_t* src, const int64_t* dst,
                         xt_graph** out) {
-  if (!ctx || !out) return XT_ERR_ARG;
-  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
-  XT_HIP(ctx, hipSetDevice(ctx->device));
-  *out = nullptr;
-  if (nv < 1 || nv >= (1LL << 31) - 1 || m < 0 || (m > 0 && (!src || !dst)))
-    return set_err(ctx, XT_ERR_ARG, "edge list: need 1 <= nv < 2^31-1, m >= 0 and both id arrays");
-  xt_graph* g = new xt_graph();
-  const int rc = graph_from_edges(ctx, nv, m, src, dst, g);
-  if (rc != XT_OK) {
-    if (g->off) (void)hipFree(g->off);
-    if (g->tails) (void)hipFree(g->tails);
-    delete g;
-    return rc;
-  }
-  *out = g;
-  return XT_OK;
+  return xt_guarded([&]() -> int {
+    if (!ctx || !out) return XT_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    XT_HIP(ctx, hipSetDevice(ctx->device));
+    *out = nullptr;
+    if (nv < 1 || nv >= (1LL << 31) - 1 || m < 0 || (m > 0 && (!src || !dst)))
+      return set_err(ctx, XT_ERR_ARG, "edge list: need 1 <= nv < 2^31-1, m >= 0 and both id arrays");
+    xt_graph* g = new xt_graph();
+    const int rc = graph_from_edges(ctx, nv, m, src, dst, g);
+    if (rc != XT_OK) {
+      if (g->off) (void)hipFree(g->off);
+      if (g->tails) (void)hipFree(g->tails);
+      delete g;
+      return rc;
+    }
+    *out = g;
+    return XT_OK;
+  });
 }
 
 }  // extern "C"
@@ -556,150 +590,170 @@ static int vite_to_device(xt_ctx* ctx, std::FILE* f, xt_graph* g) {
 extern "C" {
 
 int xt_graph_from_vite(xt_ctx* ctx, const char* path, int layout32, uint32_t flags, xt_graph** out) {
-  if (!ctx || !path || !out) return XT_ERR_ARG;
-  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
-  XT_HIP(ctx, hipSetDevice(ctx->device));
-  *out = nullptr;
-  std::FILE* f = std::fopen(path, "rb");
-  if (!f) return set_err(ctx, XT_ERR_ARG, std::string("vite: cannot open ") + path);
-  xt_graph* g = new xt_graph();
-  int rc = layout32 ? vite_to_device<int32_t, float>(ctx, f, g) : vite_to_device<int64_t, double>(ctx, f, g);
-  std::fclose(f);
-  if (rc == XT_OK && (flags & XT_VALIDATE))
-    rc = validate<int32_t>(ctx, g->nv, g->ne, g->off, g->tails, (flags & XT_CHECK_SYMMETRY) != 0, ctx->stream);
-  if (rc != XT_OK) {
-    if (g->off) (void)hipFree(g->off);
-    if (g->tails) (void)hipFree(g->tails);
-    delete g;
-    return rc;
-  }
-  *out = g;
-  return XT_OK;
+  return xt_guarded([&]() -> int {
+    if (!ctx || !path || !out) return XT_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    XT_HIP(ctx, hipSetDevice(ctx->device));
+    *out = nullptr;
+    std::FILE* f = std::fopen(path, "rb");
+    if (!f) return set_err(ctx, XT_ERR_ARG, std::string("vite: cannot open ") + path);
+    xt_graph* g = new xt_graph();
+    int rc = layout32 ? vite_to_device<int32_t, float>(ctx, f, g) : vite_to_device<int64_t, double>(ctx, f, g);
+    std::fclose(f);
+    if (rc == XT_OK && (flags & XT_VALIDATE))
+      rc = validate<int32_t>(ctx, g->nv, g->ne, g->off, g->tails, (flags & XT_CHECK_SYMMETRY) != 0, ctx->stream);
+    if (rc != XT_OK) {
+      if (g->off) (void)hipFree(g->off);
+      if (g->tails) (void)hipFree(g->tails);
+      delete g;
+      return rc;
+    }
+    *out = g;
+    return XT_OK;
+  });
 }
 
 int xt_count_edges(xt_ctx* ctx, int64_t nv, int64_t m, const int64_t* src, const int64_t* dst, uint32_t flags,
                    uint64_t* total, uint64_t* per_vertex) {
-  if (!ctx || !total) return XT_ERR_ARG;
-  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
-  xt_graph* g = nullptr;
-  XT_TRY(xt_graph_from_edges(ctx, nv, m, src, dst, &g));
-  const int rc = xt_graph_count(ctx, g, flags, total, per_vertex);
-  xt_graph_destroy(ctx, g);
-  return rc;
+  return xt_guarded([&]() -> int {
+    if (!ctx || !total) return XT_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    xt_graph* g = nullptr;
+    XT_TRY(xt_graph_from_edges(ctx, nv, m, src, dst, &g));
+    const int rc = xt_graph_count(ctx, g, flags, total, per_vertex);
+    xt_graph_destroy(ctx, g);
+    return rc;
+  });
 }
 
 int xt_graph_info(const xt_graph* g, int64_t* nv, int64_t* ne) {
-  if (!g) return XT_ERR_ARG;
-  if (nv) *nv = g->nv;
-  if (ne) *ne = g->ne;
-  return XT_OK;
+  return xt_guarded([&]() -> int {
+    if (!g) return XT_ERR_ARG;
+    if (nv) *nv = g->nv;
+    if (ne) *ne = g->ne;
+    return XT_OK;
+  });
 }
 
 int xt_graph_download(xt_ctx* ctx, const xt_graph* g, int64_t* offsets, int32_t* tails) {
-  if (!ctx || !g) return XT_ERR_ARG;
-  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
-  XT_HIP(ctx, hipSetDevice(ctx->device));
-  if (offsets)
-    XT_HIP(ctx, hipMemcpy(offsets, g->off, (size_t)(g->nv + 1) * sizeof(int64_t), hipMemcpyDeviceToHost));
-  if (tails && g->ne > 0)
-    XT_HIP(ctx, hipMemcpy(tails, g->tails, (size_t)g->ne * sizeof(int32_t), hipMemcpyDeviceToHost));
-  return XT_OK;
+  return xt_guarded([&]() -> int {
+    if (!ctx || !g) return XT_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    XT_HIP(ctx, hipSetDevice(ctx->device));
+    if (offsets)
+      XT_HIP(ctx, hipMemcpy(offsets, g->off, (size_t)(g->nv + 1) * sizeof(int64_t), hipMemcpyDeviceToHost));
+    if (tails && g->ne > 0)
+      XT_HIP(ctx, hipMemcpy(tails, g->tails, (size_t)g->ne * sizeof(int32_t), hipMemcpyDeviceToHost));
+    return XT_OK;
+  });
 }
 
 int xt_graph_count(xt_ctx* ctx, xt_graph* g, uint32_t flags, uint64_t* total, uint64_t* per_vertex) {
-  if (!ctx || !g || !total) return XT_ERR_ARG;
-  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
-  XT_HIP(ctx, hipSetDevice(ctx->device));
-  if (per_vertex && !(flags & XT_PER_VERTEX)) return set_err(ctx, XT_ERR_ARG, "per_vertex needs XT_PER_VERTEX");
-  hipStream_t st = ctx->stream;
-  XT_TRY(count_device_csr<int32_t>(ctx, ctx->s, g->nv, g->ne, g->off, g->tails, flags, st, true));
-  return finish_count(ctx, g->nv, flags, st, true, total, per_vertex);
+  return xt_guarded([&]() -> int {
+    if (!ctx || !g || !total) return XT_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    XT_HIP(ctx, hipSetDevice(ctx->device));
+    if (per_vertex && !(flags & XT_PER_VERTEX)) return set_err(ctx, XT_ERR_ARG, "per_vertex needs XT_PER_VERTEX");
+    hipStream_t st = ctx->stream;
+    XT_TRY(count_device_csr<int32_t>(ctx, ctx->s, g->nv, g->ne, g->off, g->tails, flags, st, true));
+    return finish_count(ctx, g->nv, flags, st, true, total, per_vertex);
+  });
 }
 
 int xt_graph_count_owned(xt_ctx* ctx, xt_graph* g, int64_t lo, int64_t hi, uint32_t flags, uint64_t* owned,
                          uint64_t* local, uint64_t* total) {
-  if (!ctx || !g || !owned) return XT_ERR_ARG;
-  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
-  XT_HIP(ctx, hipSetDevice(ctx->device));
-  if (lo < 0 || hi < lo || hi > g->nv) return set_err(ctx, XT_ERR_ARG, "owned range must satisfy 0 <= lo <= hi <= nv");
-  if (flags & XT_PER_VERTEX) return set_err(ctx, XT_ERR_ARG, "an owned-range count has no per-vertex output");
-  hipStream_t st = ctx->stream;
-  const OwnedRange range{lo, hi};
-  XT_TRY(count_device_csr<int32_t>(ctx, ctx->s, g->nv, g->ne, g->off, g->tails, flags, st, true, &range));
-  uint64_t t = 0;
-  XT_TRY(finish_count(ctx, g->nv, flags, st, true, &t, nullptr));
-  uint64_t h[2] = {0, 0};
-  XT_HIP(ctx, hipMemcpy(h, ctx->s.counters.as<uint64_t>() + kOwned, sizeof(h), hipMemcpyDeviceToHost));
-  *owned = h[0];
-  if (local) *local = h[1];
-  if (total) *total = t;
-  return XT_OK;
+  return xt_guarded([&]() -> int {
+    if (!ctx || !g || !owned) return XT_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    XT_HIP(ctx, hipSetDevice(ctx->device));
+    if (lo < 0 || hi < lo || hi > g->nv) return set_err(ctx, XT_ERR_ARG, "owned range must satisfy 0 <= lo <= hi <= nv");
+    if (flags & XT_PER_VERTEX) return set_err(ctx, XT_ERR_ARG, "an owned-range count has no per-vertex output");
+    hipStream_t st = ctx->stream;
+    const OwnedRange range{lo, hi};
+    XT_TRY(count_device_csr<int32_t>(ctx, ctx->s, g->nv, g->ne, g->off, g->tails, flags, st, true, &range));
+    uint64_t t = 0;
+    XT_TRY(finish_count(ctx, g->nv, flags, st, true, &t, nullptr));
+    uint64_t h[2] = {0, 0};
+    XT_HIP(ctx, hipMemcpy(h, ctx->s.counters.as<uint64_t>() + kOwned, sizeof(h), hipMemcpyDeviceToHost));
+    *owned = h[0];
+    if (local) *local = h[1];
+    if (total) *total = t;
+    return XT_OK;
+  });
 }
 
 int xt_graph_slice(xt_ctx* ctx, const xt_graph* g, int64_t lo, int64_t hi, xt_graph** out) {
-  if (!ctx || !g || !out) return XT_ERR_ARG;
-  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
-  XT_HIP(ctx, hipSetDevice(ctx->device));
-  *out = nullptr;
-  if (lo < 0 || hi < lo || hi > g->nv) return set_err(ctx, XT_ERR_ARG, "owned range must satisfy 0 <= lo <= hi <= nv");
-  xt_graph* s = new xt_graph();
-  const int rc = graph_slice(ctx, g, lo, hi, s);
-  if (rc != XT_OK) {
-    if (s->off) (void)hipFree(s->off);
-    if (s->tails) (void)hipFree(s->tails);
-    delete s;
-    return rc;
-  }
-  *out = s;
-  return XT_OK;
+  return xt_guarded([&]() -> int {
+    if (!ctx || !g || !out) return XT_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    XT_HIP(ctx, hipSetDevice(ctx->device));
+    *out = nullptr;
+    if (lo < 0 || hi < lo || hi > g->nv) return set_err(ctx, XT_ERR_ARG, "owned range must satisfy 0 <= lo <= hi <= nv");
+    xt_graph* s = new xt_graph();
+    const int rc = graph_slice(ctx, g, lo, hi, s);
+    if (rc != XT_OK) {
+      if (s->off) (void)hipFree(s->off);
+      if (s->tails) (void)hipFree(s->tails);
+      delete s;
+      return rc;
+    }
+    *out = s;
+    return XT_OK;
+  });
 }
 
 int xt_graph_alg_bytes(xt_ctx* ctx, xt_graph* g, uint32_t flags, uint64_t* bytes, uint64_t* sum_min,
                        int64_t* ne_plus) {
-  if (!ctx || !g) return XT_ERR_ARG;
-  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
-  XT_HIP(ctx, hipSetDevice(ctx->device));
-  hipStream_t st = ctx->stream;
-  const int mode = (flags & XT_ORIENT_ID) ? kOrientId : kOrientDegree;
-  XT_TRY(ensure(ctx, ctx->s.counters, kNumSlots * sizeof(uint64_t)));
-  XT_TRY(fill_u64(ctx, ctx->s.counters.as<uint64_t>(), kNumSlots, 0, st));
-  XT_TRY(orient<int32_t>(ctx, ctx->s, g->nv, g->ne, g->off, g->tails, mode, st));
-  XT_TRY(alg_stats(ctx, ctx->s, g->nv, g->ne, st));
-  uint64_t sm = 0;
-  int64_t nep = 0;
-  XT_HIP(ctx, hipMemcpyAsync(&sm, ctx->s.counters.as<uint64_t>() + kStatMin, sizeof(sm),
-                             hipMemcpyDeviceToHost, st));
-  if (g->nv > 0)
-    XT_HIP(ctx, hipMemcpyAsync(&nep, ctx->s.ooff.as<int64_t>() + g->nv, sizeof(nep), hipMemcpyDeviceToHost, st));
-  XT_HIP(ctx, hipStreamSynchronize(st));
-  if (sum_min) *sum_min = sm;
-  if (ne_plus) *ne_plus = nep;
-  if (bytes) *bytes = 8ULL * (uint64_t)(g->nv + 1) + 4ULL * (uint64_t)nep + 4ULL * sm;
-  return XT_OK;
+  return xt_guarded([&]() -> int {
+    if (!ctx || !g) return XT_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    XT_HIP(ctx, hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const int mode = (flags & XT_ORIENT_ID) ? kOrientId : kOrientDegree;
+    XT_TRY(ensure(ctx, ctx->s.counters, kNumSlots * sizeof(uint64_t)));
+    XT_TRY(fill_u64(ctx, ctx->s.counters.as<uint64_t>(), kNumSlots, 0, st));
+    XT_TRY(orient<int32_t>(ctx, ctx->s, g->nv, g->ne, g->off, g->tails, mode, st));
+    XT_TRY(alg_stats(ctx, ctx->s, g->nv, g->ne, st));
+    uint64_t sm = 0;
+    int64_t nep = 0;
+    XT_HIP(ctx, hipMemcpyAsync(&sm, ctx->s.counters.as<uint64_t>() + kStatMin, sizeof(sm),
+                               hipMemcpyDeviceToHost, st));
+    if (g->nv > 0)
+      XT_HIP(ctx, hipMemcpyAsync(&nep, ctx->s.ooff.as<int64_t>() + g->nv, sizeof(nep), hipMemcpyDeviceToHost, st));
+    XT_HIP(ctx, hipStreamSynchronize(st));
+    if (sum_min) *sum_min = sm;
+    if (ne_plus) *ne_plus = nep;
+    if (bytes) *bytes = 8ULL * (uint64_t)(g->nv + 1) + 4ULL * (uint64_t)nep + 4ULL * sm;
+    return XT_OK;
+  });
 }
 
 int xt_graph_stats_get(xt_ctx* ctx, xt_graph* g, uint32_t flags, xt_graph_stats* out) {
-  if (!ctx || !g || !out) return XT_ERR_ARG;
-  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
-  xt_graph_stats st = {};
-  XT_TRY(xt_graph_alg_bytes(ctx, g, flags, &st.alg_bytes, &st.sum_min, &st.ne_plus));
-  uint64_t h[2] = {0, 0};
-  XT_HIP(ctx, hipMemcpy(h, ctx->s.counters.as<uint64_t>() + kStatMinHeavy, sizeof(h),
-                        hipMemcpyDeviceToHost));
-  st.sum_min_heavy = h[0];
-  st.heavy_edges = (int64_t)h[1];
-  st.group_bytes = 4ULL * (st.sum_min_heavy + (uint64_t)st.heavy_edges);
-  *out = st;
-  return XT_OK;
+  return xt_guarded([&]() -> int {
+    if (!ctx || !g || !out) return XT_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    xt_graph_stats st = {};
+    XT_TRY(xt_graph_alg_bytes(ctx, g, flags, &st.alg_bytes, &st.sum_min, &st.ne_plus));
+    uint64_t h[2] = {0, 0};
+    XT_HIP(ctx, hipMemcpy(h, ctx->s.counters.as<uint64_t>() + kStatMinHeavy, sizeof(h),
+                          hipMemcpyDeviceToHost));
+    st.sum_min_heavy = h[0];
+    st.heavy_edges = (int64_t)h[1];
+    st.group_bytes = 4ULL * (st.sum_min_heavy + (uint64_t)st.heavy_edges);
+    *out = st;
+    return XT_OK;
+  });
 }
 
 int xt_graph_destroy(xt_ctx* ctx, xt_graph* g) {
-  if (!g) return XT_ERR_ARG;
-  if (ctx) (void)hipStreamSynchronize(ctx->stream);
-  if (g->off) (void)hipFree(g->off);
-  if (g->tails) (void)hipFree(g->tails);
-  delete g;
-  return XT_OK;
+  return xt_guarded([&]() -> int {
+    if (!g) return XT_ERR_ARG;
+    if (ctx) (void)hipStreamSynchronize(ctx->stream);
+    if (g->off) (void)hipFree(g->off);
+    if (g->tails) (void)hipFree(g->tails);
+    delete g;
+    return XT_OK;
+  });
 }
 
 }  // extern "C"

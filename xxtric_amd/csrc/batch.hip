@@ -439,142 +439,158 @@ extern "C" {
 
 int xt_batch_create(xt_ctx* ctx, int64_t max_vertices, int64_t max_edges, int64_t max_graphs,
                     xt_batch** out) {
-  if (!ctx || !out || max_vertices < 1 || max_edges < 1 || max_graphs < 1) return XT_ERR_ARG;
-  if (max_vertices >= (1LL << 31) - 1) return set_err(ctx, XT_ERR_ARG, "max_vertices must be below 2^31-1");
-  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
-  XT_HIP(ctx, hipSetDevice(ctx->device));
-  *out = nullptr;
-  xt_batch* b = new xt_batch();
-  b->ctx = ctx;
-  b->max_v = max_vertices;
-  b->max_e = max_edges;
-  b->max_g = max_graphs;
-  hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
-  if (e == hipSuccess) e = hipEventCreate(&b->ev0);
-  if (e == hipSuccess) e = hipEventCreate(&b->ev1);
-  if (e == hipSuccess) e = hipMalloc(&b->goff, (size_t)(max_vertices + 1) * sizeof(int64_t));
-  if (e == hipSuccess) e = hipMalloc(&b->gtails, (size_t)max_edges * sizeof(int32_t));
-  if (e == hipSuccess) e = hipMalloc(&b->gid, (size_t)max_vertices * sizeof(int32_t));
-  if (e == hipSuccess) e = hipMalloc(&b->loff, (size_t)(max_vertices + max_graphs) * sizeof(int64_t));
-  if (e == hipSuccess) e = hipMalloc(&b->ltails, (size_t)max_edges * sizeof(int32_t));
-  if (e == hipSuccess) e = hipMalloc(&b->gtotal, (size_t)max_graphs * sizeof(uint64_t));
-  if (e == hipSuccess) e = hipMalloc(&b->ptotal, (size_t)max_graphs * sizeof(uint64_t));
-  if (e != hipSuccess) {
-    int rc = hip_err(ctx, e, "xt_batch_create");
-    batch_free(b);
-    return rc;
-  }
-  int rc = reserve(ctx, b->s, max_vertices, max_edges, false);
-  if (rc == XT_OK) rc = fill_u64(ctx, b->s.counters.as<uint64_t>(), kNumSlots, 0, b->stream);
-  if (rc == XT_OK) {
-    e = hipMemsetAsync(b->goff, 0, sizeof(int64_t), b->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
-    if (e != hipSuccess) rc = hip_err(ctx, e, "xt_batch_create");
-  }
-  if (rc != XT_OK) {
-    batch_free(b);
-    return rc;
-  }
-  *out = b;
-  return XT_OK;
+  return xt_guarded([&]() -> int {
+    if (!ctx || !out || max_vertices < 1 || max_edges < 1 || max_graphs < 1) return XT_ERR_ARG;
+    if (max_vertices >= (1LL << 31) - 1) return set_err(ctx, XT_ERR_ARG, "max_vertices must be below 2^31-1");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    XT_HIP(ctx, hipSetDevice(ctx->device));
+    *out = nullptr;
+    xt_batch* b = new xt_batch();
+    b->ctx = ctx;
+    b->max_v = max_vertices;
+    b->max_e = max_edges;
+    b->max_g = max_graphs;
+    hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&b->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&b->ev1);
+    if (e == hipSuccess) e = hipMalloc(&b->goff, (size_t)(max_vertices + 1) * sizeof(int64_t));
+    if (e == hipSuccess) e = hipMalloc(&b->gtails, (size_t)max_edges * sizeof(int32_t));
+    if (e == hipSuccess) e = hipMalloc(&b->gid, (size_t)max_vertices * sizeof(int32_t));
+    if (e == hipSuccess) e = hipMalloc(&b->loff, (size_t)(max_vertices + max_graphs) * sizeof(int64_t));
+    if (e == hipSuccess) e = hipMalloc(&b->ltails, (size_t)max_edges * sizeof(int32_t));
+    if (e == hipSuccess) e = hipMalloc(&b->gtotal, (size_t)max_graphs * sizeof(uint64_t));
+    if (e == hipSuccess) e = hipMalloc(&b->ptotal, (size_t)max_graphs * sizeof(uint64_t));
+    if (e != hipSuccess) {
+      int rc = hip_err(ctx, e, "xt_batch_create");
+      batch_free(b);
+      return rc;
+    }
+    int rc = reserve(ctx, b->s, max_vertices, max_edges, false);
+    if (rc == XT_OK) rc = fill_u64(ctx, b->s.counters.as<uint64_t>(), kNumSlots, 0, b->stream);
+    if (rc == XT_OK) {
+      e = hipMemsetAsync(b->goff, 0, sizeof(int64_t), b->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+      if (e != hipSuccess) rc = hip_err(ctx, e, "xt_batch_create");
+    }
+    if (rc != XT_OK) {
+      batch_free(b);
+      return rc;
+    }
+    *out = b;
+    return XT_OK;
+  });
 }
 
 int xt_batch_add(xt_batch* b, int64_t nv, int64_t ne, const int64_t* offsets, const int32_t* tails,
                  int64_t* slot) {
-  if (!b) return XT_ERR_ARG;
-  std::lock_guard<std::recursive_mutex> lk(b->ctx->mu);
-  XT_HIP(b->ctx, hipSetDevice(b->ctx->device));
-  if (b->running) return set_err(b->ctx, XT_ERR_STATE, "batch is running");
-  XT_TRY(batch_add_host(b, nv, ne, offsets, tails));
-  if (slot) *slot = (int64_t)b->graphs.size() - 1;
-  return XT_OK;
+  return xt_guarded([&]() -> int {
+    if (!b) return XT_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(b->ctx->mu);
+    XT_HIP(b->ctx, hipSetDevice(b->ctx->device));
+    if (b->running) return set_err(b->ctx, XT_ERR_STATE, "batch is running");
+    XT_TRY(batch_add_host(b, nv, ne, offsets, tails));
+    if (slot) *slot = (int64_t)b->graphs.size() - 1;
+    return XT_OK;
+  });
 }
 
 int xt_batch_add_generated(xt_batch* b, uint64_t base_seed, int64_t first, int64_t count, int64_t nmin,
                            int64_t nmax) {
-  if (!b || first < 0 || count < 0) return XT_ERR_ARG;
-  xt_ctx* ctx = b->ctx;
-  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
-  XT_HIP(ctx, hipSetDevice(ctx->device));
-  if (b->running) return set_err(ctx, XT_ERR_STATE, "batch is running");
-  for (int64_t g = first; g < first + count; g++) {
-    int64_t n;
-    int kind;
-    uint64_t gseed, thr;
-    XT_TRY(xt_batch_params(base_seed, g, nmin, nmax, &n, &kind, &gseed, &thr));
-    if (kind == 0) {
-      // random geometric: host cell-grid generator (reference LCG stream per graph)
-      std::vector<int64_t> off(n + 1);
-      int64_t m = 0;
-      const uint32_t s32 = (uint32_t)(gseed ^ (gseed >> 32));
-      XT_TRY(xt_rgg_generate(n, s32, 0.0, off.data(), nullptr, 0, &m));
-      std::vector<int64_t> t64(m > 0 ? m : 1);
-      XT_TRY(xt_rgg_generate(n, s32, 0.0, off.data(), t64.data(), m, &m));
-      std::vector<int32_t> t32(t64.begin(), t64.begin() + m);
-      XT_TRY(batch_add_host(b, n, m, off.data(), t32.data()));
-    } else {
-      // Erdos-Renyi: generated on the device straight into the local arena,
-      // behind any staged host graphs
-      XT_TRY(batch_flush(b));
-      XT_TRY(batch_check_room(b, n, 0));
-      int64_t m = 0;
-      XT_TRY(gen_er_rows(ctx, b->s, n, gseed + 3, thr, b->loff + b->LO, nullptr, 0, &m, b->stream));
-      XT_TRY(batch_check_room(b, n, m));
-      XT_TRY(gen_er_rows(ctx, b->s, n, gseed + 3, thr, b->loff + b->LO, b->ltails + b->E, m, &m,
-                         b->stream));
-      XT_TRY(batch_commit(b, n, m));
+  return xt_guarded([&]() -> int {
+    if (!b || first < 0 || count < 0) return XT_ERR_ARG;
+    xt_ctx* ctx = b->ctx;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    XT_HIP(ctx, hipSetDevice(ctx->device));
+    if (b->running) return set_err(ctx, XT_ERR_STATE, "batch is running");
+    for (int64_t g = first; g < first + count; g++) {
+      int64_t n;
+      int kind;
+      uint64_t gseed, thr;
+      XT_TRY(xt_batch_params(base_seed, g, nmin, nmax, &n, &kind, &gseed, &thr));
+      if (kind == 0) {
+        // random geometric: host cell-grid generator (reference LCG stream per graph)
+        std::vector<int64_t> off(n + 1);
+        int64_t m = 0;
+        const uint32_t s32 = (uint32_t)(gseed ^ (gseed >> 32));
+        XT_TRY(xt_rgg_generate(n, s32, 0.0, off.data(), nullptr, 0, &m));
+        std::vector<int64_t> t64(m > 0 ? m : 1);
+        XT_TRY(xt_rgg_generate(n, s32, 0.0, off.data(), t64.data(), m, &m));
+        std::vector<int32_t> t32(t64.begin(), t64.begin() + m);
+        XT_TRY(batch_add_host(b, n, m, off.data(), t32.data()));
+      } else {
+        // Erdos-Renyi: generated on the device straight into the local arena,
+        // behind any staged host graphs
+        XT_TRY(batch_flush(b));
+        XT_TRY(batch_check_room(b, n, 0));
+        int64_t m = 0;
+        XT_TRY(gen_er_rows(ctx, b->s, n, gseed + 3, thr, b->loff + b->LO, nullptr, 0, &m, b->stream));
+        XT_TRY(batch_check_room(b, n, m));
+        XT_TRY(gen_er_rows(ctx, b->s, n, gseed + 3, thr, b->loff + b->LO, b->ltails + b->E, m, &m,
+                           b->stream));
+        XT_TRY(batch_commit(b, n, m));
+      }
     }
-  }
-  XT_TRY(batch_flush(b));
-  XT_HIP(ctx, hipStreamSynchronize(b->stream));
-  return XT_OK;
+    XT_TRY(batch_flush(b));
+    XT_HIP(ctx, hipStreamSynchronize(b->stream));
+    return XT_OK;
+  });
 }
 
 int xt_batch_run(xt_batch* b, int mode, uint32_t flags) {
-  if (!b) return XT_ERR_ARG;
-  std::lock_guard<std::recursive_mutex> lk(b->ctx->mu);
-  XT_HIP(b->ctx, hipSetDevice(b->ctx->device));
-  if (flags & XT_PER_VERTEX) return set_err(b->ctx, XT_ERR_ARG, "batches report per-graph totals only");
-  XT_TRY(batch_launch(b, mode, flags));
-  XT_TRY(batch_wait(b));
-  b->timing.n_graphs = (int64_t)b->graphs.size();
-  b->timing.n_vertices = b->V;
-  b->timing.n_edges = b->E;
-  b->timing.n_tiles = (b->E / 2 + kChunk - 1) / kChunk;
-  return XT_OK;
+  return xt_guarded([&]() -> int {
+    if (!b) return XT_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(b->ctx->mu);
+    XT_HIP(b->ctx, hipSetDevice(b->ctx->device));
+    if (flags & XT_PER_VERTEX) return set_err(b->ctx, XT_ERR_ARG, "batches report per-graph totals only");
+    XT_TRY(batch_launch(b, mode, flags));
+    XT_TRY(batch_wait(b));
+    b->timing.n_graphs = (int64_t)b->graphs.size();
+    b->timing.n_vertices = b->V;
+    b->timing.n_edges = b->E;
+    b->timing.n_tiles = (b->E / 2 + kChunk - 1) / kChunk;
+    return XT_OK;
+  });
 }
 
 int xt_batch_results(xt_batch* b, uint64_t* totals, int64_t n) {
-  if (!b || (n > 0 && !totals)) return XT_ERR_ARG;
-  std::lock_guard<std::recursive_mutex> lk(b->ctx->mu);
-  XT_HIP(b->ctx, hipSetDevice(b->ctx->device));
-  XT_TRY(batch_wait(b));
-  return batch_read(b, totals, n);
+  return xt_guarded([&]() -> int {
+    if (!b || (n > 0 && !totals)) return XT_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(b->ctx->mu);
+    XT_HIP(b->ctx, hipSetDevice(b->ctx->device));
+    XT_TRY(batch_wait(b));
+    return batch_read(b, totals, n);
+  });
 }
 
 int xt_batch_timing_get(const xt_batch* b, xt_batch_timing* out) {
-  if (!b || !out) return XT_ERR_ARG;
-  *out = b->timing;
-  return XT_OK;
+  return xt_guarded([&]() -> int {
+    if (!b || !out) return XT_ERR_ARG;
+    *out = b->timing;
+    return XT_OK;
+  });
 }
 
 int xt_batch_clear(xt_batch* b) {
-  if (!b) return XT_ERR_ARG;
-  std::lock_guard<std::recursive_mutex> lk(b->ctx->mu);
-  XT_TRY(batch_wait(b));
-  XT_TRY(batch_stage_idle(b));
-  b->graphs.clear();
-  b->V = b->E = b->LO = 0;
-  b->pend_g = 0;
-  return XT_OK;
+  return xt_guarded([&]() -> int {
+    if (!b) return XT_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(b->ctx->mu);
+    XT_TRY(batch_wait(b));
+    XT_TRY(batch_stage_idle(b));
+    b->graphs.clear();
+    b->V = b->E = b->LO = 0;
+    b->pend_g = 0;
+    return XT_OK;
+  });
 }
 
 int xt_batch_destroy(xt_batch* b) {
-  if (!b) return XT_ERR_ARG;
-  std::lock_guard<std::recursive_mutex> lk(b->ctx->mu);
-  (void)hipSetDevice(b->ctx->device);
-  batch_free(b);
-  return XT_OK;
+  return xt_guarded([&]() -> int {
+    if (!b) return XT_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(b->ctx->mu);
+    (void)hipSetDevice(b->ctx->device);
+    batch_free(b);
+    return XT_OK;
+  });
 }
 
 }  // extern "C"

@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "../../include/xtric.h"
+#include "xt_guard.hpp"
 
 namespace {
 
@@ -89,61 +90,62 @@ double xt_rgg_radius(int64_t nv) {
 
 int xt_rgg_generate(int64_t nv, uint32_t seed, double radius, int64_t* offsets, int64_t* tails,
                     int64_t cap, int64_t* ne_out) {
-  if (nv <= 0 || !offsets || !ne_out) return XT_ERR_ARG;
-  std::vector<double> X, Y;
-  lcg_coords(nv, seed, X, Y);
-  const double r = radius > 0 ? radius : xt_rgg_radius(nv);
-  double minx = X[0], maxx = X[0], miny = Y[0], maxy = Y[0];
-  for (int64_t i = 1; i < nv; i++) {
-    minx = std::min(minx, X[i]);
-    maxx = std::max(maxx, X[i]);
-    miny = std::min(miny, Y[i]);
-    maxy = std::max(maxy, Y[i]);
-  }
-  // Cells of side cs > r: any pair within distance r lies in neighbouring cells.
-  const double cs = r * 1.0000001 + 1e-300;
-  const int64_t cap_cells = 1 << 24;
-  int64_t gx = std::max<int64_t>(1, (int64_t)std::floor((maxx - minx) / cs) + 1);
-  int64_t gy = std::max<int64_t>(1, (int64_t)std::floor((maxy - miny) / cs) + 1);
-  double csx = cs, csy = cs;
-  while (gx * gy > cap_cells) {  // very small radii: coarser cells, still correct
-    csx *= 2;
-    csy *= 2;
-    gx = std::max<int64_t>(1, (int64_t)std::floor((maxx - minx) / csx) + 1);
-    gy = std::max<int64_t>(1, (int64_t)std::floor((maxy - miny) / csy) + 1);
-  }
-  std::vector<int64_t> cell(nv), cstart(gx * gy + 1, 0), order(nv);
-  for (int64_t i = 0; i < nv; i++) {
-    int64_t cx = std::min(gx - 1, (int64_t)std::floor((X[i] - minx) / csx));
-    int64_t cy = std::min(gy - 1, (int64_t)std::floor((Y[i] - miny) / csy));
-    cell[i] = cy * gx + cx;
-    cstart[cell[i] + 1]++;
-  }
-  for (int64_t c = 0; c < gx * gy; c++) cstart[c + 1] += cstart[c];
-  {
-    std::vector<int64_t> cur(cstart.begin(), cstart.end() - 1);
-    for (int64_t i = 0; i < nv; i++) order[cur[cell[i]]++] = i;  // ascending ids within a cell
-  }
-  std::vector<std::vector<int64_t>> rows(nv);
-  parallel_for(nv, [&](int64_t a, int64_t b) {
-    for (int64_t i = a; i < b; i++) {
-      const int64_t cx = cell[i] % gx, cy = cell[i] / gx;
-      auto& row = rows[i];
-      for (int64_t yy = std::max<int64_t>(0, cy - 1); yy <= std::min(gy - 1, cy + 1); yy++)
-        for (int64_t xx = std::max<int64_t>(0, cx - 1); xx <= std::min(gx - 1, cx + 1); xx++) {
-          const int64_t c = yy * gx + xx;
-          for (int64_t k = cstart[c]; k < cstart[c + 1]; k++) {
-            const int64_t j = order[k];
-            if (j == i) continue;
-            const int64_t p = std::min(i, j), q = std::max(i, j);
-            const double dx = X[p] - X[q];
-            const double dy = Y[p] - Y[q];
-            const double ed = std::sqrt(dx * dx + dy * dy);
-            if (ed <= r) row.push_back(j);
-          }
-        }
-      std::sort(row.begin(), row.end());
+  return xt_guarded([&]() -> int {
+    if (nv <= 0 || !offsets || !ne_out) return XT_ERR_ARG;
+    std::vector<double> X, Y;
+    lcg_coords(nv, seed, X, Y);
+    const double r = radius > 0 ? radius : xt_rgg_radius(nv);
+    double minx = X[0], maxx = X[0], miny = Y[0], maxy = Y[0];
+    for (int64_t i = 1; i < nv; i++) {
+      minx = std::min(minx, X[i]);
+      maxx = std::max(maxx, X[i]);
+      miny = std::min(miny, Y[i]);
+      maxy = std::max(maxy, Y[i]);
     }
+    // Cells of side cs > r: any pair within distance r lies in neighbouring cells.
+    const double cs = r * 1.0000001 + 1e-300;
+    const int64_t cap_cells = 1 << 24;
+    int64_t gx = std::max<int64_t>(1, (int64_t)std::floor((maxx - minx) / cs) + 1);
+    int64_t gy = std::max<int64_t>(1, (int64_t)std::floor((maxy - miny) / cs) + 1);
+    double csx = cs, csy = cs;
+    while (gx * gy > cap_cells) {  // very small radii: coarser cells, still correct
+      csx *= 2;
+      csy *= 2;
+      gx = std::max<int64_t>(1, (int64_t)std::floor((maxx - minx) / csx) + 1);
+      gy = std::max<int64_t>(1, (int64_t)std::floor((maxy - miny) / csy) + 1);
+    }
+    std::vector<int64_t> cell(nv), cstart(gx * gy + 1, 0), order(nv);
+    for (int64_t i = 0; i < nv; i++) {
+      int64_t cx = std::min(gx - 1, (int64_t)std::floor((X[i] - minx) / csx));
+      int64_t cy = std::min(gy - 1, (int64_t)std::floor((Y[i] - miny) / csy));
+      cell[i] = cy * gx + cx;
+      cstart[cell[i] + 1]++;
+    }
+    for (int64_t c = 0; c < gx * gy; c++) cstart[c + 1] += cstart[c];
+    {
+      std::vector<int64_t> cur(cstart.begin(), cstart.end() - 1);
+      for (int64_t i = 0; i < nv; i++) order[cur[cell[i]]++] = i;  // ascending ids within a cell
+    }
+    std::vector<std::vector<int64_t>> rows(nv);
+    parallel_for(nv, [&](int64_t a, int64_t b) {
+      for (int64_t i = a; i < b; i++) {
+        const int64_t cx = cell[i] % gx, cy = cell[i] / gx;
+        auto& row = rows[i];
+        for (int64_t yy = std::max<int64_t>(0, cy - 1); yy <= std::min(gy - 1, cy + 1); yy++)
+          for (int64_t xx = std::max<int64_t>(0, cx - 1); xx <= std::min(gx - 1, cx + 1); xx++) {
+            const int64_t c = yy * gx + xx;
+            for (int64_t k = cstart[c]; k < cstart[c + 1]; k++) {
+              const int64_t j = order[k];
+              if (j == i) continue;
+              const int64_t p = std::min(i, j), q = std::max(i, j);
+              const double dx = X[p] - X[q];
+              const double dy = Y[p] - Y[q];
+              const double ed = std::sqrt(dx * dx + dy * dy);
+              if (ed <= r) row.push_back(j);
+            }
+          }
+        std::sort(row.begin(), row.end());
+      }
   });
   offsets[0] = 0;
   for (int64_t i = 0; i < nv; i++) offsets[i + 1] = offsets[i] + (int64_t)rows[i].size();
@@ -151,6 +153,7 @@ int xt_rgg_generate(int64_t nv, uint32_t seed, double radius, int64_t* offsets, 
   if (tails && cap >= offsets[nv])
     for (int64_t i = 0; i < nv; i++) std::copy(rows[i].begin(), rows[i].end(), tails + offsets[i]);
   return XT_OK;
+  });
 }
 
 }  // extern "C"
@@ -237,113 +240,122 @@ extern "C" {
 
 int xt_vite_read(const char* path, int layout32, int64_t* nv, int64_t* ne, int64_t* offsets,
                  int64_t* tails) {
-  if (!path || !nv || !ne) return XT_ERR_ARG;
-  return layout32 ? vite_read_t<int32_t, float>(path, nv, ne, offsets, tails)
-                  : vite_read_t<int64_t, double>(path, nv, ne, offsets, tails);
+  return xt_guarded([&]() -> int {
+    if (!path || !nv || !ne) return XT_ERR_ARG;
+    return layout32 ? vite_read_t<int32_t, float>(path, nv, ne, offsets, tails)
+                    : vite_read_t<int64_t, double>(path, nv, ne, offsets, tails);
+  });
 }
 
 int xt_vite_write(const char* path, int layout32, int64_t nv, const int64_t* offsets,
                   const int64_t* tails) {
-  if (!path || nv < 0 || !offsets || (offsets[nv] > 0 && !tails)) return XT_ERR_ARG;
-  return layout32 ? vite_write_t<int32_t, float>(path, nv, offsets, tails)
-                  : vite_write_t<int64_t, double>(path, nv, offsets, tails);
+  return xt_guarded([&]() -> int {
+    if (!path || nv < 0 || !offsets || (offsets[nv] > 0 && !tails)) return XT_ERR_ARG;
+    return layout32 ? vite_write_t<int32_t, float>(path, nv, offsets, tails)
+                    : vite_write_t<int64_t, double>(path, nv, offsets, tails);
+  });
 }
 
 // ---- whitespace edge list -> symmetric CSR ----------------------------------
 int xt_edgelist_to_csr(const char* path, int64_t* nv_out, int64_t* ne_out, int64_t* offsets,
                        int64_t* tails) {
-  if (!path || !nv_out || !ne_out) return XT_ERR_ARG;
-  FILE* f = std::fopen(path, "r");
-  if (!f) return XT_ERR_ARG;
-  std::vector<uint64_t> keys;
-  std::vector<std::pair<int64_t, int64_t>> ed;
-  char line[512];
-  int64_t maxid = -1;
-  while (std::fgets(line, sizeof(line), f)) {
-    if (line[0] == '#' || line[0] == '%') continue;
-    long long a, b;
-    if (std::sscanf(line, "%lld %lld", &a, &b) != 2) continue;
-    if (a < 0 || b < 0) {
-      std::fclose(f);
-      return XT_ERR_INPUT;
+  return xt_guarded([&]() -> int {
+    if (!path || !nv_out || !ne_out) return XT_ERR_ARG;
+    FILE* f = std::fopen(path, "r");
+    if (!f) return XT_ERR_ARG;
+    std::vector<uint64_t> keys;
+    std::vector<std::pair<int64_t, int64_t>> ed;
+    char line[512];
+    int64_t maxid = -1;
+    while (std::fgets(line, sizeof(line), f)) {
+      if (line[0] == '#' || line[0] == '%') continue;
+      long long a, b;
+      if (std::sscanf(line, "%lld %lld", &a, &b) != 2) continue;
+      if (a < 0 || b < 0) {
+        std::fclose(f);
+        return XT_ERR_INPUT;
+      }
+      ed.emplace_back(a, b);
+      maxid = std::max<int64_t>(maxid, std::max<int64_t>(a, b));
     }
-    ed.emplace_back(a, b);
-    maxid = std::max<int64_t>(maxid, std::max<int64_t>(a, b));
-  }
-  std::fclose(f);
-  const int64_t nv = maxid + 1;
-  std::vector<std::pair<int64_t, int64_t>> all;
-  all.reserve(ed.size() * 2);
-  for (auto& p : ed) {
-    all.push_back(p);
-    all.emplace_back(p.second, p.first);
-  }
-  std::sort(all.begin(), all.end());
-  all.erase(std::unique(all.begin(), all.end()), all.end());
-  *nv_out = nv;
-  *ne_out = (int64_t)all.size();
-  if (offsets) {
-    std::fill(offsets, offsets + nv + 1, 0);
-    for (auto& p : all) offsets[p.first + 1]++;
-    for (int64_t i = 0; i < nv; i++) offsets[i + 1] += offsets[i];
-  }
-  if (tails)
-    for (size_t e = 0; e < all.size(); e++) tails[e] = all[e].second;
-  return XT_OK;
+    std::fclose(f);
+    const int64_t nv = maxid + 1;
+    std::vector<std::pair<int64_t, int64_t>> all;
+    all.reserve(ed.size() * 2);
+    for (auto& p : ed) {
+      all.push_back(p);
+      all.emplace_back(p.second, p.first);
+    }
+    std::sort(all.begin(), all.end());
+    all.erase(std::unique(all.begin(), all.end()), all.end());
+    *nv_out = nv;
+    *ne_out = (int64_t)all.size();
+    if (offsets) {
+      std::fill(offsets, offsets + nv + 1, 0);
+      for (auto& p : all) offsets[p.first + 1]++;
+      for (int64_t i = 0; i < nv; i++) offsets[i + 1] += offsets[i];
+    }
+    if (tails)
+      for (size_t e = 0; e < all.size(); e++) tails[e] = all[e].second;
+    return XT_OK;
+  });
 }
 
 // ---- batch mix -----------------------------------------------------------
 int xt_batch_params(uint64_t base_seed, int64_t g, int64_t nmin, int64_t nmax, int64_t* n, int* kind,
                     uint64_t* gseed, uint64_t* er_thr64) {
-  if (nmin < 2 || nmax < nmin || g < 0) return XT_ERR_ARG;
-  const uint64_t gk = mix64(mix64(base_seed) + (uint64_t)g);
-  const double u1 = (double)(mix64(gk + 1) >> 11) * 0x1.0p-53;
-  const double ln0 = std::log((double)nmin), ln1 = std::log((double)nmax);
-  int64_t nn = (int64_t)std::llround(std::exp(ln0 + u1 * (ln1 - ln0)));
-  nn = std::min(std::max(nn, nmin), nmax);
-  const double u2 = (double)(mix64(gk + 2) >> 11) * 0x1.0p-53;
-  const double deg = 4.0 + 60.0 * u2;
-  const double p = deg / (double)(nn - 1);
-  if (n) *n = nn;
-  if (kind) *kind = (int)(g & 1);
-  if (gseed) *gseed = gk;
-  if (er_thr64) *er_thr64 = p >= 1.0 ? (1ULL << 32) : (uint64_t)std::floor(p * 4294967296.0);
-  return XT_OK;
+  return xt_guarded([&]() -> int {
+    if (nmin < 2 || nmax < nmin || g < 0) return XT_ERR_ARG;
+    const uint64_t gk = mix64(mix64(base_seed) + (uint64_t)g);
+    const double u1 = (double)(mix64(gk + 1) >> 11) * 0x1.0p-53;
+    const double ln0 = std::log((double)nmin), ln1 = std::log((double)nmax);
+    int64_t nn = (int64_t)std::llround(std::exp(ln0 + u1 * (ln1 - ln0)));
+    nn = std::min(std::max(nn, nmin), nmax);
+    const double u2 = (double)(mix64(gk + 2) >> 11) * 0x1.0p-53;
+    const double deg = 4.0 + 60.0 * u2;
+    const double p = deg / (double)(nn - 1);
+    if (n) *n = nn;
+    if (kind) *kind = (int)(g & 1);
+    if (gseed) *gseed = gk;
+    if (er_thr64) *er_thr64 = p >= 1.0 ? (1ULL << 32) : (uint64_t)std::floor(p * 4294967296.0);
+    return XT_OK;
+  });
 }
 
 int xt_batch_graph_csr(uint64_t base_seed, int64_t g, int64_t nmin, int64_t nmax, int64_t* nv,
                        int64_t* ne, int64_t* offsets, int32_t* tails) {
-  int64_t n;
-  int kind;
-  uint64_t gseed, thr;
-  int rc = xt_batch_params(base_seed, g, nmin, nmax, &n, &kind, &gseed, &thr);
-  if (rc != XT_OK) return rc;
-  if (nv) *nv = n;
-  if (kind == 0) {
-    std::vector<int64_t> off(n + 1);
-    int64_t m = 0;
-    const uint32_t s = (uint32_t)(gseed ^ (gseed >> 32));
-    rc = xt_rgg_generate(n, s, 0.0, off.data(), nullptr, 0, &m);
+  return xt_guarded([&]() -> int {
+    int64_t n;
+    int kind;
+    uint64_t gseed, thr;
+    int rc = xt_batch_params(base_seed, g, nmin, nmax, &n, &kind, &gseed, &thr);
     if (rc != XT_OK) return rc;
-    if (ne) *ne = m;
-    if (offsets) std::copy(off.begin(), off.end(), offsets);
-    if (tails) {
-      std::vector<int64_t> t(m > 0 ? m : 1);
-      rc = xt_rgg_generate(n, s, 0.0, off.data(), t.data(), m, &m);
-      for (int64_t e = 0; e < m; e++) tails[e] = (int32_t)t[e];
-    }
-    return rc;
-  }
-  // Erdos-Renyi: edge {i<j} iff (mix64(mix64(gseed+3) + i*n + j) >> 32) < thr.
-  const uint64_t key = mix64(gseed + 3);
-  std::vector<std::vector<int32_t>> rows(n);
-  parallel_for(n, [&](int64_t a, int64_t b) {
-    for (int64_t i = a; i < b; i++)
-      for (int64_t j = 0; j < n; j++) {
-        if (j == i) continue;
-        const uint64_t p = (uint64_t)std::min(i, j), q = (uint64_t)std::max(i, j);
-        if ((mix64(key + p * (uint64_t)n + q) >> 32) < thr) rows[i].push_back((int32_t)j);
+    if (nv) *nv = n;
+    if (kind == 0) {
+      std::vector<int64_t> off(n + 1);
+      int64_t m = 0;
+      const uint32_t s = (uint32_t)(gseed ^ (gseed >> 32));
+      rc = xt_rgg_generate(n, s, 0.0, off.data(), nullptr, 0, &m);
+      if (rc != XT_OK) return rc;
+      if (ne) *ne = m;
+      if (offsets) std::copy(off.begin(), off.end(), offsets);
+      if (tails) {
+        std::vector<int64_t> t(m > 0 ? m : 1);
+        rc = xt_rgg_generate(n, s, 0.0, off.data(), t.data(), m, &m);
+        for (int64_t e = 0; e < m; e++) tails[e] = (int32_t)t[e];
       }
+      return rc;
+    }
+    // Erdos-Renyi: edge {i<j} iff (mix64(mix64(gseed+3) + i*n + j) >> 32) < thr.
+    const uint64_t key = mix64(gseed + 3);
+    std::vector<std::vector<int32_t>> rows(n);
+    parallel_for(n, [&](int64_t a, int64_t b) {
+      for (int64_t i = a; i < b; i++)
+        for (int64_t j = 0; j < n; j++) {
+          if (j == i) continue;
+          const uint64_t p = (uint64_t)std::min(i, j), q = (uint64_t)std::max(i, j);
+          if ((mix64(key + p * (uint64_t)n + q) >> 32) < thr) rows[i].push_back((int32_t)j);
+        }
   });
   int64_t m = 0;
   for (auto& r : rows) m += (int64_t)r.size();
@@ -355,6 +367,7 @@ int xt_batch_graph_csr(uint64_t base_seed, int64_t g, int64_t nmin, int64_t nmax
   if (tails && offsets)
     for (int64_t i = 0; i < n; i++) std::copy(rows[i].begin(), rows[i].end(), tails + offsets[i]);
   return XT_OK;
+  });
 }
 
 }  // extern "C"

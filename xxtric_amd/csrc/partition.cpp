@@ -30,6 +30,7 @@
 #include <vector>
 
 #include "../../include/xtric.h"
+#include "xt_guard.hpp"
 
 namespace {
 
@@ -72,7 +73,7 @@ int partition_linear(int64_t n, const int64_t* seq, int64_t k, int64_t* bins) {
     for (int64_t i = 1; i < n; i++) {
       const int64_t Si = S[(size_t)i];
       // best[x][j-1] + S[x] is nondecreasing: the pointer only moves forward.
-      while (c < i && prev[(size_t)c] + S[(size_t)c] < Si) c++;
+      while (c < i && (__int128)prev[(size_t)c] + S[(size_t)c] < Si) c++;  // no overflow near 2^63
       int64_t val, at;
       if (c == 0) {
         val = prev[0];
@@ -112,17 +113,21 @@ extern "C" {
 
 int xt_partition_vertices(int64_t nv, const int64_t* offsets, int64_t nparts, int method,
                           int64_t* bins) {
-  if (nv < 0 || !offsets || nparts < 1 || !bins) return XT_ERR_ARG;
-  if (nv >= (int64_t)INT32_MAX) return XT_ERR_ARG;
-  if (offsets[0] < 0) return XT_ERR_INPUT;
-  for (int64_t v = 0; v < nv; v++)
-    if (offsets[v + 1] < offsets[v]) return XT_ERR_INPUT;
-  if (method == XT_PARTITION_GREEDY) return partition_greedy(nv, offsets, nparts, bins);
-  if (method != XT_PARTITION_LINEAR) return XT_ERR_ARG;
-  // nparts > nv - 1: the reference's bins are not a partition (it fills them
-  // with ones and then adds the table walk on top).
-  if (nv < 1 || nparts > nv - 1) return XT_ERR_ARG;
-  return partition_linear(nv, offsets, nparts, bins);
+  return xt_guarded([&]() -> int {
+    if (nv < 0 || !offsets || nparts < 1 || !bins) return XT_ERR_ARG;
+    if (nv >= (int64_t)INT32_MAX) return XT_ERR_ARG;
+    if (offsets[0] < 0) return XT_ERR_INPUT;
+    for (int64_t v = 0; v < nv; v++)
+      if (offsets[v + 1] < offsets[v]) return XT_ERR_INPUT;
+    if (method == XT_PARTITION_GREEDY) return partition_greedy(nv, offsets, nparts, bins);
+    if (method != XT_PARTITION_LINEAR) return XT_ERR_ARG;
+    // nparts > nv - 1: the reference's bins are not a partition (it fills them
+    // with ones and then adds the table walk on top).
+    if (nv < 1 || nparts > nv - 1) return XT_ERR_ARG;
+    // Its tables take 4 (nv - 1)(nparts - 1) + 28 nv bytes of host memory
+    // (XT_ERR_NOMEM from the guard when they cannot be allocated).
+    return partition_linear(nv, offsets, nparts, bins);
+  });
 }
 
 }  // extern "C"

@@ -344,190 +344,204 @@ void worker_main(xt_server* s) {
 extern "C" {
 
 int xt_serve_create(xt_ctx* ctx, const xt_serve_config* cfg, xt_server** out) {
-  if (!ctx || !out) return XT_ERR_ARG;
-  *out = nullptr;
-  xt_server* s = new xt_server();
-  s->ctx = ctx;
-  if (cfg) s->cfg = *cfg;
-  if (s->cfg.hbm_budget_bytes <= 0) s->cfg.hbm_budget_bytes = 8LL << 30;
-  if (s->cfg.max_batch_graphs <= 0) s->cfg.max_batch_graphs = 4096;
-  if (s->cfg.max_inflight <= 0) s->cfg.max_inflight = 2;
-  if (s->cfg.max_queue <= 0) s->cfg.max_queue = 1 << 20;
-  if (s->cfg.max_wait_us < 0) s->cfg.max_wait_us = 0;
-  // Each slot's arena gets an equal share of the budget.
-  const int64_t share = s->cfg.hbm_budget_bytes / s->cfg.max_inflight;
-  s->slot_e = std::max<int64_t>(1024, share / (2 * kBytesPerEntry + kBytesPerVertex / 2));
-  s->slot_v = std::max<int64_t>(256, s->slot_e / 4);
-  s->slot_g = s->cfg.max_batch_graphs;
-  s->slots.resize(s->cfg.max_inflight);
-  for (ServeSlot& sl : s->slots) {
-    int rc = xt_batch_create(ctx, s->slot_v, s->slot_e, s->slot_g, &sl.b);
-    if (rc == XT_OK) rc = batch_stage_reserve(sl.b);  // pinned staging now, not on the first request
-    if (rc != XT_OK) {
-      for (ServeSlot& o : s->slots)
-        if (o.b) xt_batch_destroy(o.b);
-      delete s;
-      return rc;
+  return xt_guarded([&]() -> int {
+    if (!ctx || !out) return XT_ERR_ARG;
+    *out = nullptr;
+    xt_server* s = new xt_server();
+    s->ctx = ctx;
+    if (cfg) s->cfg = *cfg;
+    if (s->cfg.hbm_budget_bytes <= 0) s->cfg.hbm_budget_bytes = 8LL << 30;
+    if (s->cfg.max_batch_graphs <= 0) s->cfg.max_batch_graphs = 4096;
+    if (s->cfg.max_inflight <= 0) s->cfg.max_inflight = 2;
+    if (s->cfg.max_queue <= 0) s->cfg.max_queue = 1 << 20;
+    if (s->cfg.max_wait_us < 0) s->cfg.max_wait_us = 0;
+    // Each slot's arena gets an equal share of the budget.
+    const int64_t share = s->cfg.hbm_budget_bytes / s->cfg.max_inflight;
+    s->slot_e = std::max<int64_t>(1024, share / (2 * kBytesPerEntry + kBytesPerVertex / 2));
+    s->slot_v = std::max<int64_t>(256, s->slot_e / 4);
+    s->slot_g = s->cfg.max_batch_graphs;
+    s->slots.resize(s->cfg.max_inflight);
+    for (ServeSlot& sl : s->slots) {
+      int rc = xt_batch_create(ctx, s->slot_v, s->slot_e, s->slot_g, &sl.b);
+      if (rc == XT_OK) rc = batch_stage_reserve(sl.b);  // pinned staging now, not on the first request
+      if (rc != XT_OK) {
+        for (ServeSlot& o : s->slots)
+          if (o.b) xt_batch_destroy(o.b);
+        delete s;
+        return rc;
+      }
     }
-  }
-  const unsigned hc = std::thread::hardware_concurrency();
-  s->pool.reset(new CopyPool((int)std::min(7u, std::max(1u, hc / 2))));
-  s->t0 = std::chrono::steady_clock::now();
-  s->worker = std::thread(worker_main, s);
-  *out = s;
-  return XT_OK;
+    const unsigned hc = std::thread::hardware_concurrency();
+    s->pool.reset(new CopyPool((int)std::min(7u, std::max(1u, hc / 2))));
+    s->t0 = std::chrono::steady_clock::now();
+    s->worker = std::thread(worker_main, s);
+    *out = s;
+    return XT_OK;
+  });
 }
 
 int xt_serve_submit(xt_server* s, int64_t nv, int64_t ne, const int64_t* offsets, const int32_t* tails,
                     double arrival_us, uint64_t* ticket) {
-  if (!s || !ticket || nv < 1 || ne < 0 || !offsets || (ne > 0 && !tails)) return XT_ERR_ARG;
-  const int64_t fp = kBytesPerVertex * nv + kBytesPerEntry * ne;
-  auto reject = [&](int code, const char* msg) {
-    {
-      std::lock_guard<std::mutex> lk(s->mu);
-      s->submitted++;
-      s->rejected++;
+  return xt_guarded([&]() -> int {
+    if (!s || !ticket || nv < 1 || ne < 0 || !offsets || (ne > 0 && !tails)) return XT_ERR_ARG;
+    const int64_t fp = kBytesPerVertex * nv + kBytesPerEntry * ne;
+    auto reject = [&](int code, const char* msg) {
+      {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->submitted++;
+        s->rejected++;
+      }
+      return set_err(s->ctx, code, msg);  // outside the server lock
+    };
+    if (nv > s->slot_v || ne > s->slot_e || fp > s->cfg.hbm_budget_bytes)
+      return reject(XT_ERR_CAPACITY, "request rejected: larger than a batch slot, the HBM budget or the queue");
+    // Checked and copied in one pass, in the caller's thread, outside the server lock.
+    Req* r = new Req();
+    r->nv = nv;
+    r->ne = ne;
+    r->off.reset(new int64_t[nv + 1]);
+    r->tails.reset(new int32_t[ne > 0 ? ne : 1]);
+    if (const char* bad = check_host_graph(nv, ne, offsets, tails, r->off.get(), r->tails.get())) {
+      delete r;
+      return reject(XT_ERR_INPUT, bad);
     }
-    return set_err(s->ctx, code, msg);  // outside the server lock
-  };
-  if (nv > s->slot_v || ne > s->slot_e || fp > s->cfg.hbm_budget_bytes)
-    return reject(XT_ERR_CAPACITY, "request rejected: larger than a batch slot, the HBM budget or the queue");
-  // Checked and copied in one pass, in the caller's thread, outside the server lock.
-  Req* r = new Req();
-  r->nv = nv;
-  r->ne = ne;
-  r->off.reset(new int64_t[nv + 1]);
-  r->tails.reset(new int32_t[ne > 0 ? ne : 1]);
-  if (const char* bad = check_host_graph(nv, ne, offsets, tails, r->off.get(), r->tails.get())) {
-    delete r;
-    return reject(XT_ERR_INPUT, bad);
-  }
-  std::unique_lock<std::mutex> lk(s->mu);
-  if ((int64_t)s->queue.size() >= s->cfg.max_queue) {
-    lk.unlock();
-    delete r;
-    return reject(XT_ERR_CAPACITY, "request rejected: larger than a batch slot, the HBM budget or the queue");
-  }
-  s->submitted++;
-  r->ticket = s->next_ticket++;
-  r->arrival_us = arrival_us >= 0 ? arrival_us : s->now();
-  r->footprint = fp;
-  if (s->first_arrival < 0 || r->arrival_us < s->first_arrival) s->first_arrival = r->arrival_us;
-  s->live[r->ticket] = r;
-  s->queue.push_back(r);
-  *ticket = r->ticket;
-  s->cv.notify_all();
-  return XT_OK;
+    std::unique_lock<std::mutex> lk(s->mu);
+    if ((int64_t)s->queue.size() >= s->cfg.max_queue) {
+      lk.unlock();
+      delete r;
+      return reject(XT_ERR_CAPACITY, "request rejected: larger than a batch slot, the HBM budget or the queue");
+    }
+    s->submitted++;
+    r->ticket = s->next_ticket++;
+    r->arrival_us = arrival_us >= 0 ? arrival_us : s->now();
+    r->footprint = fp;
+    if (s->first_arrival < 0 || r->arrival_us < s->first_arrival) s->first_arrival = r->arrival_us;
+    s->live[r->ticket] = r;
+    s->queue.push_back(r);
+    *ticket = r->ticket;
+    s->cv.notify_all();
+    return XT_OK;
+  });
 }
 
 int xt_serve_submit_borrowed(xt_server* s, int64_t nv, int64_t ne, const int64_t* offsets,
                              const int32_t* tails, double arrival_us, uint64_t* ticket) {
-  if (!s || !ticket || nv < 1 || ne < 0 || !offsets || (ne > 0 && !tails)) return XT_ERR_ARG;
-  const int64_t fp = kBytesPerVertex * nv + kBytesPerEntry * ne;
-  std::unique_lock<std::mutex> lk(s->mu);
-  s->submitted++;
-  if (nv > s->slot_v || ne > s->slot_e || fp > s->cfg.hbm_budget_bytes ||
-      (int64_t)s->queue.size() >= s->cfg.max_queue) {
-    s->rejected++;
-    lk.unlock();
-    return set_err(s->ctx, XT_ERR_CAPACITY,
-                   "request rejected: larger than a batch slot, the HBM budget or the queue");
-  }
-  Req* r = new Req();
-  r->nv = nv;
-  r->ne = ne;
-  r->b_off = offsets;
-  r->b_tails = tails;
-  r->ticket = s->next_ticket++;
-  r->arrival_us = arrival_us >= 0 ? arrival_us : s->now();
-  r->footprint = fp;
-  if (s->first_arrival < 0 || r->arrival_us < s->first_arrival) s->first_arrival = r->arrival_us;
-  s->live[r->ticket] = r;
-  s->queue.push_back(r);
-  *ticket = r->ticket;
-  s->cv.notify_all();
-  return XT_OK;
+  return xt_guarded([&]() -> int {
+    if (!s || !ticket || nv < 1 || ne < 0 || !offsets || (ne > 0 && !tails)) return XT_ERR_ARG;
+    const int64_t fp = kBytesPerVertex * nv + kBytesPerEntry * ne;
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->submitted++;
+    if (nv > s->slot_v || ne > s->slot_e || fp > s->cfg.hbm_budget_bytes ||
+        (int64_t)s->queue.size() >= s->cfg.max_queue) {
+      s->rejected++;
+      lk.unlock();
+      return set_err(s->ctx, XT_ERR_CAPACITY,
+                     "request rejected: larger than a batch slot, the HBM budget or the queue");
+    }
+    Req* r = new Req();
+    r->nv = nv;
+    r->ne = ne;
+    r->b_off = offsets;
+    r->b_tails = tails;
+    r->ticket = s->next_ticket++;
+    r->arrival_us = arrival_us >= 0 ? arrival_us : s->now();
+    r->footprint = fp;
+    if (s->first_arrival < 0 || r->arrival_us < s->first_arrival) s->first_arrival = r->arrival_us;
+    s->live[r->ticket] = r;
+    s->queue.push_back(r);
+    *ticket = r->ticket;
+    s->cv.notify_all();
+    return XT_OK;
+  });
 }
 
 int xt_serve_poll(xt_server* s, uint64_t ticket, uint64_t* total, double* latency_us) {
-  if (!s) return XT_ERR_ARG;
-  std::lock_guard<std::mutex> lk(s->mu);
-  if (s->worker_err != XT_OK) return s->worker_err;
-  auto it = s->live.find(ticket);
-  if (it == s->live.end()) return XT_ERR_NOTFOUND;
-  Req* r = it->second;
-  if (r->state != kDone) return XT_PENDING;
-  if (r->err != XT_OK) {  // a borrowed request that failed its check
-    const int err = r->err;
+  return xt_guarded([&]() -> int {
+    if (!s) return XT_ERR_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->worker_err != XT_OK) return s->worker_err;
+    auto it = s->live.find(ticket);
+    if (it == s->live.end()) return XT_ERR_NOTFOUND;
+    Req* r = it->second;
+    if (r->state != kDone) return XT_PENDING;
+    if (r->err != XT_OK) {  // a borrowed request that failed its check
+      const int err = r->err;
+      s->live.erase(it);
+      delete r;
+      return set_err(s->ctx, err, "request rejected: adjacency out of range or offsets not monotone");
+    }
+    if (total) *total = r->total;
+    if (latency_us) *latency_us = r->done_us - r->arrival_us;
     s->live.erase(it);
     delete r;
-    return set_err(s->ctx, err, "request rejected: adjacency out of range or offsets not monotone");
-  }
-  if (total) *total = r->total;
-  if (latency_us) *latency_us = r->done_us - r->arrival_us;
-  s->live.erase(it);
-  delete r;
-  return XT_OK;
+    return XT_OK;
+  });
 }
 
 int xt_serve_drain(xt_server* s, double timeout_s) {
-  if (!s) return XT_ERR_ARG;
-  std::unique_lock<std::mutex> lk(s->mu);
-  const bool ok = s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), [&] {
-    return s->completed + s->rejected >= s->submitted || s->worker_err != XT_OK;
+  return xt_guarded([&]() -> int {
+    if (!s) return XT_ERR_ARG;
+    std::unique_lock<std::mutex> lk(s->mu);
+    const bool ok = s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), [&] {
+      return s->completed + s->rejected >= s->submitted || s->worker_err != XT_OK;
   });
   if (s->worker_err != XT_OK) return s->worker_err;
   return ok ? XT_OK : XT_ERR_STATE;
+  });
 }
 
 int xt_serve_stats_get(xt_server* s, xt_serve_stats* out) {
-  if (!s || !out) return XT_ERR_ARG;
-  std::lock_guard<std::mutex> lk(s->mu);
-  xt_serve_stats st = {};
-  st.submitted = s->submitted;
-  st.completed = s->completed;
-  st.rejected = s->rejected;
-  st.batches = s->batches;
-  st.peak_resident_bytes = s->peak_resident;
-  if (!s->latencies.empty()) {
-    std::vector<double> v = s->latencies;
-    std::sort(v.begin(), v.end());
-    auto q = [&](double p) {
-      size_t k = (size_t)std::ceil(p * (double)v.size()) - 1;
-      return v[std::min(k, v.size() - 1)];
-    };
-    st.p50_us = q(0.50);
-    st.p99_us = q(0.99);
-    double sum = 0;
-    for (double x : v) sum += x;
-    st.mean_us = sum / (double)v.size();
-    st.max_us = v.back();
-    const double span = s->last_done - s->first_arrival;
-    st.graphs_per_s = span > 0 ? (double)s->completed / (span * 1e-6) : 0.0;
-  }
-  *out = st;
-  return XT_OK;
+  return xt_guarded([&]() -> int {
+    if (!s || !out) return XT_ERR_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    xt_serve_stats st = {};
+    st.submitted = s->submitted;
+    st.completed = s->completed;
+    st.rejected = s->rejected;
+    st.batches = s->batches;
+    st.peak_resident_bytes = s->peak_resident;
+    if (!s->latencies.empty()) {
+      std::vector<double> v = s->latencies;
+      std::sort(v.begin(), v.end());
+      auto q = [&](double p) {
+        size_t k = (size_t)std::ceil(p * (double)v.size()) - 1;
+        return v[std::min(k, v.size() - 1)];
+      };
+      st.p50_us = q(0.50);
+      st.p99_us = q(0.99);
+      double sum = 0;
+      for (double x : v) sum += x;
+      st.mean_us = sum / (double)v.size();
+      st.max_us = v.back();
+      const double span = s->last_done - s->first_arrival;
+      st.graphs_per_s = span > 0 ? (double)s->completed / (span * 1e-6) : 0.0;
+    }
+    *out = st;
+    return XT_OK;
+  });
 }
 
 double xt_serve_now_us(const xt_server* s) { return s ? s->now() : 0.0; }
 
 int xt_serve_destroy(xt_server* s) {
-  if (!s) return XT_ERR_ARG;
-  {
-    std::lock_guard<std::mutex> lk(s->mu);
-    s->stop = true;
-    s->cv.notify_all();
-  }
-  if (s->worker.joinable()) s->worker.join();
-  for (ServeSlot& sl : s->slots) {
-    if (sl.b) {
-      (void)batch_wait(sl.b);
-      xt_batch_destroy(sl.b);
+  return xt_guarded([&]() -> int {
+    if (!s) return XT_ERR_ARG;
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      s->stop = true;
+      s->cv.notify_all();
     }
-  }
-  for (auto& kv : s->live) delete kv.second;
-  delete s;
-  return XT_OK;
+    if (s->worker.joinable()) s->worker.join();
+    for (ServeSlot& sl : s->slots) {
+      if (sl.b) {
+        (void)batch_wait(sl.b);
+        xt_batch_destroy(sl.b);
+      }
+    }
+    for (auto& kv : s->live) delete kv.second;
+    delete s;
+    return XT_OK;
+  });
 }
 
 }  // extern "C"

@@ -8,6 +8,7 @@
 #include <string>
 
 #include "../../include/xtric.h"
+#include "xt_guard.hpp"
 
 namespace xt {
 
