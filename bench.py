@@ -88,6 +88,34 @@ def rank0_slice(ctx, g, triangles, parts=8, steps=3):
     return out
 
 
+def small_configs(ctx, steps=5):
+    """The reference's CPU-runnable RGG (n = 100,000, LCG seed, reference radius)
+    and R-MAT scale 22: device count time (median of `steps`, graph resident)."""
+    import xxtric_amd
+    out = {}
+    off, tails = xxtric_amd.rgg_generate(100000)
+    g = ctx.graph(off, tails)
+    tot, pv = g.count(per_vertex=True)
+    ms = []
+    for _ in range(steps):
+        g.count()
+        ms.append(ctx.timing()["total_ms"])
+    out["rgg_1e5"] = {"nv": g.nv, "directed_entries": g.ne, "triangles": tot,
+                      "per_vertex_sum_equals_total": int(pv.sum(dtype=np.uint64)) == tot,
+                      "ms": round(float(np.median(ms)), 3)}
+    g.close()
+    g = ctx.rmat(22, 16, 1, True)
+    ms = []
+    for _ in range(steps + 1):
+        tot = g.count()
+        ms.append(ctx.timing()["total_ms"])
+    med = float(np.median(ms[1:]))
+    out["rmat_s22"] = {"nv": g.nv, "directed_entries": g.ne, "triangles": tot, "ms": round(med, 3),
+                       "edges_per_s": round((g.ne // 2) / (med * 1e-3), 1)}
+    g.close()
+    return out
+
+
 def per_vertex_count(ctx, g, triangles, steps=3):
     """The same count with per-vertex output (each triangle credited to its
     smallest vertex, uint64 per vertex in HBM, then copied to the host)."""
@@ -309,6 +337,10 @@ def main():
             line["rank0_of_8"] = {"error": str(e)}
     g.close()
     if not args.no_extras:
+        try:
+            line["small_configs"] = small_configs(ctx)
+        except Exception as e:
+            line["small_configs"] = {"error": str(e)}
         try:
             line["batch_4096"] = batch_workload(ctx)
         except Exception as e:  # reported, not hidden
