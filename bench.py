@@ -293,7 +293,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the single-device path only "
-              "(multi-GPU sharding is described in DESIGN.md, not implemented)", file=sys.stderr)
+              "(multi-GPU sharding is a design, not implemented)", file=sys.stderr)
         sys.exit(2)
     import xxtric_amd
     ctx = xxtric_amd.Context(0)
