@@ -882,44 +882,51 @@ __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(W
       }
     };
     // Per-vertex probe: membership of the ids of one chunk of item k (r of its
-    // four ids lie inside the list) in the LDS hash, four bucket reads and the
-    // slot of each hit.  Only tables with a spilled bucket take the (uniform)
-    // branch to the rare continuations.  Each triangle {x, y, w} is credited to
-    // its smallest vertex without branches: x (a register count), y (the
+    // four ids lie inside the list) in the LDS hash: four bucket reads and raw
+    // compares, as in the plain probe; the slot index is formed from the
+    // compares only for a hit credited to its table id.  Each triangle
+    // {x, y, w} goes to its smallest vertex: x (a register count), y (the
     // chunk's partner count, one LDS add per chunk) or w (its table slot's
     // counter, one LDS add per such hit).
     auto probe_pv = [&](const int4 v, int k, int r) {
-      const int32_t w[4] = {r > 0 ? v.x : -1, r > 1 ? v.y : -1, r > 2 ? v.z : -1, r > 3 ? v.w : -1};
+      const int32_t w[4] = {v.x, v.y, v.z, v.w};
       int4 t[4];
 #pragma unroll
       for (int j = 0; j < 4; j++) t[j] = bucket_at(s_tab, bucket_addr(w[j]));
-      int slot[4];
-      bool need = false;
-#pragma unroll
-      for (int j = 0; j < 4; j++) {
-        slot[j] = bucket_slot(t[j], bucket_addr(w[j]) >> 4, w[j]);
-        need |= slot[j] < 0 && bucket_spilled(t[j]) && w[j] >= 0;
-      }
-      if (spl) {  // uniform: only tables with a spilled bucket can need this
-        if (need) {
-#pragma unroll
-          for (int j = 0; j < 4; j++) {
-            if (w[j] < 0 || slot[j] >= 0 || !bucket_spilled(t[j])) continue;
-            slot[j] = tab_continue_slot(s_tab, bucket_addr(w[j]) >> 4, w[j]);
-          }
-        }
-      }
       const int32_t x32 = (int32_t)x;  // ids are below 2^31
       const int32_t y = s_rec[k].w;
       const int32_t lo = min(x32, y);
       uint32_t nlo = 0;  // hits whose smallest vertex is min(x, y)
+      // One hit of w at slot index `slot`: counted, and credited to w when w is
+      // the smallest of the three.
+      auto hit = [&](int32_t wj, int slot) {
+        hits++;
+        if (wj < lo)
+          atomicAdd(&s_wcnt[slot >> 1], 1u << (16 * (slot & 1)));
+        else
+          nlo++;
+      };
 #pragma unroll
       for (int j = 0; j < 4; j++) {
-        const bool h = slot[j] >= 0;
-        const bool to_w = h && w[j] < lo;
-        hits += h;
-        nlo += h && !to_w;
-        if (to_w) atomicAdd(&s_wcnt[slot[j] >> 1], 1u << (16 * (slot[j] & 1)));
+        const bool e0 = t[j].x == w[j], e1 = t[j].y == w[j], e2 = t[j].z == w[j], e3 = t[j].w == w[j];
+        if (j < r && (e0 | e1 | e2 | e3))
+          hit(w[j], (int)(bucket_addr(w[j]) >> 2) + (e1 | e3 ? 1 : 0) + (e2 | e3 ? 2 : 0));
+      }
+      // Slot 3 of a spilled bucket carries the mark, so the raw compare missed an
+      // id there; lanes with a spilled bucket re-test it and walk on (only
+      // tables with a spilled bucket can have one; uniform test).
+      if (spl && __builtin_amdgcn_ballot_w64((t[0].w | t[1].w | t[2].w | t[3].w) < 0)) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          if (j >= r || t[j].w >= 0) continue;
+          if (t[j].x == w[j] || t[j].y == w[j] || t[j].z == w[j]) continue;  // counted above
+          int slot = -1;
+          if ((t[j].w & 0x7fffffff) == w[j])
+            slot = (int)(bucket_addr(w[j]) >> 2) + 3;
+          else
+            slot = tab_continue_slot(s_tab, bucket_addr(w[j]) >> 4, w[j]);
+          if (slot >= 0) hit(w[j], slot);
+        }
       }
       if (x32 < y)
         at_x += nlo;
