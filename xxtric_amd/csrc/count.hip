@@ -910,7 +910,7 @@ __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(W
       for (int j = 0; j < 4; j++) {
         const bool e0 = t[j].x == w[j], e1 = t[j].y == w[j], e2 = t[j].z == w[j], e3 = t[j].w == w[j];
         if (j < r && (e0 | e1 | e2 | e3))
-          hit(w[j], (int)(bucket_addr(w[j]) >> 2) + (e1 | e3 ? 1 : 0) + (e2 | e3 ? 2 : 0));
+          hit(w[j], (int)(bucket_addr(w[j]) >> 2) + ((e1 | e3) ? 1 : 0) + ((e2 | e3) ? 2 : 0));
       }
       // Slot 3 of a spilled bucket carries the mark, so the raw compare missed an
       // id there; lanes with a spilled bucket re-test it and walk on (only
