@@ -17,9 +17,10 @@
 //                  last row (tailcnt) -- the carry needed by rows that continue
 //                  into the next chunk.
 //  scans:          ooff = excl_scan(dplus), q = excl_scan(tailcnt).
-//  k_orient_fill:  writes each kept entry at ooff[u] + (kept entries of u in
-//                  earlier chunks = q[c] - q[first chunk of u]) + (kept entries
-//                  of u earlier in this chunk, from a block scan).
+//  k_orient_fill:  a chunk's kept entries fill one contiguous range of the
+//                  oriented ids, from ooff[u] + (kept entries of its first row u
+//                  in earlier chunks = q[c] - q[first chunk of u]); each entry
+//                  goes to that start + its rank in the chunk (a block scan).
 #include "xt_device.hpp"
 #include "xt_internal.hpp"
 
@@ -231,57 +232,30 @@ __global__ __launch_bounds__(kBlock) void k_orient_fill(
     int64_t nv, int64_t ne, const int64_t* __restrict__ off, const T* __restrict__ tails,
     const uint8_t* __restrict__ oflags, const int64_t* __restrict__ ooff,
     const int64_t* __restrict__ q, const int64_t* __restrict__ crows, int32_t* __restrict__ otails) {
-  __shared__ int64_t s_off[kOffCap + 1];
-  __shared__ uint16_t s_x[kChunk];
   __shared__ int64_t s_scan[kBlock / kWave];
   const int64_t c = blockIdx.x;
   const int64_t eb = c * kChunk;
   const int64_t ee = min(ne, eb + (int64_t)kChunk);
-  const ChunkRows r = chunk_rows_at(off, crows, nv, c, s_off);
-
   const int64_t e0 = eb + (int64_t)threadIdx.x * kPerThread;
   int64_t vals[kPerThread];
-  int64_t rows[kPerThread];
   uint32_t fl = 0;
   if (e0 < ee) {
     fl = oflags[e0 / kPerThread];
     if (fl) load_run<T>(tails, e0, ee, vals);
-    int64_t u = chunk_row_of(r, s_off, off, e0);
-    int64_t nxt = chunk_off(r, s_off, off, u + 1);
-#pragma unroll
-    for (int j = 0; j < kPerThread; j++) {
-      const int64_t e = e0 + j;
-      rows[j] = u;
-      if (e < ee) {
-        while (e >= nxt) {
-          ++u;
-          nxt = chunk_off(r, s_off, off, u + 1);
-        }
-        rows[j] = u;
-      }
-    }
   }
   int64_t total;
-  int64_t x = block_excl_scan_i64<kBlock>((int64_t)__popc(fl), s_scan, &total);
-#pragma unroll
-  for (int j = 0; j < kPerThread; j++) {
-    s_x[threadIdx.x * kPerThread + j] = (uint16_t)x;
-    x += (fl >> j) & 1u;
-  }
-  __syncthreads();
+  const int64_t x = block_excl_scan_i64<kBlock>((int64_t)__popc(fl), s_scan, &total);
   if (fl) {
-    const int64_t qc = q[c];
+    // Rows are consecutive and each row's out-list follows the previous one,
+    // so the chunk's kept entries, in entry order, fill one contiguous range:
+    // it starts in the chunk's first row u after u's kept entries in earlier
+    // chunks (q[c] - q[first chunk of u]).
+    const int64_t u = chunk_span(crows, c, nv).lo;
+    const int64_t ou = off[u];
+    const int64_t base = ooff[u] + (ou < eb ? q[c] - q[ou / kChunk] : 0) + x;
 #pragma unroll
-    for (int j = 0; j < kPerThread; j++) {
-      if ((fl >> j) & 1u) {
-        const int64_t u = rows[j];
-        const int64_t ou = chunk_off(r, s_off, off, u);
-        const int64_t first_local = (ou > eb ? ou : eb) - eb;
-        const int64_t rank = (int64_t)s_x[threadIdx.x * kPerThread + j] - (int64_t)s_x[first_local];
-        const int64_t prev = ou < eb ? qc - q[ou / kChunk] : 0;
-        otails[ooff[u] + prev + rank] = (int32_t)vals[j];
-      }
-    }
+    for (int j = 0; j < kPerThread; j++)
+      if ((fl >> j) & 1u) otails[base + __popc(fl & ((1u << j) - 1u))] = (int32_t)vals[j];
   }
 }
 
