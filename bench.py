@@ -242,16 +242,24 @@ def serve_workload(ctx, rate, n_req=4000, pool=400, seed=7, borrow=True):
     srv.drain(120.0)
     solo = {}
     ok = True
+    lat_small, lat_large = [], []
     for tk, k in tickets:
-        tot, _ = srv.poll(tk)
+        tot, lat = srv.poll(tk)
+        (lat_large if k >= pool else lat_small).append(lat)
         if k not in solo:
             solo[k] = ctx.count_csr(*graphs[k])
         ok &= tot == solo[k]
     st = srv.stats()
     srv.close()
+    pct = lambda v, q: round(float(np.percentile(v, q)), 1) if v else None  # noqa: E731
     return {"rate_graphs_per_s": rate, "requests": n_req, "graphs_per_s": round(st["graphs_per_s"], 1),
             "p50_us": round(st["p50_us"], 1), "p99_us": round(st["p99_us"], 1),
-            "batches": st["batches"], "counts_equal_solo": bool(ok)}
+            "batches": st["batches"], "counts_equal_solo": bool(ok),
+            # the same latencies split by request class: the pool's small graphs and the
+            # 1 % R-MAT s18-s20 requests
+            "small": {"requests": len(lat_small), "p50_us": pct(lat_small, 50), "p99_us": pct(lat_small, 99)},
+            "rmat_s18_s20": {"requests": len(lat_large), "p50_us": pct(lat_large, 50),
+                             "p99_us": pct(lat_large, 99)}}
 
 
 def copy_calibration(ctx, nbytes=1 << 30, reps=3):

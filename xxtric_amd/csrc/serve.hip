@@ -4,7 +4,8 @@
 // `max_inflight` batch slots, each an xt_batch with its own HIP stream and
 // pre-sized HBM arena.  Whenever a slot is free it admits queued requests in
 // arrival order while their HBM footprint fits the budget and the slot's
-// arena, uploads them and launches the persistent batch pipeline on the slot's
+// arena (below saturation, a request of kLargeRequest entries or more alone),
+// uploads them and launches the persistent batch pipeline on the slot's
 // stream; slots in flight run concurrently on the device.  Finished slots are
 // read back, their requests completed (latency = completion - arrival) and
 // their footprint released, which admits the next requests.
@@ -65,6 +66,8 @@ struct Req {
 // mirror in parallel, ranges of at most kCopyRange elements, checking borrowed
 // requests in the same pass (offsets monotone, ids in [0, nv)).
 constexpr int64_t kCopyRange = 1 << 22;
+// Requests with at least this many directed entries get a batch of their own.
+constexpr int64_t kLargeRequest = int64_t(1) << 22;
 
 struct CopyTask {
   const int64_t* off = nullptr;  // offsets [lo, hi) (off != nullptr) or ids [lo, hi)
@@ -261,8 +264,16 @@ void worker_main(xt_server* s) {
           }
         }
         int64_t V = free_slot->V, E = free_slot->E, G = (int64_t)free_slot->reqs.size();
+        // Below saturation (less than an eighth of a slot's graphs queued) a
+        // large request is counted in a batch of its own, so small graphs never
+        // wait for it inside one batch: they go on through the other slots
+        // while it runs.  Near saturation everything is batched together, for
+        // throughput.
+        const bool isolate = (int64_t)s->queue.size() * 8 < s->slot_g;
         while (!s->queue.empty()) {
           Req* r = s->queue.front();
+          const bool large = isolate && r->ne >= kLargeRequest;
+          if (large && (G > 0 || !take.empty())) break;
           if (G + 1 > s->slot_g || V + r->nv > s->slot_v || E + r->ne > s->slot_e) break;
           if (s->resident + r->footprint > s->cfg.hbm_budget_bytes) break;  // wait for memory
           s->queue.pop_front();
@@ -273,6 +284,7 @@ void worker_main(xt_server* s) {
           V += r->nv;
           E += r->ne;
           G++;
+          if (large) break;
         }
       }
       // Lay the admitted requests out in the slot's pinned mirror, then copy
