@@ -79,6 +79,30 @@ def test_alone_equals_in_mix(ctx):
     srv.close()
 
 
+def test_large_request_among_small_ones(ctx):
+    """An R-MAT s18 request (over 4 M entries) arriving between small graphs is
+    counted in a batch of its own while little is queued; every count must
+    still equal the graph's solo count, and the small graphs after it are not
+    held back in its batch."""
+    big = ctx.rmat(18, 16, 3, True)
+    bo, bt = big.download()
+    big.close()
+    assert len(bt) >= 1 << 22
+    exp_big = ctx.count_csr(bo, np.asarray(bt, np.int64))
+    gs = _graphs(900, 40)
+    exp = [O.count_degree_ordered(o, t) for o, t in gs]
+    srv = xxtric_amd.Server(ctx, hbm_budget_bytes=4 << 30, max_batch_graphs=4096, max_inflight=2)
+    _say("large request among small ones")
+    tickets = [srv.submit(o, t) for o, t in gs[:20]]
+    tb = srv.submit(bo, bt)
+    tickets += [srv.submit(o, t) for o, t in gs[20:]]
+    res = _drain_all(srv, tickets + [tb])
+    assert [res[t][0] for t in tickets] == exp
+    assert res[tb][0] == exp_big
+    assert srv.stats()["batches"] >= 2
+    srv.close()
+
+
 def test_admission_budget(ctx):
     budget = 16 << 20  # 16 MiB: small graphs only, few at a time
     srv = xxtric_amd.Server(ctx, hbm_budget_bytes=budget, max_batch_graphs=64)
