@@ -108,24 +108,32 @@ int xt_count_csr(xt_ctx* ctx, int64_t nv, int64_t ne, const int64_t* offsets,
 
 /* ---- lower-FFI drop-ins (device pointers, caller-owned buffers) ---------
  * Same arguments and ownership as the reference launchers; the count is
- * ADDED to *d_local_count (the caller zeroes it, xxtric-gpu.hpp:150) and the
- * call is asynchronous on `stream` (a hipStream_t, NULL = the context's). */
+ * ADDED to *d_local_count (the caller zeroes it, xxtric-gpu.hpp:150).
+ * `stream` is passed as the reference passes it: a stream value (hipStream_t)
+ * for xt_single_gpu_count and xt_single_gpu_count_masked, the ADDRESS of one
+ * (hipStream_t*) for xt_local_count_gpu_inner; NULL = the context's stream. */
 
 /* counting.cuh:259-265 single_gpu_count (counting.cu:537-577): whole-graph count from a device
- * int64 CSR (symmetric, sorted rows). */
+ * int64 CSR (symmetric, sorted rows), 0 <= nv < 2^31-1 (XT_ERR_ARG otherwise).
+ * Returns once the count has been added to *d_local_count (the call reads
+ * d_indices[nv] and checks the ids, so it synchronises with `stream`). */
 int xt_single_gpu_count(xt_ctx* ctx, int64_t nv, const int64_t* d_indices,
                         const int64_t* d_edge_list, int64_t* d_local_count,
                         void* stream);
 
 /* counting.cuh:229-245 single_gpu_count_masked (counting.cu:297-359): nedges pairs (e1, m) in
  * d_edge_mask (int64, 2 per edge, built as xxtric-gpu.hpp:91-107); counts
- * |{n in (m, e1) : d_edge_list[n] in N(d_edge_list[m])}| summed over pairs. */
+ * |{n in (m, e1) : d_edge_list[n] in N(d_edge_list[m])}| summed over pairs.
+ * Asynchronous on `stream` (a hipStream_t value). */
 int xt_single_gpu_count_masked(xt_ctx* ctx, int64_t nedges, const int64_t* d_edge_mask,
                                const int64_t* d_indices, const int64_t* d_edge_list,
                                int64_t* d_local_count, void* stream);
 
 /* counting.cuh:314-323 local_count_gpu_inner (counting.cu:640-657): counts
- * |{i in [start,end) : d_edge_list[i] in d_edge_list[jstart..jend)}|. */
+ * |{i in [start,end) : d_edge_list[i] in d_edge_list[jstart..jend)}|.
+ * Asynchronous on *(hipStream_t*)stream: like the reference's cudaStream_t*
+ * argument (xxtric-gpu.hpp:225 passes &streams[id]), `stream` is the address
+ * of a stream, not the stream. */
 int xt_local_count_gpu_inner(xt_ctx* ctx, int64_t start, int64_t end, int64_t jstart,
                              int64_t jend, const int64_t* d_edge_list,
                              int64_t* d_local_count, void* stream);

@@ -148,3 +148,75 @@ def test_staged_host_adds_survive_reuse(ctx):
             assert (b.results() == exp).all()
         b.clear()
     b.close()
+
+
+def _fixture_4096():
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "batch_4096.json")) as f:
+        return json.load(f)
+
+
+def test_batch_4096_at_full_size(ctx, golden):
+    """The 4096-graph mix itself (18 M vertices, 388 M entries), in all three
+    launch modes, against the oracle's count of every graph (the fixture is
+    made by the oracle's own generators and counter) and graphs 0-7 against
+    the reference binary's counts."""
+    fx = _fixture_4096()
+    n = len(fx["graphs"])
+    assert n == 4096 and fx["base_seed"] == SEED
+    exp = np.array([x[3] for x in fx["graphs"]], np.uint64)
+    b = xxtric_amd.Batch(ctx, 20_000_000, 400_000_000, n)
+    _say("batch 4096 build")
+    b.add_generated(SEED, 0, n, fx["nmin"], fx["nmax"])
+    for mode, reps in ((XT_BATCH_PERSISTENT, 2), (XT_BATCH_HIPGRAPH, 3), (XT_BATCH_PER_GRAPH, 1)):
+        for r in range(reps):
+            _say("batch 4096 mode", mode, "run", r)
+            b.run(mode)
+            got = b.results()
+            bad = np.nonzero(got != exp)[0]
+            assert len(bad) == 0, (mode, r, bad[:10], got[bad[:10]], exp[bad[:10]])
+    t = b.timing()
+    assert t["n_graphs"] == n
+    assert t["n_vertices"] == sum(x[0] for x in fx["graphs"])
+    assert t["n_edges"] == sum(x[2] for x in fx["graphs"])
+    for g in golden["batch"]:
+        assert int(got[g["index"]]) == g["triangles"]
+    b.close()
+
+
+def _tiny_graphs(count, seed):
+    """Graphs of 3-8 vertices with closed-form counts: K3 (1), K4 (4), K5 (10),
+    paths (0), stars (0), a 4-cycle with one chord (2), two triangles sharing
+    a vertex (2)."""
+    from helpers import complete, csr_from_edges
+    shapes = [(complete(3), 1), (complete(4), 4), (complete(5), 10),
+              (csr_from_edges(4, [(0, 1), (1, 2), (2, 3), (3, 0), (0, 2)]), 2),
+              (csr_from_edges(5, [(0, 1), (1, 2), (2, 0), (0, 3), (3, 4), (4, 0)]), 2)]
+    for k in range(3, 9):
+        shapes.append((csr_from_edges(k, [(i, i + 1) for i in range(k - 1)]), 0))
+    shapes.append((csr_from_edges(6, [(0, i) for i in range(1, 6)]), 0))
+    rng = np.random.default_rng(seed)
+    pick = rng.integers(0, len(shapes), count)
+    return [shapes[i][0] for i in pick], np.array([shapes[i][1] for i in pick], np.uint64)
+
+
+def test_tiny_graphs_many_per_chunk(ctx):
+    """Hundreds of graphs of 4-20 entries: one 2048-entry chunk of the light
+    pass spans far more than its 64 LDS graph slots, so most totals take the
+    direct per-graph atomic; every mode must still give the closed forms."""
+    graphs, exp = _tiny_graphs(900, 11)
+    # a K3 run long enough that one chunk holds 341 graphs
+    from helpers import complete
+    k3 = complete(3)
+    graphs += [k3] * 400
+    exp = np.concatenate([exp, np.ones(400, np.uint64)])
+    b = xxtric_amd.Batch(ctx, 100_000, 1_000_000, len(graphs))
+    for o, t in graphs:
+        b.add(o, t.astype(np.int32))
+    for mode in (XT_BATCH_PERSISTENT, XT_BATCH_HIPGRAPH, XT_BATCH_PER_GRAPH, XT_BATCH_HIPGRAPH):
+        _say("tiny graphs mode", mode)
+        b.run(mode)
+        got = b.results()
+        assert (got == exp).all(), (mode, np.nonzero(got != exp)[0][:10])
+    b.close()

@@ -277,6 +277,73 @@ def test_reference_ffi_shapes(ctx):
         b.close()
 
 
+def _hip():
+    """The HIP runtime libxtric.so already loaded (same soname, same handle)."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    hip.hipStreamCreateWithFlags.argtypes = [ctypes.c_void_p, ctypes.c_uint]
+    hip.hipStreamSynchronize.argtypes = [ctypes.c_void_p]
+    hip.hipStreamDestroy.argtypes = [ctypes.c_void_p]
+    return hip
+
+
+def test_reference_ffi_streams(ctx):
+    """The launchers on the caller's streams, passed as the reference passes
+    them: the inner launcher gets the ADDRESS of one of eight streams
+    (xxtric-gpu.hpp:201-227, &streams[id % MAX_CUDA_STREAMS]), the masked and
+    whole-graph launchers a stream value."""
+    import ctypes
+    hip = _hip()
+    streams = (ctypes.c_void_p * 8)()
+    for t in range(8):
+        assert hip.hipStreamCreateWithFlags(ctypes.byref(streams, t * ctypes.sizeof(ctypes.c_void_p)), 1) == 0
+    o2, t2 = random_graph(80, 0.3, 5)
+    t2 = np.ascontiguousarray(t2, np.int64)
+    exp2 = O.count_reference(o2, t2)
+    d_ind = ctx.to_device(np.ascontiguousarray(o2, np.int64))
+    d_el = ctx.to_device(t2)
+    zero = np.zeros(1, np.int64)
+    d_cnt = ctx.to_device(zero)
+    _say("ffi inner on stream addresses")
+    nst = 0
+    for i in range(len(o2) - 1):
+        e0, e1 = o2[i], o2[i + 1]
+        for m in range(e0, e1 - 1):
+            v = t2[m]
+            if i >= v:
+                continue
+            addr = ctypes.addressof(streams) + (nst % 8) * ctypes.sizeof(ctypes.c_void_p)
+            nst += 1
+            ctx.local_count_gpu_inner(int(m + 1), int(e1), int(o2[v]), int(o2[v + 1]), d_el.ptr,
+                                      d_cnt.ptr, addr)
+    for t in range(8):
+        assert hip.hipStreamSynchronize(streams[t]) == 0
+    assert int(d_cnt.download(np.int64, 1)[0]) == exp2
+    _say("ffi single_gpu_count on a stream value")
+    d_cnt.upload(zero)
+    ctx.single_gpu_count(len(o2) - 1, d_ind.ptr, d_el.ptr, d_cnt.ptr, streams[3])
+    # the call returns once the count has been added: no synchronisation here
+    assert int(d_cnt.download(np.int64, 1)[0]) == exp2
+    for t in range(8):
+        assert hip.hipStreamDestroy(streams[t]) == 0
+    for b in (d_ind, d_el, d_cnt):
+        b.close()
+
+
+def test_single_gpu_count_rejects_wide_vertex_counts(ctx):
+    """nv >= 2^31 - 1 is refused before anything is read from the device
+    arrays (the device ids are int32)."""
+    d_ind = ctx.to_device(np.zeros(4, np.int64))
+    d_cnt = ctx.to_device(np.zeros(1, np.int64))
+    _say("ffi single_gpu_count nv=2^31")
+    with pytest.raises(xxtric_amd.XtError) as e:
+        ctx.single_gpu_count(1 << 31, d_ind.ptr, d_ind.ptr, d_cnt.ptr)
+    assert e.value.code == -1
+    assert int(d_cnt.download(np.int64, 1)[0]) == 0
+    d_ind.close()
+    d_cnt.close()
+
+
 def test_tric_driver(golden, tmp_path):
     """The reference-compatible driver: same flags, same output lines."""
     import os

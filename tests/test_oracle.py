@@ -187,3 +187,26 @@ def test_rmat_generator_invariants():
     o2, t2 = O.rmat_csr(10, 16, 5, False)
     assert sorted(np.diff(o2)) == sorted(d)
     assert O.count_degree_ordered(o2, t2) == O.count_degree_ordered(off, tails)
+
+
+def test_batch_4096_fixture_is_pinned(golden):
+    """tests/golden/batch_4096.json (the oracle's count of every graph of the
+    4096-graph mix) agrees with the reference binary on graphs 0-7 and with the
+    reference-shaped count recomputed now on a random spread of graphs."""
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "batch_4096.json")) as f:
+        fx = json.load(f)
+    graphs = fx["graphs"]
+    assert len(graphs) == 4096
+    for g in golden["batch"]:
+        n, kind, ne, tri = graphs[g["index"]]
+        assert (n, kind, ne, tri) == (g["n"], g["kind"], g["ne"], g["triangles"])
+    rng = np.random.default_rng(3)
+    for gi in rng.choice(4096, 10, replace=False):
+        n, kind, gseed, thr = O.batch_params(fx["base_seed"], int(gi), fx["nmin"], fx["nmax"])
+        if kind == 0:
+            off, tails = O.rgg_lcg(n, seed=(gseed ^ (gseed >> 32)) & 0xFFFFFFFF)
+        else:
+            off, tails = O.er_csr(n, gseed + 3, thr)
+        assert graphs[gi] == [n, kind, int(off[-1]), O.count_reference(off, tails)], gi

@@ -342,6 +342,8 @@ int xt_single_gpu_count(xt_ctx* ctx, int64_t nv, const int64_t* d_indices, const
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     XT_HIP(ctx, hipSetDevice(ctx->device));
     if (nv < 0 || !d_indices || !d_local_count) return set_err(ctx, XT_ERR_ARG, "bad arguments");
+    // Device ids are int32 past this point: reject before d_indices[nv] is read.
+    if (nv >= (1LL << 31) - 1) return set_err(ctx, XT_ERR_ARG, "nv must be below 2^31-1 (int32 device ids)");
     if (nv == 0) return XT_OK;
     hipStream_t st = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
     int64_t ne = 0;
@@ -358,7 +360,11 @@ int xt_single_gpu_count(xt_ctx* ctx, int64_t nv, const int64_t* d_indices, const
                                hipMemcpyDeviceToHost, st));
     XT_HIP(ctx, hipStreamSynchronize(st));
     if (bad) return set_err(ctx, XT_ERR_INPUT, "neighbour id out of range [0, nv)");
-    return launch_add_total(ctx, d_local_count, st);
+    XT_TRY(launch_add_total(ctx, d_local_count, st));
+    // The add reads the context's counters, which the next call on this
+    // context (on any stream) clears: it must be done before this returns.
+    XT_HIP(ctx, hipStreamSynchronize(st));
+    return XT_OK;
   });
 }
 
@@ -383,7 +389,9 @@ int xt_local_count_gpu_inner(xt_ctx* ctx, int64_t start, int64_t end, int64_t js
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     XT_HIP(ctx, hipSetDevice(ctx->device));
     if (!d_edge_list || !d_local_count || start < 0 || jstart < 0) return set_err(ctx, XT_ERR_ARG, "bad arguments");
-    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+    // The reference passes the address of a stream (counting.cuh:314-323,
+    // xxtric-gpu.hpp:225) and its launcher dereferences it (counting.cu:656).
+    hipStream_t st = stream ? *static_cast<hipStream_t*>(stream) : ctx->stream;
     return launch_inner(ctx, start, end, jstart, jend, d_edge_list, d_local_count, st);
   });
 }
