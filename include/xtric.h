@@ -121,6 +121,21 @@ int xt_single_gpu_count(xt_ctx* ctx, int64_t nv, const int64_t* d_indices,
                         const int64_t* d_edge_list, int64_t* d_local_count,
                         void* stream);
 
+/* No counterpart in the reference: xt_single_gpu_count for callers that
+ * cannot allocate device memory (tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, checks and result; the call itself allocates
+ * nothing (it still synchronises with `stream`, so it is not for stream
+ * capture).  Its scratch is one statically sized 64 MiB device array per GPU,
+ * held from the first context created on that GPU until the process ends; a
+ * graph needing more (at most 68 bytes per vertex plus 20.3 per entry: e.g.
+ * 65,536 vertices and 3 M entries fit) is rejected with XT_ERR_CAPACITY
+ * before any kernel runs.  All calls on one GPU share that array, so calls
+ * that overlapped would race: the library serialises them per device and
+ * each returns only after its last kernel has finished. */
+int xt_single_gpu_count_noalloc(xt_ctx* ctx, int64_t nv, const int64_t* d_indices,
+                                const int64_t* d_edge_list, int64_t* d_local_count,
+                                void* stream);
+
 /* counting.cuh:229-245 single_gpu_count_masked (counting.cu:297-359): nedges pairs (e1, m) in
  * d_edge_mask (int64, 2 per edge, built as xxtric-gpu.hpp:91-107); counts
  * |{n in (m, e1) : d_edge_list[n] in N(d_edge_list[m])}| summed over pairs.

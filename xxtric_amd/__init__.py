@@ -162,6 +162,11 @@ class Context:
         self._check(self._L.xt_single_gpu_count(self.h, nv, d_indices, d_edge_list, d_local_count,
                                                 stream))
 
+    def single_gpu_count_noalloc(self, nv, d_indices, d_edge_list, d_local_count, stream=None):
+        """xt_single_gpu_count without any device allocation in the call (64 MiB static scratch)."""
+        self._check(self._L.xt_single_gpu_count_noalloc(self.h, nv, d_indices, d_edge_list,
+                                                        d_local_count, stream))
+
     def single_gpu_count_masked(self, nedges, d_edge_mask, d_indices, d_edge_list, d_local_count,
                                 stream=None):
         self._check(self._L.xt_single_gpu_count_masked(self.h, nedges, d_edge_mask, d_indices,

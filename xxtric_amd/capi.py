@@ -91,6 +91,7 @@ _SIGS = {
     "xt_dev_sync": (_int, [_p]),
     "xt_count_csr": (_int, [_p, _i64, _i64, _p, _p, _u32, _p, _p]),
     "xt_single_gpu_count": (_int, [_p, _i64, _p, _p, _p, _p]),
+    "xt_single_gpu_count_noalloc": (_int, [_p, _i64, _p, _p, _p, _p]),
     "xt_single_gpu_count_masked": (_int, [_p, _i64, _p, _p, _p, _p, _p]),
     "xt_local_count_gpu_inner": (_int, [_p, _i64, _i64, _i64, _i64, _p, _p, _p]),
     "xt_graph_from_csr": (_int, [_p, _i64, _i64, _p, _p, _int, _u32, _pp]),

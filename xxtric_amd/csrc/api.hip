@@ -13,7 +13,7 @@ int launch_masked(xt_ctx* ctx, int64_t nedges, const int64_t* mask, const int64_
                   const int64_t* L, int64_t* out, hipStream_t st);
 int launch_inner(xt_ctx* ctx, int64_t start, int64_t end, int64_t jstart, int64_t jend,
                  const int64_t* L, int64_t* out, hipStream_t st);
-int launch_add_total(xt_ctx* ctx, int64_t* out, hipStream_t st);
+int launch_add_total(xt_ctx* ctx, int64_t* out, hipStream_t st, const uint64_t* counters = nullptr);
 
 __global__ __launch_bounds__(256) void k_copy16(int4* __restrict__ dst, const int4* __restrict__ src,
                                                 int64_t n) {
@@ -46,6 +46,9 @@ int hip_err(xt_ctx* ctx, hipError_t e, const char* what) {
 int ensure(xt_ctx* ctx, DevBuf& b, size_t bytes) {
   if (bytes == 0) bytes = 8;
   if (b.bytes >= bytes) return XT_OK;
+  if (b.fixed)
+    return set_err(ctx, XT_ERR_CAPACITY, std::string("fixed scratch slice of ") + std::to_string(b.bytes) +
+                                             " bytes asked for " + std::to_string(bytes));
   if (b.p) {
     // Work queued on any stream (a caller's stream included) may still read it.
     XT_HIP(ctx, hipDeviceSynchronize());
@@ -61,6 +64,36 @@ int ensure(xt_ctx* ctx, DevBuf& b, size_t bytes) {
                                           " bytes failed: " + hipGetErrorString(e));
   }
   b.bytes = want;
+  return XT_OK;
+}
+
+// ---- scratch of the allocation-free entry point ----------------------------
+// One statically sized device array, resident on every GPU the library runs
+// on until the process ends.  Its host-side name is not a device address:
+// host code uses only the per-device address from hipGetSymbolAddress.
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;  // 64 MiB
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+struct NoAllocDevice {
+  std::mutex init_mu;   // guards ptr / ready
+  bool ready = false;
+  unsigned char* ptr = nullptr;  // this device's copy of `scratch`
+  std::mutex call_mu;   // one allocation-free count at a time on this device
+};
+static NoAllocDevice g_noalloc[kMaxDevices];
+
+// Looks up this device's address of `scratch` once; the device must be current.
+static int noalloc_init(xt_ctx* ctx, int device) {
+  if (device < 0 || device >= kMaxDevices) return set_err(ctx, XT_ERR_ARG, "device ordinal beyond 63");
+  NoAllocDevice& d = g_noalloc[device];
+  std::lock_guard<std::mutex> lk(d.init_mu);
+  if (d.ready) return XT_OK;
+  void* p = nullptr;
+  XT_HIP(ctx, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  if (!p) return set_err(ctx, XT_ERR_HIP, "hipGetSymbolAddress(scratch) returned null");
+  d.ptr = static_cast<unsigned char*>(p);
+  d.ready = true;
   return XT_OK;
 }
 
@@ -203,8 +236,14 @@ int xt_ctx_create(int device, xt_ctx** out) {
         return XT_ERR_HIP;
       }
     if (ensure(ctx, ctx->s.counters, kNumSlots * sizeof(uint64_t)) != XT_OK) {
+      release(ctx->s);
       delete ctx;
       return XT_ERR_NOMEM;
+    }
+    if (noalloc_init(ctx, device) != XT_OK) {
+      release(ctx->s);
+      delete ctx;
+      return XT_ERR_HIP;
     }
     *out = ctx;
     return XT_OK;
@@ -364,6 +403,54 @@ int xt_single_gpu_count(xt_ctx* ctx, int64_t nv, const int64_t* d_indices, const
     // The add reads the context's counters, which the next call on this
     // context (on any stream) clears: it must be done before this returns.
     XT_HIP(ctx, hipStreamSynchronize(st));
+    return XT_OK;
+  });
+}
+
+int xt_single_gpu_count_noalloc(xt_ctx* ctx, int64_t nv, const int64_t* d_indices,
+                                const int64_t* d_edge_list, int64_t* d_local_count, void* stream) {
+  return xt_guarded([&]() -> int {
+    if (!ctx) return XT_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    XT_HIP(ctx, hipSetDevice(ctx->device));
+    if (nv < 0 || !d_indices || !d_local_count) return set_err(ctx, XT_ERR_ARG, "bad arguments");
+    if (nv >= (1LL << 31) - 1) return set_err(ctx, XT_ERR_ARG, "nv must be below 2^31-1 (int32 device ids)");
+    if (ctx->device < 0 || ctx->device >= kMaxDevices || !g_noalloc[ctx->device].ready)
+      return set_err(ctx, XT_ERR_STATE, "no scratch address for this device");
+    NoAllocDevice& dev = g_noalloc[ctx->device];
+    if (nv == 0) return XT_OK;
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+    int64_t ne = 0;
+    XT_HIP(ctx, hipMemcpyAsync(&ne, d_indices + nv, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    XT_HIP(ctx, hipStreamSynchronize(st));
+    if (ne < 0) return set_err(ctx, XT_ERR_INPUT, "negative edge count in d_indices[nv]");
+    if (ne > 0 && !d_edge_list) return set_err(ctx, XT_ERR_ARG, "null d_edge_list");
+    const size_t need = carve_bytes(nv, ne);
+    if (need > kNoAllocScratchBytes)
+      return set_err(ctx, XT_ERR_CAPACITY,
+                     "graph needs " + std::to_string(need) + " bytes of scratch; the allocation-free count holds " +
+                         std::to_string(kNoAllocScratchBytes));
+    // Every call on this device shares its one array: calls are serialised
+    // here, and each returns only after its last kernel has finished.
+    std::lock_guard<std::mutex> dl(dev.call_mu);
+    Scratch s;
+    carve(s, dev.ptr, nv, ne);
+    uint64_t* counters = s.counters.as<uint64_t>();
+    auto run = [&]() -> int {
+      XT_TRY(validate<int64_t>(ctx, nv, ne, d_indices, d_edge_list, false, st, counters));
+      XT_TRY(count_device_csr<int64_t>(ctx, s, nv, ne, d_indices, d_edge_list, XT_ORIENT_DEGREE, st, false));
+      uint64_t bad = 0;
+      XT_HIP(ctx, hipMemcpyAsync(&bad, counters + kBadIds, sizeof(bad), hipMemcpyDeviceToHost, st));
+      XT_HIP(ctx, hipStreamSynchronize(st));
+      if (bad) return set_err(ctx, XT_ERR_INPUT, "neighbour id out of range [0, nv)");
+      return launch_add_total(ctx, d_local_count, st, counters);
+    };
+    const int rc = run();
+    // On every path, nothing queued by this call may still use the array
+    // once the device lock is released.
+    const hipError_t e = hipStreamSynchronize(st);
+    if (rc != XT_OK) return rc;
+    if (e != hipSuccess) return hip_err(ctx, e, "hipStreamSynchronize");
     return XT_OK;
   });
 }

@@ -1238,27 +1238,56 @@ int alg_stats(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne_bound, hipStream_t
   return XT_OK;
 }
 
-int reserve(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne, bool per_vertex) {
+// Every scratch buffer a count of a graph of up to (nv, ne) entries can
+// request, with the largest size orient(), count_oriented() and the scans ask
+// for: f(buffer, bytes).
+template <typename F>
+static void for_each_need(Scratch& s, int64_t nv, int64_t ne, bool per_vertex, F&& f) {
   const int64_t nvv = nv > 0 ? nv : 1, nee = ne > 0 ? ne : 1;
   const int64_t nchunks = (nee + kChunk - 1) / kChunk;
-  XT_TRY(ensure(ctx, s.deg, (size_t)nvv * sizeof(uint32_t)));
-  XT_TRY(ensure(ctx, s.dplus, (size_t)nvv * sizeof(uint32_t)));
-  XT_TRY(ensure(ctx, s.tailcnt, (size_t)nchunks * sizeof(uint32_t)));
-  XT_TRY(ensure(ctx, s.qscan, (size_t)(nchunks + 1) * sizeof(int64_t)));
-  XT_TRY(ensure(ctx, s.ooff, (size_t)(nvv + 1) * sizeof(int64_t)));
-  XT_TRY(ensure(ctx, s.otails, (size_t)(nee + kOtailsPad) * sizeof(int32_t)));
-  XT_TRY(ensure(ctx, s.counters, kNumSlots * sizeof(uint64_t)));
-  XT_TRY(ensure(ctx, s.oflags, (size_t)(nchunks * kBlock)));
-  XT_TRY(ensure(ctx, s.crows, (size_t)(2 * nchunks) * sizeof(int64_t)));
-  XT_TRY(ensure(ctx, s.itcnt, (size_t)nvv * sizeof(uint32_t)));
-  XT_TRY(ensure(ctx, s.items, (size_t)nee * sizeof(Item)));
-  XT_TRY(ensure(ctx, s.tileoff, (size_t)(nvv + 1) * sizeof(int64_t)));
-  XT_TRY(ensure(ctx, s.tiledesc, (size_t)tiles_bound(nvv, nee) * sizeof(TileDesc)));
-  if (nee < (1ll << 32)) XT_TRY(ensure(ctx, s.vrec, (size_t)nvv * sizeof(uint64_t)));
-  const size_t st = scan_tmp_bytes(nvv) > scan_tmp_bytes(nchunks) ? scan_tmp_bytes(nvv) : scan_tmp_bytes(nchunks);
-  XT_TRY(ensure(ctx, s.scan_tmp, st));
-  if (per_vertex) XT_TRY(ensure(ctx, s.pv, (size_t)nvv * sizeof(uint64_t)));
-  return XT_OK;
+  f(s.deg, (size_t)nvv * sizeof(uint32_t));
+  f(s.dplus, (size_t)nvv * sizeof(uint32_t));
+  f(s.tailcnt, (size_t)nchunks * sizeof(uint32_t));
+  f(s.qscan, (size_t)(nchunks + 1) * sizeof(int64_t));
+  f(s.ooff, (size_t)(nvv + 1) * sizeof(int64_t));
+  f(s.otails, (size_t)(nee + kOtailsPad) * sizeof(int32_t));
+  f(s.counters, kNumSlots * sizeof(uint64_t));
+  f(s.oflags, (size_t)(nchunks * kBlock));
+  f(s.crows, (size_t)(2 * nchunks) * sizeof(int64_t));
+  f(s.itcnt, (size_t)nvv * sizeof(uint32_t));
+  f(s.items, (size_t)nee * sizeof(Item));
+  f(s.tileoff, (size_t)(nvv + 1) * sizeof(int64_t));
+  f(s.tiledesc, (size_t)tiles_bound(nvv, nee) * sizeof(TileDesc));
+  if (nee < (1ll << 32)) f(s.vrec, (size_t)nvv * sizeof(uint64_t));
+  f(s.scan_tmp, scan_tmp_bytes(nvv) > scan_tmp_bytes(nchunks) ? scan_tmp_bytes(nvv) : scan_tmp_bytes(nchunks));
+  if (per_vertex) f(s.pv, (size_t)nvv * sizeof(uint64_t));
+}
+
+int reserve(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne, bool per_vertex) {
+  int rc = XT_OK;
+  for_each_need(s, nv, ne, per_vertex, [&](DevBuf& b, size_t bytes) {
+    if (rc == XT_OK) rc = ensure(ctx, b, bytes);
+  });
+  return rc;
+}
+
+static size_t round256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+size_t carve_bytes(int64_t nv, int64_t ne) {
+  Scratch tmp;
+  size_t tot = 0;
+  for_each_need(tmp, nv, ne, false, [&](DevBuf&, size_t bytes) { tot += round256(bytes); });
+  return tot;
+}
+
+void carve(Scratch& s, unsigned char* base, int64_t nv, int64_t ne) {
+  size_t at = 0;
+  for_each_need(s, nv, ne, false, [&](DevBuf& b, size_t bytes) {
+    b.p = base + at;
+    b.bytes = round256(bytes);
+    b.fixed = true;
+    at += b.bytes;
+  });
 }
 
 void release(Scratch& s) {
@@ -1266,6 +1295,7 @@ void release(Scratch& s) {
                     &s.oflags, &s.crows, &s.itcnt, &s.items, &s.tileoff, &s.tiledesc, &s.vrec, &s.pv, &s.scan_tmp, &s.in_off,
                     &s.in_tails};
   for (DevBuf* b : bufs) {
+    if (b->fixed) continue;
     if (b->p) (void)hipFree(b->p);
     b->p = nullptr;
     b->bytes = 0;
@@ -1343,9 +1373,9 @@ int launch_inner(xt_ctx* ctx, int64_t start, int64_t end, int64_t jstart, int64_
   return XT_OK;
 }
 
-int launch_add_total(xt_ctx* ctx, int64_t* out, hipStream_t st) {
-  hipLaunchKernelGGL(k_add_total, dim3(1), dim3(64), 0, st,
-                     (const uint64_t*)ctx->s.counters.as<uint64_t>(), (unsigned long long*)out);
+int launch_add_total(xt_ctx* ctx, int64_t* out, hipStream_t st, const uint64_t* counters) {
+  if (!counters) counters = ctx->s.counters.as<uint64_t>();
+  hipLaunchKernelGGL(k_add_total, dim3(1), dim3(64), 0, st, counters, (unsigned long long*)out);
   XT_HIP(ctx, hipGetLastError());
   return XT_OK;
 }

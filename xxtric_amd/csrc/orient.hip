@@ -485,8 +485,8 @@ __global__ __launch_bounds__(kBlock) void k_validate(int64_t nv, int64_t ne,
 // Offsets check followed by the entry checks; reads the verdict back (syncs).
 template <typename T>
 int validate(xt_ctx* ctx, int64_t nv, int64_t ne, const int64_t* off, const T* tails,
-             bool symmetry, hipStream_t st) {
-  uint64_t* counters = ctx->s.counters.as<uint64_t>();
+             bool symmetry, hipStream_t st, uint64_t* counters) {
+  if (!counters) counters = ctx->s.counters.as<uint64_t>();
   XT_TRY(fill_u64(ctx, counters + kBadIds, 4, 0, st));
   hipLaunchKernelGGL(k_check_offsets, dim3(grid_for(nv, 256, 4096)), dim3(256), 0, st, nv, ne, off,
                      counters);
@@ -510,8 +510,8 @@ int validate(xt_ctx* ctx, int64_t nv, int64_t ne, const int64_t* off, const T* t
 }
 
 template int validate<int32_t>(xt_ctx*, int64_t, int64_t, const int64_t*, const int32_t*, bool,
-                               hipStream_t);
+                               hipStream_t, uint64_t*);
 template int validate<int64_t>(xt_ctx*, int64_t, int64_t, const int64_t*, const int64_t*, bool,
-                               hipStream_t);
+                               hipStream_t, uint64_t*);
 
 }  // namespace xt

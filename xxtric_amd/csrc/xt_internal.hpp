@@ -49,6 +49,7 @@ enum Orient : int { kOrientDegree = 0, kOrientId = 1 };
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  bool fixed = false;  // a slice of caller-provided memory: never allocated, grown or freed
   template <typename T>
   T* as() const { return static_cast<T*>(p); }
 };
@@ -143,6 +144,13 @@ int count_oriented(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne_bound, const 
 int alg_stats(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne_bound, hipStream_t st);
 // Pre-sizes every scratch buffer for graphs up to (nv, ne).
 int reserve(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne, bool per_vertex);
+// Bytes a single-graph count of (nv, ne) needs when every scratch buffer is a
+// 256-byte-aligned slice of one region (global count, device input CSR).
+size_t carve_bytes(int64_t nv, int64_t ne);
+// Points every buffer a global count of (nv, ne) uses at a slice of
+// [base, base + carve_bytes(nv, ne)), marked fixed so that no later request
+// can allocate.  The caller checks the region is large enough.
+void carve(Scratch& s, unsigned char* base, int64_t nv, int64_t ne);
 void release(Scratch& s);
 
 size_t scan_tmp_bytes(int64_t n);
@@ -152,9 +160,10 @@ int fill_u32(xt_ctx* ctx, uint32_t* p, int64_t n, uint32_t v, hipStream_t st);
 int fill_u64(xt_ctx* ctx, uint64_t* p, int64_t n, uint64_t v, hipStream_t st);
 
 // Validation of a device CSR; returns XT_ERR_INPUT with a message on failure.
+// counters (nullable): the uint64[kNumSlots] slots to use, else the context's.
 template <typename T>
 int validate(xt_ctx* ctx, int64_t nv, int64_t ne, const int64_t* off, const T* tails,
-             bool symmetry, hipStream_t st);
+             bool symmetry, hipStream_t st, uint64_t* counters = nullptr);
 
 // Whole single-graph count on the device (orientation + intersection).
 template <typename T>
