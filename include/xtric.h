@@ -225,6 +225,17 @@ int xt_graph_destroy(xt_ctx* ctx, xt_graph* g);
  * reference's all-pairs loop.  Returns ne in *ne; fills tails when cap >= ne. */
 int xt_rgg_generate(int64_t nv, uint32_t seed, double radius, int64_t* offsets,
                     int64_t* tails, int64_t cap, int64_t* ne);
+/* The RGG the reference builds when run on nprocs MPI ranks (graph.hpp:745-1097,
+ * utils.hpp:245-430), all ranks' rows in one CSR with global ids: rank r owns
+ * vertices [r*nv/nprocs, (r+1)*nv/nprocs), its LCG stream starts from the
+ * reference's parallel prefix (2x2 matrix powers in wrapping int64 arithmetic,
+ * as its mat_power leaves them unreduced), its points lie in the strip
+ * Y in [r/nprocs, (r+1)/nprocs), and points of neighbouring ranks are joined
+ * only when their local indices differ (the reference's cross loops test
+ * j > i).  nprocs = 1 is xt_rgg_generate.  XT_ERR_ARG unless nprocs is a
+ * power of two dividing nv and the radius is below 1/nprocs (graph.hpp:765-793). */
+int xt_rgg_generate_ranks(int64_t nv, int nprocs, uint32_t seed, double radius, int64_t* offsets,
+                          int64_t* tails, int64_t cap, int64_t* ne);
 double xt_rgg_radius(int64_t nv);
 /* vite binary file (graph.hpp:294-395); layout32 selects the USE_32_BIT_GRAPH
  * layout.  Header only when offsets == NULL; header and offsets (rebased to

@@ -8,6 +8,7 @@
 #include "oracle.h"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -92,6 +93,146 @@ extern "C" int64_t or_rgg_lcg(int64_t n, uint32_t seed, double radius, int64_t* 
     for (int64_t i = 0; i < n; i++)
       for (int64_t j : up[i]) tails[cur[i]++] = j;
   }
+  return ne;
+}
+
+// ---------------------------------------------------------------------------
+// The RGG of a P-rank run (graph.hpp:745-1374 with utils.hpp:245-430), all
+// ranks' rows in one CSR.  Follows the reference step by step: each rank's
+// LCG stream starts from the parallel prefix of 2x2 matrix powers computed in
+// wrapping 64-bit integer arithmetic (mat_power does not reduce mod MLCG,
+// utils.hpp:271-303, so the products overflow exactly as the reference's do),
+// then the local pair loop and the two cross-rank loops, whose inner index
+// runs only over j > i (graph.hpp:983-1066).
+
+typedef uint64_t Mat2[4];  // row-major 2x2, wrapping arithmetic
+
+static void mat_mul_wrap(Mat2 c, const Mat2 a, const Mat2 b) {
+  // utils.hpp:271-283 matmat_2x2
+  Mat2 t;
+  for (int i = 0; i < 2; i++)
+    for (int j = 0; j < 2; j++) {
+      uint64_t sum = 0;
+      for (int k = 0; k < 2; k++) sum += a[i * 2 + k] * b[k * 2 + j];
+      t[i * 2 + j] = sum;
+    }
+  std::memcpy(c, t, sizeof(Mat2));
+}
+
+static void or_rank_first_numbers(int64_t len, int nprocs, int64_t x0, std::vector<int64_t>& first) {
+  // utils.hpp:248-268 (x0 broadcast) and 308-348 (parallel_prefix_op), all
+  // ranks simulated together: mat_power multiplies by M (k - 1) times.
+  std::vector<std::array<uint64_t, 4>> glob(nprocs), pref(nprocs);
+  Mat2 m = {(uint64_t)kAlcg, 0, (uint64_t)kBlcg, 1};
+  Mat2 g;
+  std::memcpy(g, m, sizeof(Mat2));
+  for (int64_t p = 0; p < len - 1; p++) mat_mul_wrap(g, g, m);
+  for (int r = 0; r < nprocs; r++) {
+    std::memcpy(glob[r].data(), g, sizeof(Mat2));
+    pref[r] = {1, 0, 0, 1};
+  }
+  const int steps = (int)(std::log2((double)nprocs));
+  for (int s = 0; s < steps; s++) {
+    std::vector<std::array<uint64_t, 4>> recv(nprocs);
+    for (int r = 0; r < nprocs; r++) recv[r] = glob[r ^ (1 << s)];  // MPI_Sendrecv with the mate
+    for (int r = 0; r < nprocs; r++) {
+      mat_mul_wrap(glob[r].data(), glob[r].data(), recv[r].data());
+      if ((r ^ (1 << s)) < r) mat_mul_wrap(pref[r].data(), pref[r].data(), recv[r].data());
+    }
+  }
+  first.assign(nprocs, 0);
+  for (int r = 0; r < nprocs; r++) {
+    if (r == 0) {
+      first[r] = x0;
+    } else {
+      const int64_t t = (int64_t)((uint64_t)x0 * pref[r][0] + pref[r][2]);
+      first[r] = t % kMlcg;  // C++ remainder: the sign of t
+    }
+  }
+}
+
+extern "C" void or_rgg_lcg_coords_ranks(int64_t n, int nprocs, uint32_t seed, double* X, double* Y) {
+  // Coordinates of all ranks, rank r's points at [r*n/P, (r+1)*n/P):
+  // utils.hpp:352-394 (generate), 399-420 (rescale to [r/P, (r+1)/P)),
+  // graph.hpp:836-872.
+  const int64_t nl = n / nprocs, len = 2 * nl;
+  std::vector<int64_t> first;
+  or_rank_first_numbers(len, nprocs, (int64_t)or_reseeder(seed), first);
+  const double mult = 1.0 / (double)(1.0 + (double)(kMlcg - 1));
+  const double rec_np = (double)(1.0 / (double)nprocs);
+  const double range = (1.0 / (double)nprocs);
+  std::vector<int64_t> r(len);
+  for (int p = 0; p < nprocs; p++) {
+    r[0] = first[p];
+    for (int64_t i = 1; i < len; i++) r[i] = (r[i - 1] * kAlcg + kBlcg) % kMlcg;
+    const double lo = p * rec_np;
+    for (int64_t i = 0; i < nl; i++) X[p * nl + i] = (double)((double)std::fabs((double)r[i]) * mult);
+    for (int64_t j = 0; j < nl; j++) {
+      const double d = (double)((double)std::fabs((double)r[nl + j]) * mult);
+      Y[p * nl + j] = lo + (double)(range * d);
+    }
+  }
+}
+
+extern "C" int64_t or_rgg_lcg_ranks(int64_t n, int nprocs, uint32_t seed, double radius, int64_t* off,
+                                    int64_t* tails, int64_t cap, int nthreads) {
+  // graph.hpp:919-948 (local pairs), 983-1066 (cross pairs with the rank
+  // above and below: j on the neighbour runs over j > i only), 1073-1097 (the
+  // received cross edges), 1289-1341 (rows sorted by neighbour id).
+  if (nprocs < 1 || (nprocs & (nprocs - 1)) != 0 || n % nprocs != 0) return -1;
+  const int64_t nl = n / nprocs;
+  std::vector<double> X(n), Y(n);
+  or_rgg_lcg_coords_ranks(n, nprocs, seed, X.data(), Y.data());
+  const double rn = radius > 0 ? radius : or_rgg_radius_ref(n);
+  if (!((1.0 / (double)nprocs) > rn)) return -1;  // graph.hpp:793 assert
+  std::vector<std::vector<int64_t>> rows(n);
+  if (nthreads < 1) nthreads = 1;
+  for (int p = 0; p < nprocs; p++) {
+    const int64_t b = p * nl;
+    // local pairs: both entries
+    std::vector<std::vector<int64_t>> up(nl);
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads)
+    for (int64_t i = 0; i < nl; i++)
+      for (int64_t j = i + 1; j < nl; j++) {
+        double dx = X[b + i] - X[b + j];
+        double dy = Y[b + i] - Y[b + j];
+        double ed = std::sqrt(dx * dx + dy * dy);
+        if (ed <= rn) up[i].push_back(j);
+      }
+    for (int64_t i = 0; i < nl; i++)
+      for (int64_t j : up[i]) {
+        rows[b + i].push_back(b + j);
+        rows[b + j].push_back(b + i);
+      }
+    // cross pairs with the rank above (p - 1) and below (p + 1): an entry at
+    // the local vertex and the mirrored entry sent to the neighbour
+    for (int q : {p - 1, p + 1}) {
+      if (q < 0 || q >= nprocs) continue;
+      const int64_t c = q * nl;
+      std::vector<std::vector<int64_t>> hit(nl);
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads)
+      for (int64_t i = 0; i < nl; i++)
+        for (int64_t j = i + 1; j < nl; j++) {
+          double dx = X[b + i] - X[c + j];
+          double dy = Y[b + i] - Y[c + j];
+          double ed = std::sqrt(dx * dx + dy * dy);
+          if (ed <= rn) hit[i].push_back(j);
+        }
+      for (int64_t i = 0; i < nl; i++)
+        for (int64_t j : hit[i]) {
+          rows[b + i].push_back(c + j);
+          rows[c + j].push_back(b + i);
+        }
+    }
+  }
+  off[0] = 0;
+  for (int64_t i = 0; i < n; i++) {
+    std::sort(rows[i].begin(), rows[i].end());
+    off[i + 1] = off[i] + (int64_t)rows[i].size();
+  }
+  const int64_t ne = off[n];
+  if (tails && ne <= cap)
+    for (int64_t i = 0; i < n; i++) std::copy(rows[i].begin(), rows[i].end(), tails + off[i]);
   return ne;
 }
 

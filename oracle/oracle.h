@@ -16,6 +16,9 @@
  *   or_count_degree_ordered equality with or_count_reference (global + per-vertex)
  *                           and with the closed forms above.
  *   or_rgg_lcg              golden edge and triangle counts from the reference binary.
+ *   or_rgg_lcg_ranks        the CSR (sha256) the reference's generator builds on P
+ *                           MPI ranks (oracle/_ref/ref_rgg_ranks under mpiexec); at
+ *                           P = 1 equal to or_rgg_lcg.
  *   or_vite_read/write      byte-layout round trip and reading by the reference binary.
  *   or_partition_greedy /   bins of the reference's own routines (graph.hpp:397-602,
  *   or_partition_linear     utils.hpp:46-135) compiled from its headers (oracle/_ref/
@@ -48,6 +51,18 @@ void or_rgg_lcg_coords(int64_t n, uint32_t seed, double* X, double* Y);
  * entries ne; fills off[n+1] always and tails[0..ne) when ne <= cap. */
 int64_t or_rgg_lcg(int64_t n, uint32_t seed, double radius, int64_t* off,
                    int64_t* tails, int64_t cap, int nthreads);
+
+/* LCG coordinates of a P-rank run (utils.hpp:245-430, graph.hpp:836-872): the
+ * parallel prefix in wrapping int64 arithmetic, rank r's points at
+ * [r*n/P, (r+1)*n/P) with Y in [r/P, (r+1)/P). */
+void or_rgg_lcg_coords_ranks(int64_t n, int nprocs, uint32_t seed, double* X, double* Y);
+
+/* O(n^2/P) RGG of a P-rank run (graph.hpp:919-1097, 1289-1341): local pairs,
+ * cross pairs with the neighbouring ranks (inner index j > i), one CSR with
+ * global ids.  Returns ne, or -1 when P is not a power of two, does not divide
+ * n or the radius is not below 1/P. */
+int64_t or_rgg_lcg_ranks(int64_t n, int nprocs, uint32_t seed, double radius, int64_t* off,
+                         int64_t* tails, int64_t cap, int nthreads);
 
 /* Reference CPU count (xxtric.hpp:60-208, check_edgelist 516-545, pruning
  * 569-588) over vertices vstart, vstart+vstride, ... .  per_vertex (nullable)

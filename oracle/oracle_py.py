@@ -35,6 +35,8 @@ def lib():
             "or_rgg_radius_ref": (_dbl, [_i64]),
             "or_rgg_lcg_coords": (None, [_i64, _u32, _p, _p]),
             "or_rgg_lcg": (_i64, [_i64, _u32, _dbl, _p, _p, _i64, _int]),
+            "or_rgg_lcg_coords_ranks": (None, [_i64, _int, _u32, _p, _p]),
+            "or_rgg_lcg_ranks": (_i64, [_i64, _int, _u32, _dbl, _p, _p, _i64, _int]),
             "or_count_reference": (_u64, [_i64, _p, _p, _p, _int, _i64, _i64]),
             "or_count_reference32": (_u64, [_i64, _p, _p, _p, _int, _i64, _i64]),
             "or_count_degree_ordered": (_u64, [_i64, _p, _p, _p, _int]),
@@ -89,6 +91,25 @@ def rgg_lcg(n, seed=1, radius=0.0, nthreads=None):
     ne = lib().or_rgg_lcg(n, seed, radius, _ptr(off), None, 0, nt)
     tails = np.empty(ne, np.int64)
     lib().or_rgg_lcg(n, seed, radius, _ptr(off), _ptr(tails), ne, nt)
+    return off, tails
+
+
+def rgg_lcg_coords_ranks(n, nprocs, seed=1):
+    X = np.empty(n, np.float64)
+    Y = np.empty(n, np.float64)
+    lib().or_rgg_lcg_coords_ranks(n, nprocs, seed, _ptr(X), _ptr(Y))
+    return X, Y
+
+
+def rgg_lcg_ranks(n, nprocs, seed=1, radius=0.0, nthreads=None):
+    """(off, tails) of the RGG a run on nprocs MPI ranks generates (all ranks' rows)."""
+    nt = nthreads or nthreads_default()
+    off = np.empty(n + 1, np.int64)
+    ne = lib().or_rgg_lcg_ranks(n, nprocs, seed, radius, _ptr(off), None, 0, nt)
+    if ne < 0:
+        raise ValueError("nprocs must be a power of two dividing n, with radius below 1/nprocs")
+    tails = np.empty(ne, np.int64)
+    lib().or_rgg_lcg_ranks(n, nprocs, seed, radius, _ptr(off), _ptr(tails), ne, nt)
     return off, tails
 
 
@@ -216,6 +237,8 @@ def batch_params(base_seed, g, nmin=200, nmax=20000):
 
 
 REF_PARTITION_BIN = os.path.join(_HERE, "_ref", "ref_partition")
+REF_RGG_RANKS_BIN = os.path.join(_HERE, "_ref", "ref_rgg_ranks")
+MPIEXEC = "/opt/conda/bin/mpiexec"
 
 
 def partition(off, nparts, method="greedy"):

@@ -13,6 +13,11 @@ built from main.cpp/xxtric.hpp/graph.hpp where they lie):
     from oracle/_ref/ref_partition (our driver over the reference's
     BinaryEdgeList routines, graph.hpp:397-602) on fixture offsets.
     `python tests/golden/make_golden.py --partitions` refreshes only these.
+  * rgg_ranks: the RGG the reference's generator builds on P MPI ranks, from
+    `mpiexec -n P oracle/_ref/ref_rgg_ranks N FILE` (our driver over
+    GenerateRGG, graph.hpp:745-1097): its CSR sha256 and entry count, and the
+    triangles `tric_ref -f` counts on it (the reference's own distributed count
+    segfaults at P > 1 in this container).  `--ranks` refreshes only these.
   * vite inputs read with `tric_ref -f FILE`: R-MAT and Erdos-Renyi graphs of
     this build's own generators and amazon0302 (cugraph/example-inputs), each
     with a sha256 of its CSR so a test can tell that the same graph is rebuilt.
@@ -125,9 +130,36 @@ def partitions():
     return out
 
 
+def rgg_ranks():
+    out = []
+    with tempfile.TemporaryDirectory() as tmp:
+        for n, p in ((4096, 2), (4096, 4), (4096, 8), (65536, 2), (65536, 4), (100000, 2), (100000, 4),
+                     (100000, 8), (100000, 32)):
+            path = os.path.join(tmp, "ranks.bin")
+            subprocess.run([O.MPIEXEC, "-n", str(p), O.REF_RGG_RANKS_BIN, str(n), path], check=True,
+                           capture_output=True, timeout=3600)
+            a = np.fromfile(path, np.int64)
+            nv, ne = int(a[0]), int(a[1])
+            off, tails = a[2:3 + nv], a[3 + nv:]
+            assert nv == n and len(tails) == ne and off[-1] == ne
+            tri = ref_on_csr(off, tails, tmp, "ranks")
+            out.append({"n": n, "nprocs": p, "seed": 1, "ne": ne, "triangles": tri,
+                        "csr_sha256": csr_sha256(off, tails)})
+            print("rgg ranks", n, p, ne, tri, flush=True)
+    return out
+
+
 def main():
     if not os.path.exists(REF):
         sys.exit("build the reference first: make -C oracle ref")
+    if sys.argv[1:] == ["--ranks"]:  # refresh only the multi-rank generator fixtures
+        with open(os.path.join(HERE, "golden.json")) as f:
+            g = json.load(f)
+        g["rgg_ranks"] = rgg_ranks()
+        g["reference_rgg_ranks_binary"] = "oracle/_ref/ref_rgg_ranks"
+        with open(os.path.join(HERE, "golden.json"), "w") as f:
+            json.dump(g, f, indent=1)
+        return
     if sys.argv[1:] == ["--partitions"]:  # refresh only the partition bins
         with open(os.path.join(HERE, "golden.json")) as f:
             g = json.load(f)
@@ -179,6 +211,8 @@ def main():
             print("amazon", len(off) - 1, off[-1], tri, flush=True)
     g["partitions"] = partitions()
     g["reference_partition_binary"] = "oracle/_ref/ref_partition"
+    g["rgg_ranks"] = rgg_ranks()
+    g["reference_rgg_ranks_binary"] = "oracle/_ref/ref_rgg_ranks"
     with open(os.path.join(HERE, "golden.json"), "w") as f:
         json.dump(g, f, indent=1)
 

@@ -12,7 +12,7 @@ import pytest
 import oracle_py as O
 import xxtric_amd
 from helpers import csr_sha256
-from xxtric_amd import capi
+from xxtric_amd import XtError, capi
 
 
 def test_library_exports_every_declared_function():
@@ -52,6 +52,29 @@ def test_rgg_cell_grid_other_seeds_and_radii(n, seed, radius):
     o1, t1 = xxtric_amd.rgg_generate(n, seed=seed, radius=radius)
     o2, t2 = O.rgg_lcg(n, seed=seed, radius=radius)
     assert (o1 == o2).all() and (t1 == t2).all()
+
+
+@pytest.mark.parametrize("idx", range(9))
+def test_rgg_ranks_matches_reference_generator(golden, idx):
+    """xt_rgg_generate_ranks (cell grid) equals the reference's P-rank generator."""
+    g = golden["rgg_ranks"][idx]
+    off, tails = xxtric_amd.rgg_generate(g["n"], seed=g["seed"], nprocs=g["nprocs"])
+    assert off[-1] == g["ne"] and csr_sha256(off, tails) == g["csr_sha256"]
+
+
+@pytest.mark.parametrize("n,p,seed", [(6000, 2, 5), (8192, 16, 1), (3000, 8, 9)])
+def test_rgg_ranks_equals_oracle_other_cases(n, p, seed):
+    o1, t1 = xxtric_amd.rgg_generate(n, seed=seed, nprocs=p)
+    o2, t2 = O.rgg_lcg_ranks(n, p, seed=seed)
+    assert np.array_equal(o1, o2) and np.array_equal(t1, t2)
+
+
+@pytest.mark.parametrize("n,p", [(4096, 3), (4098, 4), (4096, 256), (4096, 0)])
+def test_rgg_ranks_rejects_what_the_reference_aborts_on(n, p):
+    """graph.hpp:765-793: ranks a power of two dividing n, strips wider than the radius."""
+    with pytest.raises(XtError) as e:
+        xxtric_amd.rgg_generate(n, nprocs=p)
+    assert e.value.code == capi.XT_ERR_ARG
 
 
 def test_rgg_radius_matches_oracle():

@@ -36,6 +36,49 @@ def test_rgg_lcg_matches_reference_binary(golden, idx):
     assert O.count_reference(off, tails) == g["triangles"]
 
 
+def _ranks_cases(golden, nmax):
+    return [i for i, g in enumerate(golden["rgg_ranks"]) if g["n"] <= nmax]
+
+
+@pytest.mark.parametrize("idx", range(9))
+def test_rgg_lcg_ranks_matches_reference_generator(golden, idx):
+    """The P-rank RGG restated (LCG parallel prefix in wrapping int64, per-rank
+    strips, cross-rank loops over j > i) equals, byte for byte, the CSR the
+    reference's GenerateRGG builds under mpiexec -n P; the reference binary's
+    count of it is the golden count."""
+    g = golden["rgg_ranks"][idx]
+    if g["n"] > 65536 and g["nprocs"] < 8:
+        pytest.skip("O(n^2/P) restatement kept to the smaller fixtures")
+    off, tails = O.rgg_lcg_ranks(g["n"], g["nprocs"], seed=g["seed"])
+    assert off[-1] == g["ne"]
+    assert csr_sha256(off, tails) == g["csr_sha256"]
+    assert O.count_degree_ordered(off, tails) == g["triangles"]
+
+
+def test_rgg_lcg_ranks_one_rank_is_the_serial_generator():
+    o1, t1 = O.rgg_lcg_ranks(4096, 1)
+    o2, t2 = O.rgg_lcg(4096)
+    assert np.array_equal(o1, o2) and np.array_equal(t1, t2)
+
+
+def test_rgg_lcg_ranks_first_numbers_wrap():
+    """Rank r > 0 does not continue the serial stream: the reference's matrix
+    powers are not reduced mod 2^31-1 and wrap in int64, so rank 1's first
+    coordinate differs from the serial stream's number 2n/P (which a reduced
+    power would give)."""
+    n, p = 4096, 2
+    Xs, _ = O.rgg_lcg_coords(n)          # serial stream: X = numbers 0..n-1
+    Xr, Yr = O.rgg_lcg_coords_ranks(n, p)
+    assert np.array_equal(Xr[:n // p], Xs[:n // p])
+    # the serial continuation of rank 0's 2n/P numbers
+    x = O.reseeder(1)
+    for _ in range(2 * (n // p)):
+        x = (x * 16807) % 2147483647
+    assert Xr[n // p] != x / 2147483647.0
+    assert 0.0 <= Xr.min() and Xr.max() < 1.0
+    assert (Yr[:n // p] < 0.5).all() and (Yr[n // p:] >= 0.5).all()
+
+
 def test_rgg_radius_reference_formula():
     # graph.hpp:789-791 at n = 1e5 (PI = 3.14159)
     assert abs(O.rgg_radius_ref(100000) - 5.3037e-3) < 1e-6

@@ -313,18 +313,20 @@ def rgg_radius(n):
     return capi.lib().xt_rgg_radius(n)
 
 
-def rgg_generate(n, seed=1, radius=0.0):
-    """Reference RGG (graph.hpp:745-1374, LCG mode): (off int64[n+1], tails int64)."""
+def rgg_generate(n, seed=1, radius=0.0, nprocs=1):
+    """Reference RGG (graph.hpp:745-1374, LCG mode): (off int64[n+1], tails int64).
+    nprocs > 1: the graph a run on that many MPI ranks generates, all rows in one CSR."""
     L = capi.lib()
     off = np.empty(n + 1, np.int64)
     ne = ctypes.c_int64()
-    rc = L.xt_rgg_generate(n, seed, radius, capi.ptr(off), None, 0, ctypes.byref(ne))
+    rc = L.xt_rgg_generate_ranks(n, nprocs, seed, radius, capi.ptr(off), None, 0, ctypes.byref(ne))
     if rc != capi.XT_OK:
-        raise XtError(rc, "xt_rgg_generate")
+        raise XtError(rc, "xt_rgg_generate_ranks")
     tails = np.empty(ne.value, np.int64)
-    rc = L.xt_rgg_generate(n, seed, radius, capi.ptr(off), capi.ptr(tails), ne.value, ctypes.byref(ne))
+    rc = L.xt_rgg_generate_ranks(n, nprocs, seed, radius, capi.ptr(off), capi.ptr(tails), ne.value,
+                                 ctypes.byref(ne))
     if rc != capi.XT_OK:
-        raise XtError(rc, "xt_rgg_generate")
+        raise XtError(rc, "xt_rgg_generate_ranks")
     return off, tails
 
 

@@ -109,6 +109,7 @@ _SIGS = {
     "xt_graph_stats_get": (_int, [_p, _p, _u32, ctypes.POINTER(XtGraphStats)]),
     "xt_graph_destroy": (_int, [_p, _p]),
     "xt_rgg_generate": (_int, [_i64, _u32, _dbl, _p, _p, _i64, _p]),
+    "xt_rgg_generate_ranks": (_int, [_i64, _int, _u32, _dbl, _p, _p, _i64, _p]),
     "xt_rgg_radius": (_dbl, [_i64]),
     "xt_vite_read": (_int, [ctypes.c_char_p, _int, _p, _p, _p, _p]),
     "xt_vite_write": (_int, [ctypes.c_char_p, _int, _i64, _p, _p]),

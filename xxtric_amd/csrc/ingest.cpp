@@ -1,7 +1,8 @@
 // ingest.cpp -- host-side graph ingest of libxtric (no device work).
 //
-//  * RGG of the reference (graph.hpp:745-1374, one rank, LCG mode): same LCG
-//    stream (utils.hpp:245-430), same radius (graph.hpp:789-791), same
+//  * RGG of the reference (graph.hpp:745-1374, LCG mode, one rank or the
+//    graph a run on P ranks builds): same LCG streams (utils.hpp:245-430),
+//    same radius (graph.hpp:789-791), same
 //    predicate sqrt(dx*dx + dy*dy) <= r evaluated with the same operand order
 //    (graph.hpp:922-926).  The reference tests all n^2/2 pairs; here points
 //    are binned into square cells of side slightly above r and only the 3x3
@@ -35,18 +36,64 @@ uint32_t seed_seq_first(uint32_t seed) {
   return out[0];
 }
 
-void lcg_coords(int64_t n, uint32_t seed, std::vector<double>& X, std::vector<double>& Y) {
+// 2x2 integer matrices (row-major) in wrapping 64-bit arithmetic.
+struct M2 {
+  uint64_t a[4];
+};
+M2 mul_wrap(const M2& x, const M2& y) {
+  M2 c;
+  c.a[0] = x.a[0] * y.a[0] + x.a[1] * y.a[2];
+  c.a[1] = x.a[0] * y.a[1] + x.a[1] * y.a[3];
+  c.a[2] = x.a[2] * y.a[0] + x.a[3] * y.a[2];
+  c.a[3] = x.a[2] * y.a[1] + x.a[3] * y.a[3];
+  return c;
+}
+M2 pow_wrap(M2 m, uint64_t k) {
+  M2 r = {{1, 0, 0, 1}};
+  for (; k; k >>= 1) {
+    if (k & 1) r = mul_wrap(r, m);
+    m = mul_wrap(m, m);
+  }
+  return r;
+}
+
+// First LCG number of rank r of a P-rank run.  The reference gives each rank
+// the step matrix M = [[a, 0], [b, 1]] raised to its 2n/P numbers, multiplied
+// without reduction mod 2^31-1 (utils.hpp:271-303), and combines the ranks'
+// matrices by a parallel prefix (utils.hpp:308-340), so rank r starts from
+// x0 * (M^(2n/P))^r in wrapping int64 arithmetic, reduced by C++'s signed
+// remainder (utils.hpp:343-347).  The ranks' matrices are all powers of M,
+// which commute, so the power is taken directly.
+int64_t rank_first(int64_t x0, int64_t len, int r) {
+  if (r == 0) return x0;
+  const M2 step = {{(uint64_t)kAlcg, 0, 0, 1}};
+  const M2 p = pow_wrap(pow_wrap(step, (uint64_t)len), (uint64_t)r);
+  const int64_t t = (int64_t)((uint64_t)x0 * p.a[0] + p.a[2]);
+  return t % kMlcg;
+}
+
+// LCG coordinates of a P-rank run, rank r's points at [r*n/P, (r+1)*n/P):
+// X from the first n/P numbers of its stream, Y from the next n/P rescaled to
+// [r/P, (r+1)/P) (utils.hpp:352-420, graph.hpp:836-872).
+void lcg_coords(int64_t n, int nprocs, uint32_t seed, std::vector<double>& X, std::vector<double>& Y) {
   X.resize(n);
   Y.resize(n);
-  int64_t r = (int64_t)seed_seq_first(seed);
+  const int64_t nl = n / nprocs, len = 2 * nl;
+  const int64_t x0 = (int64_t)seed_seq_first(seed);
   const double mult = 1.0 / (double)(1.0 + (double)(kMlcg - 1));
-  for (int64_t i = 0; i < 2 * n; i++) {
-    if (i > 0) r = (r * kAlcg) % kMlcg;
-    const double d = (double)((double)std::fabs((double)r) * mult);
-    if (i < n)
-      X[i] = d;
-    else
-      Y[i - n] = 0.0 + (double)(1.0 * d);
+  const double rec_np = (double)(1.0 / (double)nprocs);
+  const double range = (1.0 / (double)nprocs);
+  for (int p = 0; p < nprocs; p++) {
+    int64_t r = rank_first(x0, len, p);
+    const double lo = p * rec_np;
+    for (int64_t i = 0; i < len; i++) {
+      if (i > 0) r = (r * kAlcg) % kMlcg;
+      const double d = (double)((double)std::fabs((double)r) * mult);
+      if (i < nl)
+        X[p * nl + i] = d;
+      else
+        Y[p * nl + i - nl] = lo + (double)(range * d);
+    }
   }
 }
 
@@ -88,13 +135,17 @@ double xt_rgg_radius(int64_t nv) {
   return (rc + rt) / (double)2.0;
 }
 
-int xt_rgg_generate(int64_t nv, uint32_t seed, double radius, int64_t* offsets, int64_t* tails,
-                    int64_t cap, int64_t* ne_out) {
+int xt_rgg_generate_ranks(int64_t nv, int nprocs, uint32_t seed, double radius, int64_t* offsets,
+                          int64_t* tails, int64_t cap, int64_t* ne_out) {
   return xt_guarded([&]() -> int {
     if (nv <= 0 || !offsets || !ne_out) return XT_ERR_ARG;
-    std::vector<double> X, Y;
-    lcg_coords(nv, seed, X, Y);
+    // graph.hpp:765-793: a power-of-two rank count dividing nv, strips wider than r
+    if (nprocs < 1 || (nprocs & (nprocs - 1)) != 0 || nv % nprocs != 0) return XT_ERR_ARG;
     const double r = radius > 0 ? radius : xt_rgg_radius(nv);
+    if (!((1.0 / (double)nprocs) > r)) return XT_ERR_ARG;
+    const int64_t nl = nv / nprocs;
+    std::vector<double> X, Y;
+    lcg_coords(nv, nprocs, seed, X, Y);
     double minx = X[0], maxx = X[0], miny = Y[0], maxy = Y[0];
     for (int64_t i = 1; i < nv; i++) {
       minx = std::min(minx, X[i]);
@@ -137,6 +188,10 @@ int xt_rgg_generate(int64_t nv, uint32_t seed, double radius, int64_t* offsets, 
             for (int64_t k = cstart[c]; k < cstart[c + 1]; k++) {
               const int64_t j = order[k];
               if (j == i) continue;
+              // Between ranks the reference tests only pairs whose local
+              // indices differ (its cross loops run j > i over the
+              // neighbour's points and mirror each hit, graph.hpp:983-1066).
+              if (nprocs > 1 && i / nl != j / nl && i % nl == j % nl) continue;
               const int64_t p = std::min(i, j), q = std::max(i, j);
               const double dx = X[p] - X[q];
               const double dy = Y[p] - Y[q];
@@ -154,6 +209,11 @@ int xt_rgg_generate(int64_t nv, uint32_t seed, double radius, int64_t* offsets, 
     for (int64_t i = 0; i < nv; i++) std::copy(rows[i].begin(), rows[i].end(), tails + offsets[i]);
   return XT_OK;
   });
+}
+
+int xt_rgg_generate(int64_t nv, uint32_t seed, double radius, int64_t* offsets, int64_t* tails,
+                    int64_t cap, int64_t* ne_out) {
+  return xt_rgg_generate_ranks(nv, 1, seed, radius, offsets, tails, cap, ne_out);
 }
 
 }  // extern "C"

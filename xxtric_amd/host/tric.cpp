@@ -16,6 +16,9 @@
 //   -t K      time K counts (the reference times one)
 //   -B K      with -f: print the vertex bins of a balanced read over K ranks
 //             (greedy, or linear partitioning with -d)
+//   -N P      with -n: generate the graph the reference builds on P MPI ranks
+//             (`mpiexec -n P`: per-rank LCG streams and strips, cross-rank
+//             edges as its exchange leaves them), all of it counted here
 #include <unistd.h>
 
 #include <chrono>
@@ -38,6 +41,7 @@ struct Opts {
   int rmat = 0, ef = 16, device = 0, reps = 1;
   int balanced = 0;   // 1: -b (greedy bins), 2: -d (linear partitioning)
   int bin_ranks = 0;  // -B K
+  int gen_ranks = 1;  // -N P
   uint64_t seed = 1;
   bool seed_set = false;
   uint32_t flags = XT_ORIENT_DEGREE;
@@ -51,13 +55,14 @@ struct Opts {
 Opts parse(int argc, char* argv[]) {
   Opts o;
   int c;
-  while ((c = getopt(argc, argv, "f:r:n:p:olbds:R:e:S:Pm:g:v:w:xLjt:B:")) != -1) {
+  while ((c = getopt(argc, argv, "f:r:n:p:olbds:R:e:S:Pm:g:v:w:xLjt:B:N:")) != -1) {
     switch (c) {
       case 'f': o.file = optarg; break;
       case 'b': o.balanced = 1; break;
       case 'd': o.balanced = 2; break;
       case 'o': break;
       case 'B': o.bin_ranks = atoi(optarg); break;
+      case 'N': o.gen_ranks = atoi(optarg); break;
       case 'r': case 's': break;             // ranks per node / buffer size: no meaning at one process
       case 'n': o.nv = atol(optarg); break;
       case 'l': o.lcg = true; break;
@@ -104,7 +109,7 @@ int main(int argc, char* argv[]) {
       if (xt_graph_rmat(ctx, o.rmat, o.ef, 0.57, 0.19, 0.19, o.seed, o.permute, &dg) != XT_OK)
         die(std::string("R-MAT generation failed: ") + xt_last_error(ctx));
     } else if (o.nv > 0) {
-      GenerateRGG gr(o.nv);
+      GenerateRGG gr(o.nv, 0.0, o.gen_ranks);
       g = gr.generate(o.lcg, true, o.pct, o.seed_set ? (uint32_t)o.seed : 0);
     } else {
       BinaryEdgeList rm;

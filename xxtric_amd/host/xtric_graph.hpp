@@ -142,10 +142,19 @@ class BinaryEdgeList {         // graph.hpp:279-395 (read), one process
   }
 };
 
-class GenerateRGG {            // graph.hpp:745-1374, one process
+class GenerateRGG {            // graph.hpp:745-1374
  public:
-  explicit GenerateRGG(GraphElem nv, double radius = 0.0) : nv_(nv), radius_(radius) {
+  // nprocs > 1: the graph the reference generates on that many MPI ranks
+  // (its constructor's checks, graph.hpp:765-793), all ranks' rows here.
+  explicit GenerateRGG(GraphElem nv, double radius = 0.0, int nprocs = 1)
+      : nv_(nv), radius_(radius), nprocs_(nprocs) {
     rn_ = radius > 0 ? radius : xt_rgg_radius(nv);
+    if (nprocs_ < 1 || (nprocs_ & (nprocs_ - 1)) != 0)
+      throw std::runtime_error("[ERROR] Number of processes must be a power of 2.");
+    if (nv_ % nprocs_ != 0)
+      throw std::runtime_error("[ERROR] Number of vertices must be perfectly divisible by number of processes.");
+    if (!((1.0 / (double)nprocs_) > rn_))
+      throw std::runtime_error("[ERROR] The radius must be below 1/(number of processes).");
   }
   // isLCG: the reference's seeded LCG stream (x0 = seed_seq{1}); otherwise a
   // distinct seeded stream (seed 2) -- the reference draws from random_device
@@ -157,10 +166,10 @@ class GenerateRGG {            // graph.hpp:745-1374, one process
     const uint32_t s = seed ? seed : (isLCG ? 1u : 2u);
     int64_t ne = 0;
     std::vector<int64_t> off(nv_ + 1);
-    if (xt_rgg_generate(nv_, s, radius_, off.data(), nullptr, 0, &ne) != XT_OK)
+    if (xt_rgg_generate_ranks(nv_, nprocs_, s, radius_, off.data(), nullptr, 0, &ne) != XT_OK)
       throw std::runtime_error("RGG generation failed");
     std::vector<int64_t> tails(ne);
-    xt_rgg_generate(nv_, s, radius_, off.data(), tails.data(), ne, &ne);
+    xt_rgg_generate_ranks(nv_, nprocs_, s, radius_, off.data(), tails.data(), ne, &ne);
     if (randomEdgePercent > 0.0) add_random_edges(off, tails, randomEdgePercent, s);
     return graph_from_csr(off, tails);
   }
@@ -194,4 +203,5 @@ class GenerateRGG {            // graph.hpp:745-1374, one process
   }
   GraphElem nv_;
   double radius_, rn_;
+  int nprocs_;
 };
