@@ -69,13 +69,13 @@ def rank0_slice(ctx, g, triangles, parts=8, steps=3):
     t0 = time.perf_counter()
     s = g.slice(lo, hi)
     slice_s = time.perf_counter() - t0
-    owned, local, _ = s.count_owned(lo, hi)  # warm-up
+    owned, local, _ = s.count_owned(lo, hi, total=False)  # warm-up
     ev = []
     for _ in range(steps):
-        o, l, _ = s.count_owned(lo, hi)
+        o, l, _ = s.count_owned(lo, hi, total=False)
         assert (o, l) == (owned, local)
         ev.append(ctx.timing())
-    whole = g.count_owned(lo, hi)[:2]
+    whole = g.count_owned(lo, hi, total=False)[:2]
     ms = float(np.median([e["total_ms"] for e in ev]))
     out = {"owned_range": [lo, hi], "parts": parts, "slice_entries": s.ne, "whole_entries": g.ne,
            "slice_build_s": round(slice_s, 3), "owned": owned, "local": local, "remote": owned - local,

@@ -196,7 +196,9 @@ int xt_graph_count(xt_ctx* ctx, xt_graph* g, uint32_t flags, uint64_t* total,
  * *owned, and into *local those whose middle vertex is owned too (the rank
  * counts these itself; the others it forwards as (v, w) pairs to the middle
  * vertex's rank, xxtric.hpp:189-204).  *total (nullable) gets every triangle
- * of g.  The count is the same on g and on its slice for [lo, hi).  flags
+ * of g, from a second, plain count.  The owned count skips edges whose smaller
+ * endpoint is below lo and streams, per edge, only the ids of the shorter
+ * list that can complete an owned triangle.  The count is the same on g and on its slice for [lo, hi).  flags
  * selects the orientation; XT_PER_VERTEX is rejected. */
 int xt_graph_slice(xt_ctx* ctx, const xt_graph* g, int64_t lo, int64_t hi, xt_graph** out);
 int xt_graph_count_owned(xt_ctx* ctx, xt_graph* g, int64_t lo, int64_t hi, uint32_t flags,

@@ -38,6 +38,18 @@ def test_rmat_s22_full_parity(ctx):
     verts = _sample(off, 4099, 3000)
     ref = O.count_reference_list(off, tails, verts)     # the reference algorithm itself
     assert (pv[verts] == ref).all()
+    # ranks of an 8-way 1-D split (pruned owned-range counts): the oracle's
+    # per-vertex counts summed over the owned range; rank 0 also on its slice
+    n = len(off) - 1
+    for r in (0, 3, 7):
+        lo, hi = n * r // 8, n * (r + 1) // 8
+        _say("rmat s22 rank %d of 8" % r)
+        owned, _, _ = g.count_owned(lo, hi, total=False)
+        assert owned == int(exp_pv[lo:hi].sum(dtype=np.uint64)), r
+        if r == 0:
+            sl = g.slice(lo, hi)
+            assert sl.count_owned(lo, hi, total=False)[0] == owned
+            sl.close()
     g.close()
 
 

@@ -232,17 +232,17 @@ class DeviceGraph:
                                                    ctypes.byref(total), capi.ptr(pv)))
         return (total.value, pv) if per_vertex else total.value
 
-    def count_owned(self, lo, hi, orient="degree"):
+    def count_owned(self, lo, hi, orient="degree", total=True):
         """Triangles whose minimum vertex lies in [lo, hi), and of those the ones
         whose middle vertex does too (one rank of a 1-D vertex split); also the
-        graph's total."""
+        graph's total (a second count; None with total=False)."""
         owned = ctypes.c_uint64()
         local = ctypes.c_uint64()
-        total = ctypes.c_uint64()
+        tot = ctypes.c_uint64()
         self.ctx._check(self.ctx._L.xt_graph_count_owned(self.ctx.h, self.h, int(lo), int(hi), _flags(orient),
                                                          ctypes.byref(owned), ctypes.byref(local),
-                                                         ctypes.byref(total)))
-        return owned.value, local.value, total.value
+                                                         ctypes.byref(tot) if total else None))
+        return owned.value, local.value, (tot.value if total else None)
 
     def slice(self, lo, hi):
         """Rows of [lo, hi) and of their neighbours (ghosts) kept, others emptied."""

@@ -245,11 +245,17 @@ template <typename T, bool NARROW, int CM, bool BATCH>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_light_items(
     int64_t nv, int64_t ne, const int64_t* __restrict__ off, const T* __restrict__ tails,
     const uint8_t* __restrict__ oflags, const int64_t* __restrict__ ooff, const uint64_t* __restrict__ vrec,
-    const uint16_t* __restrict__ dp16, const int32_t* __restrict__ ot, const int64_t* __restrict__ crows,
-    int light_cut, uint64_t* __restrict__ counters, uint32_t* __restrict__ itcnt, Item* __restrict__ items,
-    unsigned long long* __restrict__ pv, const int32_t* __restrict__ gid,
-    unsigned long long* __restrict__ gtotal, int64_t rlo, int64_t rhi) {
+    const uint4* __restrict__ vrange, const uint16_t* __restrict__ dp16, const int32_t* __restrict__ ot,
+    const int64_t* __restrict__ crows, int light_cut, uint64_t* __restrict__ counters,
+    uint32_t* __restrict__ itcnt, Item* __restrict__ items, unsigned long long* __restrict__ pv,
+    const int32_t* __restrict__ gid, unsigned long long* __restrict__ gtotal, int64_t rlo, int64_t rhi) {
   constexpr bool PV = CM == kCreditVertex;
+  // Owned-range counts of graphs below 2^32 entries are pruned: an edge whose
+  // smaller endpoint a is below lo closes no owned triangle and is skipped; an
+  // item streams only the ids w of its partner's list that can make the
+  // triangle owned -- w >= lo when a < hi, lo <= w < hi when a >= hi -- a
+  // contiguous piece of the sorted list whose bounds come with the record.
+  constexpr bool PRUNE = NARROW && CM == kCreditRange;
   __shared__ int64_t s_off[kOffCap + 1];  // input offsets of the chunk's rows
   __shared__ uint64_t s_red[kBlock / kWave];
   __shared__ int64_t s_scan[kBlock / kWave];
@@ -271,6 +277,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8))) 
   // Partner lists: base and length (32-bit each when NARROW).
   using LT = typename std::conditional<NARROW, uint32_t, int64_t>::type;
   LT gb[kPerThread], gl[kPerThread];
+  uint32_t nlo[kPerThread], nhi[kPerThread];  // PRUNE: ids of the partner's list below lo / below hi
   int64_t u0 = 0;
   if (e0 < ee) {
     {
@@ -286,7 +293,18 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8))) 
     const uint32_t fl = oflags[e0 / kPerThread];
     // 1. The eight neighbours' out-degrees (and, NARROW, their list offsets),
     //    eight gathers in flight.
-    if constexpr (NARROW) {
+    if constexpr (PRUNE) {
+      uint4 rec[kPerThread];  // one 16-byte gather: offset, out-degree, ids below lo, ids below hi
+#pragma unroll
+      for (int j = 0; j < kPerThread; j++) rec[j] = vrange[ids[j]];
+#pragma unroll
+      for (int j = 0; j < kPerThread; j++) {
+        gb[j] = rec[j].x;
+        gl[j] = rec[j].y;
+        nlo[j] = rec[j].z;
+        nhi[j] = rec[j].w;
+      }
+    } else if constexpr (NARROW) {
       uint64_t rec[kPerThread];
 #pragma unroll
       for (int j = 0; j < kPerThread; j++) rec[j] = vrec[ids[j]];
@@ -322,9 +340,23 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8))) 
         if (!((valid >> j) & 1u) || ids[j] == u) continue;  // out of range (unvalidated input) or self-loop
         const bool kept = (fl >> j) & 1u;
         const int64_t du = du_row, dv = (int64_t)gl[j];
+        const int64_t a = u < (int64_t)ids[j] ? u : (int64_t)ids[j];
         if (du > light_cut || dv > light_cut) {  // heavy: listed under the longer list's owner
-          if ((kept ? du >= dv : du > dv) && dv > 0) own |= 1u << j;  // an empty N+(v) closes nothing
-        } else if (kept && dv > 0) {  // light in-edges are counted from their source row
+          if ((kept ? du >= dv : du > dv) && dv > 0) {  // an empty N+(v) closes nothing
+            if constexpr (PRUNE) {
+              if (a >= rlo) {
+                const uint32_t s0 = nlo[j], s1 = a >= rhi ? nhi[j] : (uint32_t)dv;
+                if (s1 > s0) {
+                  gb[j] += s0;
+                  gl[j] = s1 - s0;
+                  own |= 1u << j;
+                }
+              }
+            } else {
+              own |= 1u << j;
+            }
+          }
+        } else if (kept && dv > 0 && (!PRUNE || a >= rlo)) {  // light in-edges are counted from their source row
           light |= 1u << j;
         }
       }
@@ -525,6 +557,21 @@ __global__ __launch_bounds__(256) void k_vertex_records(int64_t nv, const int64_
   for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < nv; x += stride) {
     const int64_t a = ooff[x], b = ooff[x + 1];
     vrec[x] = (uint64_t)(uint32_t)a | ((uint64_t)(uint32_t)(b - a) << 32);
+  }
+}
+
+// Owned-range records {ooff[x], d+(x), ids of N+(x) below lo, ids below hi}
+// (when |E+| < 2^32): the out-list is ascending, so the ids in [lo, hi) and
+// those >= lo are contiguous pieces of it.
+__global__ __launch_bounds__(256) void k_vertex_ranges(int64_t nv, const int64_t* __restrict__ ooff,
+                                                       const int32_t* __restrict__ ot, int64_t lo, int64_t hi,
+                                                       uint4* __restrict__ rec) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < nv; x += stride) {
+    const int64_t a = ooff[x], b = ooff[x + 1];
+    const int64_t pl = lower_bound_dev<int32_t>(ot, a, b, (int32_t)lo);  // lo <= hi <= nv < 2^31
+    const int64_t ph = lower_bound_dev<int32_t>(ot, pl, b, (int32_t)hi);
+    rec[x] = make_uint4((uint32_t)a, (uint32_t)(b - a), (uint32_t)(pl - a), (uint32_t)(ph - a));
   }
 }
 
@@ -1143,8 +1190,13 @@ int count_oriented(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne_bound, const 
   // every oriented offset and out-degree fits 32 bits (unless the wide layout is forced)
   const bool narrow = ne_bound < (1ll << 32) && !ctx->force_wide;
   uint64_t* vrec = nullptr;
+  uint4* vrange = nullptr;
   uint16_t* dp16 = nullptr;
-  if (narrow) {
+  if (narrow && range) {
+    XT_TRY(ensure(ctx, s.vrec, (size_t)nv * sizeof(uint4)));
+    vrange = s.vrec.as<uint4>();
+    hipLaunchKernelGGL(k_vertex_ranges, dim3((unsigned)g), dim3(256), 0, st, nv, ooff, ot, rlo, rhi, vrange);
+  } else if (narrow) {
     XT_TRY(ensure(ctx, s.vrec, (size_t)nv * sizeof(uint64_t)));
     vrec = s.vrec.as<uint64_t>();
     hipLaunchKernelGGL(k_vertex_records, dim3((unsigned)g), dim3(256), 0, st, nv, ooff, vrec);
@@ -1158,7 +1210,7 @@ int count_oriented(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne_bound, const 
 #define XT_LIGHT(N, C, B)                                                                           \
   hipLaunchKernelGGL((k_light_items<T, N, C, B>), dim3((unsigned)nchunks), dim3(kBlock), 0, st, nv,  \
                      ne_bound, in_off, in_tails, oflags, ooff, (const uint64_t*)vrec,              \
-                     (const uint16_t*)dp16, ot, (const int64_t*)s.crows.as<int64_t>(), cut,       \
+                     (const uint4*)vrange, (const uint16_t*)dp16, ot, (const int64_t*)s.crows.as<int64_t>(), cut, \
                      counters, s.itcnt.as<uint32_t>(), s.items.as<Item>(), pvp, gid, gt, rlo, rhi)
 #define XT_LIGHT_PB(N)                    \
   if (range)                              \
