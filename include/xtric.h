@@ -41,6 +41,10 @@ extern "C" {
 #define XT_PER_VERTEX 0x2u     /* also produce per-vertex counts (min-id credit) */
 #define XT_VALIDATE 0x4u       /* check rows strictly ascending and ids in range */
 #define XT_CHECK_SYMMETRY 0x8u /* with XT_VALIDATE: also check (u,v) => (v,u) */
+#define XT_CLEAN 0x10u         /* graph input (xt_graph_from_csr / _from_vite): rebuild it on the
+                                  device as a simple undirected graph -- rows sorted, duplicate
+                                  entries merged, self-loops dropped, missing reverse entries
+                                  added -- for files the reference's binary search would misread */
 
 typedef struct xt_ctx xt_ctx;
 typedef struct xt_graph xt_graph;
@@ -175,7 +179,10 @@ int xt_graph_from_edges(xt_ctx* ctx, int64_t nv, int64_t m, const int64_t* src, 
  * streamed into a resident graph: offsets rebased to 0 (graph.hpp:386-388)
  * and checked monotone, ids checked in [0, nv), read in 4 M-element blocks
  * through two pinned buffers (host memory stays bounded for s26-sized
- * files).  flags: XT_VALIDATE / XT_CHECK_SYMMETRY as for xt_graph_from_csr. */
+ * files).  flags: XT_CLEAN (applied first), then XT_VALIDATE /
+ * XT_CHECK_SYMMETRY as for xt_graph_from_csr.  The reference assumes sorted,
+ * unique rows and checks nothing; without XT_CLEAN, XT_VALIDATE rejects a
+ * file whose rows are not strictly ascending (XT_ERR_INPUT). */
 int xt_graph_from_vite(xt_ctx* ctx, const char* path, int layout32, uint32_t flags, xt_graph** out);
 /* xt_graph_from_edges, xt_graph_count and xt_graph_destroy in one call. */
 int xt_count_edges(xt_ctx* ctx, int64_t nv, int64_t m, const int64_t* src, const int64_t* dst, uint32_t flags,

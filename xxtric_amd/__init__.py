@@ -23,7 +23,7 @@ __all__ = ["Context", "DeviceGraph", "TriangleCountingSingle", "rgg_generate", "
            "Batch", "Server", "XtError"]
 
 
-def _flags(orient="degree", per_vertex=False, validate=False, symmetry=False):
+def _flags(orient="degree", per_vertex=False, validate=False, symmetry=False, clean=False):
     f = XT_ORIENT_ID if orient == "id" else XT_ORIENT_DEGREE
     if orient not in ("id", "degree"):
         raise ValueError("orient must be 'degree' or 'id'")
@@ -33,6 +33,8 @@ def _flags(orient="degree", per_vertex=False, validate=False, symmetry=False):
         f |= XT_VALIDATE
     if symmetry:
         f |= XT_VALIDATE | XT_CHECK_SYMMETRY
+    if clean:
+        f |= capi.XT_CLEAN
     return f
 
 
@@ -97,14 +99,16 @@ class Context:
                                          ctypes.byref(total), capi.ptr(pv)))
         return (total.value, pv) if per_vertex else total.value
 
-    def graph(self, off, tails, validate=False, symmetry=False):
+    def graph(self, off, tails, validate=False, symmetry=False, clean=False):
+        """Upload a CSR; clean=True rebuilds it on the device as a simple undirected
+        graph (rows sorted, duplicates merged, self-loops dropped, reverse entries added)."""
         off = np.ascontiguousarray(off, np.int64)
         tb = 4 if np.asarray(tails).dtype == np.int32 else 8
         tails = np.ascontiguousarray(tails, np.int32 if tb == 4 else np.int64)
         h = ctypes.c_void_p()
         self._check(self._L.xt_graph_from_csr(self.h, len(off) - 1, len(tails), capi.ptr(off),
                                               capi.ptr(tails), tb,
-                                              _flags(validate=validate, symmetry=symmetry),
+                                              _flags(validate=validate, symmetry=symmetry, clean=clean),
                                               ctypes.byref(h)))
         return DeviceGraph(self, h)
 
@@ -120,11 +124,13 @@ class Context:
                                                 ctypes.byref(h)))
         return DeviceGraph(self, h)
 
-    def graph_from_vite(self, path, layout32=False, validate=False, symmetry=False):
-        """A vite binary file streamed into a resident graph (bounded host memory)."""
+    def graph_from_vite(self, path, layout32=False, validate=False, symmetry=False, clean=False):
+        """A vite binary file streamed into a resident graph (bounded host memory);
+        clean=True as for graph()."""
         h = ctypes.c_void_p()
         self._check(self._L.xt_graph_from_vite(self.h, str(path).encode(), 1 if layout32 else 0,
-                                               _flags(validate=validate, symmetry=symmetry), ctypes.byref(h)))
+                                               _flags(validate=validate, symmetry=symmetry, clean=clean),
+                                               ctypes.byref(h)))
         return DeviceGraph(self, h)
 
     def count_edges(self, nv, src, dst, per_vertex=False, orient="degree"):

@@ -31,6 +31,7 @@ XT_ORIENT_ID = 0x1
 XT_PER_VERTEX = 0x2
 XT_VALIDATE = 0x4
 XT_CHECK_SYMMETRY = 0x8
+XT_CLEAN = 0x10
 
 XT_BATCH_PERSISTENT = 0
 XT_BATCH_HIPGRAPH = 1

@@ -505,8 +505,9 @@ int xt_graph_from_csr(xt_ctx* ctx, int64_t nv, int64_t ne, const int64_t* offset
       return hip_err(ctx, e, "hipMalloc(graph)");
     }
     int rc = upload_csr(ctx, nv, ne, offsets, tails, tails_bytes, g->off, g->tails, ctx->stream);
+    if (rc == XT_OK && (flags & XT_CLEAN)) rc = graph_clean(ctx, g);  // may change g->ne
     if (rc == XT_OK && (flags & XT_VALIDATE))
-      rc = validate<int32_t>(ctx, nv, ne, g->off, g->tails, (flags & XT_CHECK_SYMMETRY) != 0, ctx->stream);
+      rc = validate<int32_t>(ctx, g->nv, g->ne, g->off, g->tails, (flags & XT_CHECK_SYMMETRY) != 0, ctx->stream);
     if (rc != XT_OK) {
       (void)hipFree(g->off);
       (void)hipFree(g->tails);
@@ -695,6 +696,7 @@ int xt_graph_from_vite(xt_ctx* ctx, const char* path, int layout32, uint32_t fla
     xt_graph* g = new xt_graph();
     int rc = layout32 ? vite_to_device<int32_t, float>(ctx, f, g) : vite_to_device<int64_t, double>(ctx, f, g);
     std::fclose(f);
+    if (rc == XT_OK && (flags & XT_CLEAN)) rc = graph_clean(ctx, g);
     if (rc == XT_OK && (flags & XT_VALIDATE))
       rc = validate<int32_t>(ctx, g->nv, g->ne, g->off, g->tails, (flags & XT_CHECK_SYMMETRY) != 0, ctx->stream);
     if (rc != XT_OK) {

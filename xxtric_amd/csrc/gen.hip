@@ -252,6 +252,64 @@ int graph_from_edges(xt_ctx* ctx, int64_t nv, int64_t m, const int64_t* src, con
 }
 
 // ---------------------------------------------------------------------------
+// A resident CSR made a simple undirected graph: every entry (u, v) becomes the
+// keys (u << bits | v) and (v << bits | u), so rows come out sorted, duplicates
+// merged and missing reverse entries added; self-loops become the sentinel and
+// are dropped.  One wave per row (a hub row spreads over the wave's lanes).
+__global__ __launch_bounds__(256) void k_csr_keys(int64_t nv, const int64_t* __restrict__ off,
+                                                  const int32_t* __restrict__ tails, int bits,
+                                                  uint64_t sentinel, uint64_t* __restrict__ keys) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t nw = (int64_t)gridDim.x * (blockDim.x / kWave);
+  for (int64_t u = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; u < nv; u += nw) {
+    const int64_t a = off[u], b = off[u + 1];
+    for (int64_t e = a + lane; e < b; e += kWave) {
+      const int64_t v = tails[e];
+      if (v == u) {
+        keys[2 * e] = sentinel;
+        keys[2 * e + 1] = sentinel;
+      } else {
+        keys[2 * e] = ((uint64_t)u << bits) | (uint64_t)v;
+        keys[2 * e + 1] = ((uint64_t)v << bits) | (uint64_t)u;
+      }
+    }
+  }
+}
+
+int graph_clean(xt_ctx* ctx, xt_graph* g) {
+  hipStream_t st = ctx->stream;
+  const int64_t nv = g->nv, ne = g->ne;
+  int bits = 1;
+  while ((1ll << bits) < nv) bits++;
+  const uint64_t sentinel = (1ULL << (2 * bits)) - 1;
+  const int64_t nk = 2 * ne;
+  uint64_t *ka = nullptr, *kb = nullptr;
+  hipError_t e = hipMalloc(&ka, (size_t)(nk + 1) * sizeof(uint64_t));
+  if (e == hipSuccess) e = hipMalloc(&kb, (size_t)(nk + 1) * sizeof(uint64_t));
+  if (e == hipSuccess && nv > 0) {
+    hipLaunchKernelGGL(k_csr_keys, dim3(grid_of(nv * kWave)), dim3(256), 0, st, nv, (const int64_t*)g->off,
+                       (const int32_t*)g->tails, bits, sentinel, ka);
+    e = hipGetLastError();
+  }
+  xt_graph out;
+  int rc = e == hipSuccess ? build_csr_from_keys(ctx, ka, kb, nk, 2 * bits, bits, nv, sentinel, &out)
+                           : hip_err(ctx, e, "graph clean keys");
+  if (ka) (void)hipFree(ka);
+  if (kb) (void)hipFree(kb);
+  if (rc != XT_OK) {
+    if (out.off) (void)hipFree(out.off);
+    if (out.tails) (void)hipFree(out.tails);
+    return rc;
+  }
+  (void)hipFree(g->off);
+  (void)hipFree(g->tails);
+  g->off = out.off;
+  g->tails = out.tails;
+  g->ne = out.ne;
+  return XT_OK;
+}
+
+// ---------------------------------------------------------------------------
 // G(n,p): pass 0 counts each row, pass 1 writes it (ordered by ballot rank).
 template <int PASS>
 __global__ __launch_bounds__(256) void k_er_rows(int64_t n, uint64_t key, uint64_t thr64,

@@ -176,6 +176,9 @@ int gen_rmat(xt_ctx* ctx, int scale, int edgefactor, const uint32_t thr[3], uint
 int gen_er(xt_ctx* ctx, int64_t n, uint64_t seed, uint64_t thr64, xt_graph* g);
 // Edge list (host pairs) -> symmetric, merged, self-loop-free CSR on the device.
 int graph_from_edges(xt_ctx* ctx, int64_t nv, int64_t m, const int64_t* src, const int64_t* dst, xt_graph* g);
+// A resident CSR rebuilt as a simple undirected graph in place: rows sorted,
+// duplicates merged, self-loops dropped, missing reverse entries added.
+int graph_clean(xt_ctx* ctx, xt_graph* g);
 // Rows of [lo, hi) and of their neighbours kept, the others emptied (fills out).
 int graph_slice(xt_ctx* ctx, const xt_graph* g, int64_t lo, int64_t hi, xt_graph* out);
 
