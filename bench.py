@@ -36,12 +36,23 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def rmat_workload(ctx, scale, steps, warmup, seed=1):
+def tuning_summary(entry):
+    b = entry["best"]
+    return {"chosen": {k: v for k, v in b.items() if k != "ms"}, "ms": b["ms"], "default_ms": entry["default_ms"],
+            "settings_timed": len(entry["trials"]), "search_s": entry["search_s"]}
+
+
+def rmat_workload(ctx, scale, steps, warmup, seed=1, tune=True):
     t0 = time.time()
     g = ctx.rmat(scale, 16, seed, True)
     gen_s = time.time() - t0
     m = g.ne // 2
     log("R-MAT s%d generated on device: nv=%d directed=%d (%.1f s)" % (scale, g.nv, g.ne, gen_s))
+    tuning = None
+    if tune:  # knobs chosen by measurement on this graph, before the timed steps
+        from xxtric_amd import autotune
+        tuning = tuning_summary(autotune.tune(ctx, "rmat", scale, reps=2, passes=1, graph=g, cache=None,
+                                              log=log))
     for _ in range(warmup):
         tri = g.count()
     ev = []
@@ -54,7 +65,7 @@ def rmat_workload(ctx, scale, steps, warmup, seed=1):
     med = lambda k: float(np.median([e[k] for e in ev]))  # noqa: E731
     st = g.stats()
     out = {"triangles": tri, "nv": g.nv, "m": m, "ms_per_step": ms_step, "gen_s": gen_s,
-           "n_heavy": ev[-1]["n_heavy"], "n_tiles": ev[-1]["n_tiles"]}
+           "n_heavy": ev[-1]["n_heavy"], "n_tiles": ev[-1]["n_tiles"], "tuning": tuning}
     for k in ("orient_ms", "count_ms", "total_ms", "light_ms", "items_ms", "group_ms"):
         out[k] = med(k)
     out.update(st)
@@ -186,7 +197,7 @@ def cpu_baseline(g, seconds):
                       % (done, seconds, done, nv, 100 * frac)}
 
 
-def batch_workload(ctx, n_graphs=4096, runs=5):
+def batch_workload(ctx, n_graphs=4096, runs=5, tune=True):
     import xxtric_amd
     from xxtric_amd import XT_BATCH_HIPGRAPH, XT_BATCH_PER_GRAPH, XT_BATCH_PERSISTENT
     b = xxtric_amd.Batch(ctx, 40_000_000, 700_000_000, n_graphs)
@@ -194,6 +205,10 @@ def batch_workload(ctx, n_graphs=4096, runs=5):
     b.add_generated(2026, 0, n_graphs)
     add_s = time.time() - t0
     out = {"n_graphs": n_graphs, "build_s": round(add_s, 2)}
+    if tune:  # the same knobs for all three launch modes
+        from xxtric_amd import autotune
+        out["tuning"] = tuning_summary(autotune.tune(ctx, "batch", graphs=n_graphs, reps=3, passes=1, batch=b,
+                                                     cache=None, log=log))
     res = {}
     for name, mode, k in (("per_graph", XT_BATCH_PER_GRAPH, 2), ("persistent", XT_BATCH_PERSISTENT, runs),
                           ("hipgraph", XT_BATCH_HIPGRAPH, runs)):
@@ -208,6 +223,12 @@ def batch_workload(ctx, n_graphs=4096, runs=5):
     out["speedup_vs_per_graph"] = round(out["per_graph_ms"] / out["hipgraph_ms"], 1)
     out["results_agree"] = bool((res["per_graph"] == res["persistent"]).all() and
                                 (res["persistent"] == res["hipgraph"]).all())
+    fx = os.path.join(ROOT, "tests", "golden", "batch_4096.json")
+    if n_graphs == 4096 and os.path.exists(fx):  # per-graph totals of the oracle's degree-ordered counter
+        with open(fx) as f:
+            exp = json.load(f)
+        tot = np.array([x[3] for x in exp["graphs"]], np.uint64)  # [n, kind, entries, triangles]
+        out["results_equal_oracle_fixture"] = bool(np.array_equal(res["hipgraph"].astype(np.uint64), tot))
     t = b.timing()
     out["vertices"] = t["n_vertices"]
     out["directed_entries"] = t["n_edges"]
@@ -298,6 +319,8 @@ def main():
                     help="only run the serving workload at 1k..50k graphs/s and print one line per rate")
     ap.add_argument("--batch-only", action="store_true",
                     help="only run the batched 4096-graph workload")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the library's default knobs instead of tuning them on each workload first")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the single-device path only "
@@ -313,10 +336,11 @@ def main():
             print(json.dumps(serve_workload(ctx, float(rate), n_req=max(2000, min(20000, rate)))),
                   flush=True)
         return
+    tune = not args.no_autotune
     if args.batch_only:
-        print(json.dumps(batch_workload(ctx)), flush=True)
+        print(json.dumps(batch_workload(ctx, tune=tune)), flush=True)
         return
-    g, r = rmat_workload(ctx, args.scale, args.steps, args.warmup)
+    g, r = rmat_workload(ctx, args.scale, args.steps, args.warmup, tune=tune)
     value = r["m"] / (r["ms_per_step"] * 1e-3)
     line = {
         "metric": METRIC, "value": round(value, 1), "unit": "edges/s", "n_gpus": 1,
@@ -332,6 +356,7 @@ def main():
         "edges_per_s_device": round(r["m"] / (r["total_ms"] * 1e-3), 1),
         "ne_plus": r["ne_plus"], "sum_min": r["sum_min"], "heavy_edges": r["heavy_edges"],
         "sum_min_heavy": r["sum_min_heavy"], "group_tiles": r["n_tiles"],
+        "tuning": r["tuning"],
     }
     line["cpu_baseline"] = cpu_baseline(g, args.cpu_seconds)
     if not args.no_extras:
@@ -350,7 +375,7 @@ def main():
         except Exception as e:
             line["small_configs"] = {"error": str(e)}
         try:
-            line["batch_4096"] = batch_workload(ctx)
+            line["batch_4096"] = batch_workload(ctx, tune=tune)
         except Exception as e:  # reported, not hidden
             line["batch_4096"] = {"error": str(e)}
         try:

@@ -80,6 +80,14 @@ int xt_last_timing(const xt_ctx* ctx, xt_timing* out);
  * in flight per streaming round (1, 2 or 4, default 2).  None of them changes
  * a result. */
 int xt_ctx_tune(xt_ctx* ctx, int light_cut, int group_blocks, int group_round);
+/* Group-tile knobs (0 keeps the current value, -1 restores the default):
+ * items_per_tile -- heavy edges of one owner per group tile (64, 128 or 256,
+ * default 256); tile_run -- consecutive tiles a group-kernel workgroup draws
+ * from the work queue at once (1..64, default 16; an owner's tiles are
+ * consecutive, so longer runs keep its table staged more often).  Neither
+ * changes a result.  The counterpart of the reference's compile-time tuning
+ * macros (counting.cuh:150-176). */
+int xt_ctx_tune_tiles(xt_ctx* ctx, int items_per_tile, int tile_run);
 /* on != 0: count every graph with the layout used for graphs of 2^32 or more
  * entries (64-bit list offsets gathered by the light pass, 16-bit saturated
  * out-degrees for classification) -- for tests and comparisons on graphs of

@@ -148,6 +148,61 @@ def test_kernel_split_does_not_change_counts(ctx, small_cut):
         ctx.tune(small_cut=xxtric_amd.DEFAULT_LIGHT_CUT, group_blocks=4, group_round=2)
 
 
+@pytest.mark.parametrize("ipt,run", [(64, 1), (128, 64), (256, 7), (64, 16)])
+def test_tile_knobs_do_not_change_counts(ctx, ipt, run):
+    """Items per group tile and tiles per queue draw: global, per-vertex and
+    owned-range counts unchanged (every edge heavy, so every edge is tiled)."""
+    off, tails = O.rmat_csr(14, 16, 3, True)
+    exp, exp_pv = O.count_degree_ordered(off, tails, per_vertex=True)
+    n = len(off) - 1
+    ctx.tune(small_cut=1, items_per_tile=ipt, tile_run=run)
+    try:
+        _say("items_per_tile", ipt, "tile_run", run)
+        tot, pv = ctx.count_csr(off, tails.astype(np.int64), per_vertex=True)
+        assert tot == exp and (pv == exp_pv).all()
+        t = ctx.timing()
+        assert t["n_tiles"] >= (t["n_heavy"] + ipt - 1) // ipt
+        g = ctx.graph(off, tails)
+        owned, _, _ = g.count_owned(n // 4, n // 2, total=False)
+        assert owned == int(exp_pv[n // 4: n // 2].sum())
+        g.close()
+    finally:
+        ctx.tune(small_cut=xxtric_amd.DEFAULT_LIGHT_CUT, items_per_tile=-1, tile_run=-1)
+
+
+def test_tile_knobs_reject_bad_values(ctx):
+    for ipt, run in ((100, 0), (512, 0), (0, 65), (0, -2)):
+        with pytest.raises(xxtric_amd.XtError) as e:
+            ctx.tune(items_per_tile=ipt, tile_run=run)
+        assert e.value.code == -1
+
+
+def test_autotune_search_keeps_counts(ctx, tmp_path):
+    """The autotuner's coordinate descent over a small space: every setting it
+    times gives the same count, and the chosen one is applied and cached."""
+    import json
+    from xxtric_amd import autotune
+    g = ctx.rmat(13, 16, 2, True)
+    exp = g.count()
+    space = {"light_cut": (16, 64), "group_round": (1, 2), "group_blocks": (4,), "items_per_tile": (64, 256),
+             "tile_run": (4, 16)}
+    cache = str(tmp_path / "tuning.json")
+    runner = lambda: autotune._time_graph(ctx, g, 1)  # noqa: E731
+    try:
+        best, trials = autotune.search(ctx, runner, autotune.DEFAULTS, space=space, passes=2, log=_say)
+        assert len(trials) >= 5 and best["ms"] == min(t["ms"] for t in trials)
+        assert g.count() == exp
+        e = autotune.tune(ctx, "rmat", 13, reps=1, passes=1, graph=g, cache=cache, log=_say)
+        with open(cache) as f:
+            assert json.load(f)["rmat_s13"]["best"] == e["best"]
+        assert autotune.apply_cached(ctx, "rmat_s13", cache=cache) is not None
+        assert g.count() == exp
+    finally:
+        ctx.tune(small_cut=xxtric_amd.DEFAULT_LIGHT_CUT, group_blocks=4, group_round=2, items_per_tile=-1,
+                 tile_run=-1)
+        g.close()
+
+
 @pytest.mark.parametrize("n", [1500, 4098])
 def test_complete_graph_fills_the_lds_hash(ctx, n):
     """K_n: out-lists of up to n-1 ids, so the group kernel's LDS hash runs at

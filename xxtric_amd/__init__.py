@@ -72,11 +72,14 @@ class Context:
         except Exception:
             pass
 
-    def tune(self, small_cut=0, group_blocks=0, group_round=0):
-        """Set the light-edge cut, the group kernel's workgroups per CU and/or its
-        chunks in flight per thread (0 = keep; small_cut=-1 restores the default
-        cuts).  No setting changes a result."""
+    def tune(self, small_cut=0, group_blocks=0, group_round=0, items_per_tile=0, tile_run=0):
+        """Set the light-edge cut, the group kernel's workgroups per CU, its chunks in
+        flight per thread, the items per group tile and/or the tiles a workgroup draws
+        at once (0 = keep; -1 restores the default of small_cut, items_per_tile,
+        tile_run).  No setting changes a result."""
         self._check(self._L.xt_ctx_tune(self.h, small_cut, group_blocks, group_round))
+        if items_per_tile or tile_run:
+            self._check(self._L.xt_ctx_tune_tiles(self.h, items_per_tile, tile_run))
 
     def force_wide(self, on=True):
         """Count with the layout of graphs of 2^32 or more entries (tests, comparisons)."""

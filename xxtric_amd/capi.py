@@ -85,6 +85,7 @@ _SIGS = {
     "xt_last_error": (ctypes.c_char_p, [_p]),
     "xt_last_timing": (_int, [_p, ctypes.POINTER(XtTiming)]),
     "xt_ctx_tune": (_int, [_p, _int, _int, _int]),
+    "xt_ctx_tune_tiles": (_int, [_p, _int, _int]),
     "xt_ctx_force_wide": (_int, [_p, _int]),
     "xt_dev_alloc": (_int, [_p, ctypes.c_size_t, _pp]),
     "xt_dev_free": (_int, [_p, _p]),

@@ -287,6 +287,22 @@ int xt_ctx_tune(xt_ctx* ctx, int light_cut, int group_blocks, int group_round) {
   });
 }
 
+int xt_ctx_tune_tiles(xt_ctx* ctx, int items_per_tile, int tile_run) {
+  return xt_guarded([&]() -> int {
+    if (!ctx) return XT_ERR_ARG;
+    if (items_per_tile != 0 && items_per_tile != -1 && items_per_tile != 64 && items_per_tile != 128 &&
+        items_per_tile != 256)
+      return set_err(ctx, XT_ERR_ARG, "items_per_tile must be 64, 128 or 256 (0 keeps, -1 restores 256)");
+    if (tile_run < -1 || tile_run > 64) return set_err(ctx, XT_ERR_ARG, "tile_run must be in 1..64 (0 keeps, -1 restores 16)");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    if (items_per_tile > 0) ctx->items_per_tile = items_per_tile;
+    if (items_per_tile == -1) ctx->items_per_tile = 256;
+    if (tile_run > 0) ctx->tile_run = tile_run;
+    if (tile_run == -1) ctx->tile_run = 16;
+    return XT_OK;
+  });
+}
+
 int xt_ctx_force_wide(xt_ctx* ctx, int on) {
   return xt_guarded([&]() -> int {
     if (!ctx) return XT_ERR_ARG;

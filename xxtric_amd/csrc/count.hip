@@ -16,9 +16,9 @@
 // Both happen in one pass over the symmetric input rows (k_light_items): entry
 // (x, w) of row x is an out-edge or an in-edge of x, so every oriented edge is
 // seen from both of its rows and is counted or listed from exactly one of them.
-// The item lists are cut into tiles of up to 256 items with one descriptor each
-// (k_tile_desc).  k_count_groups, a persistent kernel, draws runs of 16
-// consecutive tiles from a device work queue, stages N+(x) in an LDS hash
+// The item lists are cut into tiles of up to 256 items (a tuning knob) with one
+// descriptor each (k_tile_desc).  k_count_groups, a persistent kernel, draws
+// runs of consecutive tiles (16 by default) from a device work queue, stages N+(x) in an LDS hash
 // table of 4-slot buckets (kept for the next tile of the same owner), and streams the items' shorter lists N+(y) through it: the tile's
 // lists are concatenated into one run of 16-byte chunks that all 512 threads
 // read with coalesced loads, the next round of loads in flight while the
@@ -42,7 +42,8 @@ constexpr int kGraphSlots = 64;        // per-tile LDS slots for per-graph batch
 #ifndef XT_ITEMS_PER_TILE
 #define XT_ITEMS_PER_TILE 256
 #endif
-constexpr int kItemsPerTile = XT_ITEMS_PER_TILE;  // items of one group tile
+constexpr int kItemsPerTile = XT_ITEMS_PER_TILE;  // most items of one group tile (LDS arrays)
+constexpr int kMinItemsPerTile = 64;              // fewest a tiling may ask for (sizes the descriptors)
 constexpr int kTabLog = 11;            // LDS hash: 2048 buckets of four ids (32 KB)
 constexpr int kTabBuckets = 1 << kTabLog;
 constexpr int kGroupThreads = 512;     // eight waves share one staged hash table
@@ -575,11 +576,12 @@ __global__ __launch_bounds__(256) void k_vertex_ranges(int64_t nv, const int64_t
   }
 }
 
+// Tiles of each owner: its items cut into tiles of ipt (<= kItemsPerTile).
 __global__ __launch_bounds__(256) void k_tile_counts(int64_t nv, const uint32_t* __restrict__ itcnt,
-                                                     uint32_t* __restrict__ tcnt) {
+                                                     uint32_t* __restrict__ tcnt, int ipt) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < nv; x += stride)
-    tcnt[x] = (itcnt[x] + kItemsPerTile - 1) / kItemsPerTile;
+    tcnt[x] = (itcnt[x] + (uint32_t)ipt - 1) / (uint32_t)ipt;
 }
 
 // One descriptor per tile; x's items start at off[x].
@@ -587,7 +589,7 @@ __global__ __launch_bounds__(256) void k_tile_desc(int64_t nv, const int64_t* __
                                                    const int64_t* __restrict__ ooff,
                                                    const uint32_t* __restrict__ itcnt,
                                                    const int64_t* __restrict__ tileoff,
-                                                   TileDesc* __restrict__ desc) {
+                                                   TileDesc* __restrict__ desc, int ipt) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < nv; x += stride) {
     const int64_t t0 = tileoff[x], t1 = tileoff[x + 1];
@@ -595,8 +597,8 @@ __global__ __launch_bounds__(256) void k_tile_desc(int64_t nv, const int64_t* __
     const int64_t ib = off[x], cnt = itcnt[x], a0 = ooff[x];
     const int32_t dx = (int32_t)(ooff[x + 1] - a0);
     for (int64_t t = t0; t < t1; t++) {
-      const int64_t k = (t - t0) * kItemsPerTile;
-      const int64_t n = cnt - k < kItemsPerTile ? cnt - k : kItemsPerTile;
+      const int64_t k = (t - t0) * ipt;
+      const int64_t n = cnt - k < ipt ? cnt - k : ipt;
       desc[t] = TileDesc{ib + k, a0, (int32_t)x, (int32_t)n, dx, 0};
     }
   }
@@ -679,7 +681,7 @@ __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(W
     const int32_t* __restrict__ ot, const Item* __restrict__ items, const TileDesc* __restrict__ tdesc,
     const int64_t* __restrict__ ntiles_p, uint64_t* __restrict__ counters,
     unsigned long long* __restrict__ pv, const int32_t* __restrict__ gid,
-    unsigned long long* __restrict__ gtotal, int64_t rlo, int64_t rhi) {
+    unsigned long long* __restrict__ gtotal, int64_t rlo, int64_t rhi, uint32_t tile_run) {
   constexpr bool PV = CM == kCreditVertex;
   __shared__ int4 s_tab[kTabBuckets];
   // Per item k: {adj (64-bit), end, y}.  Chunk c of item k (16 bytes) starts at
@@ -708,25 +710,24 @@ __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(W
   const int tid = threadIdx.x;
   const int lane = tid & (kWave - 1);
   const int wave = __builtin_amdgcn_readfirstlane(tid / kWave);  // uniform per wave
-  // Tile indices fit 32 bits: there are fewer than nv + ne/256 + 1 tiles, with
+  // Tile indices fit 32 bits: there are fewer than nv + ne/64 + 1 tiles, with
   // nv < 2^31 and ne < 2^33 (checked on the host), so the queue, which overshoots
-  // ntiles by at most 3 runs per workgroup, cannot wrap.
+  // ntiles by at most 3 runs of at most 64 tiles per workgroup, cannot wrap.
   const uint32_t ntiles = (uint32_t)*ntiles_p;
   unsigned int* queue = reinterpret_cast<unsigned int*>(&counters[kTileQueue]);
 
   // Work queue, fetched ahead by wave 0: while tile T runs, the descriptor of
   // T+1 is loaded and the index of T+2 is drawn.
-  // Tiles are drawn in runs of kTileRun consecutive indices: an owner's tiles
+  // Tiles are drawn in runs of tile_run consecutive indices: an owner's tiles
   // are consecutive, so a workgroup often keeps the owner's table for the
   // next tile.
-  constexpr uint32_t kTileRun = 16;
   uint32_t run_next = 0, run_end = 0;  // wave 0: tiles drawn, not yet taken
   auto draw = [&]() -> uint32_t {
     if (run_next >= run_end) {
       uint32_t v = 0;
-      if (lane == 0) v = atomicAdd(queue, kTileRun);
+      if (lane == 0) v = atomicAdd(queue, tile_run);
       run_next = __builtin_amdgcn_readfirstlane(v);
-      run_end = run_next + kTileRun;
+      run_end = run_next + tile_run;
     }
     return run_next++;
   };
@@ -1152,7 +1153,7 @@ __global__ __launch_bounds__(kBlock) void k_alg_stats(int64_t nv, const int64_t*
 // Tiles of all owners: at most one partial tile per owner plus the full ones,
 // and never more than the heavy edges.
 static int64_t tiles_bound(int64_t nv, int64_t ne) {
-  const int64_t b = nv + ne / kItemsPerTile + 1;
+  const int64_t b = nv + ne / kMinItemsPerTile + 1;
   return b < ne + 1 ? b : ne + 1;
 }
 
@@ -1232,20 +1233,22 @@ int count_oriented(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne_bound, const 
 #undef XT_LIGHT
   XT_HIP(ctx, hipGetLastError());
   if (evs) XT_HIP(ctx, hipEventRecord(evs[0], st));
+  const int ipt = ctx->items_per_tile;
   hipLaunchKernelGGL(k_tile_counts, dim3((unsigned)g), dim3(256), 0, st, nv,
-                     (const uint32_t*)s.itcnt.as<uint32_t>(), s.deg.as<uint32_t>());
+                     (const uint32_t*)s.itcnt.as<uint32_t>(), s.deg.as<uint32_t>(), ipt);
   XT_HIP(ctx, hipGetLastError());
   XT_TRY(scan_u32_to_i64(ctx, s, s.deg.as<uint32_t>(), nv, s.tileoff.as<int64_t>(), st));
   hipLaunchKernelGGL(k_tile_desc, dim3((unsigned)g), dim3(256), 0, st, nv, in_off, ooff,
                      (const uint32_t*)s.itcnt.as<uint32_t>(), (const int64_t*)s.tileoff.as<int64_t>(),
-                     s.tiledesc.as<TileDesc>());
+                     s.tiledesc.as<TileDesc>(), ipt);
   XT_HIP(ctx, hipGetLastError());
   if (evs) XT_HIP(ctx, hipEventRecord(evs[1], st));
   const unsigned groups = (unsigned)(ctx->num_cus * ctx->group_blocks);
 #define XT_GROUPS(C, B, R, W)                                                                      \
   hipLaunchKernelGGL((k_count_groups<C, B, R, W>), dim3(groups), dim3(kGroupThreads), 0, st, ot,     \
                      (const Item*)s.items.as<Item>(), (const TileDesc*)s.tiledesc.as<TileDesc>(),   \
-                     (const int64_t*)s.tileoff.as<int64_t>() + nv, counters, pvp, gid, gt, rlo, rhi)
+                     (const int64_t*)s.tileoff.as<int64_t>() + nv, counters, pvp, gid, gt, rlo, rhi,   \
+                     (uint32_t)ctx->tile_run)
 #define XT_GROUPS_R(P, B)          \
   if (ctx->group_round == 1)       \
     XT_GROUPS(P, B, 1, 8);         \

@@ -95,6 +95,8 @@ struct xt_ctx {
   int small_cut = 0;
   int group_blocks = 4;   // persistent group-kernel workgroups per CU
   int group_round = 2;    // 16-byte chunks per thread per group-kernel streaming round (1, 2, 4)
+  int items_per_tile = 256;  // items of one group tile (64, 128 or 256)
+  int tile_run = 16;         // consecutive tiles a group-kernel workgroup draws at once (1..64)
   bool force_wide = false;  // use the 64-bit offset layout of graphs with >= 2^32 entries everywhere
 };
 
