@@ -99,10 +99,30 @@ def rank0_slice(ctx, g, triangles, parts=8, steps=3):
     return out
 
 
-def small_configs(ctx, steps=5):
+def _oracle():
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py as O
+    return O
+
+
+def _timed(fn, reps=1):
+    best, out = float("inf"), None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        out = fn()
+        best = min(best, time.perf_counter() - t0)
+    return best, out
+
+
+def small_configs(ctx, steps=5, cpu=True, cpu_seconds=10.0):
     """The reference's CPU-runnable RGG (n = 100,000, LCG seed, reference radius)
-    and R-MAT scale 22: device count time (median of `steps`, graph resident)."""
+    and R-MAT scale 22: device count time (median of `steps`, graph resident),
+    and beside each the reference algorithm on the host cores (the oracle's
+    restatement of xxtric.hpp:60-208, timed around the local count as
+    xxtric.hpp:58,210-224 times it)."""
     import xxtric_amd
+    O = _oracle() if cpu else None
+    cores = O.nthreads_default() if cpu else 0
     out = {}
     off, tails = xxtric_amd.rgg_generate(100000)
     g = ctx.graph(off, tails)
@@ -111,10 +131,21 @@ def small_configs(ctx, steps=5):
     for _ in range(steps):
         g.count()
         ms.append(ctx.timing()["total_ms"])
+    m = len(tails) // 2
     out["rgg_1e5"] = {"nv": g.nv, "directed_entries": g.ne, "triangles": tot,
                       "per_vertex_sum_equals_total": int(pv.sum(dtype=np.uint64)) == tot,
-                      "ms": round(float(np.median(ms)), 3)}
+                      "ms": round(float(np.median(ms)), 3),
+                      "edges_per_s": round(m / (float(np.median(ms)) * 1e-3), 1)}
     g.close()
+    if cpu:  # the whole count, one thread and all cores, best of 3
+        t1, c1 = _timed(lambda: O.count_reference(off, tails, nthreads=1), 3)
+        tn, cn = _timed(lambda: O.count_reference(off, tails, nthreads=cores), 3)
+        out["rgg_1e5"]["cpu_baseline"] = {
+            "kind": "port", "unit": "edges/s", "cores": cores, "value": round(m / tn, 1), "s": round(tn, 4),
+            "single_thread_value": round(m / t1, 1), "single_thread_s": round(t1, 4),
+            "counts_equal": c1 == cn == tot,
+            "sample": "whole graph: the reference algorithm (oracle restatement of xxtric.hpp:60-208), "
+                      "best of 3 runs"}
     g = ctx.rmat(22, 16, 1, True)
     ms = []
     for _ in range(steps + 1):
@@ -123,8 +154,63 @@ def small_configs(ctx, steps=5):
     med = float(np.median(ms[1:]))
     out["rmat_s22"] = {"nv": g.nv, "directed_entries": g.ne, "triangles": tot, "ms": round(med, 3),
                        "edges_per_s": round((g.ne // 2) / (med * 1e-3), 1)}
+    if cpu:
+        o22, t22 = g.download()
+        m22 = len(t22) // 2
+        t0 = time.perf_counter()
+        done, _, edges = O.count_reference_budget(o22, t22, cpu_seconds, cores)
+        el = time.perf_counter() - t0
+        frac = edges / max(1, len(t22))
+        td, cd = _timed(lambda: O.count_degree_ordered(o22, t22, nthreads=cores))
+        out["rmat_s22"]["cpu_baseline"] = {
+            "kind": "port", "unit": "edges/s", "cores": cores,
+            "value": round(m22 / (el / frac), 1) if frac > 0 else 0.0,
+            "cap_s": cpu_seconds, "vertices_done": done, "adjacency_fraction_done": round(frac, 6),
+            "degree_ordered_exact_s": round(td, 3), "degree_ordered_exact_edges_per_s": round(m22 / td, 1),
+            "degree_ordered_count_equal": cd == tot,
+            "sample": "the reference algorithm on vertices 0, 1, 2, ... (labels permuted) until the %.0f s cap, "
+                      "extrapolated by adjacency fraction; beside it the oracle's exact degree-ordered counter "
+                      "over the whole graph" % cpu_seconds}
     g.close()
     return out
+
+
+def batch_cpu_baseline(ctx, n_graphs=4096, seconds=10.0):
+    """Summed per-graph time of the reference algorithm (one thread per graph)
+    over the 4096-graph mix, from the graphs it finishes in `seconds`,
+    extrapolated by their share of the mix's entries."""
+    import xxtric_amd
+    O = _oracle()
+    tot_entries = 0
+    spent, done, entries = 0.0, 0, 0
+    for i in range(n_graphs):
+        n, kind, gseed, thr = xxtric_amd.batch_params(2026, i)
+        if kind == 1:  # the mix's G(n,p) graphs come from the device generator (as the batch builds them)
+            dg = ctx.er(n, gseed + 3, thr)
+            o, t = dg.download()
+            dg.close()
+        else:
+            o, t = xxtric_amd.batch_graph_csr(2026, i)
+        dt, _ = _timed(lambda: O.count_reference(o, t, nthreads=1))
+        spent += dt
+        done += 1
+        entries += len(t)
+        if spent >= seconds:
+            break
+    # entries of the whole mix without generating it again: the fixture records them
+    fx = os.path.join(ROOT, "tests", "golden", "batch_4096.json")
+    if n_graphs == 4096 and os.path.exists(fx):
+        with open(fx) as f:
+            tot_entries = sum(x[2] for x in json.load(f)["graphs"])
+    est = spent * tot_entries / entries if tot_entries and entries else float("nan")
+    cores = O.nthreads_default()
+    return {"kind": "port", "unit": "graphs/s", "cores": 1, "value": round(n_graphs / est, 1),
+            "summed_per_graph_s": round(est, 2), "graphs_timed": done, "entries_timed": entries,
+            "all_cores_value": round(cores * n_graphs / est, 1), "all_cores": cores,
+            "sample": "the reference algorithm (oracle restatement of xxtric.hpp:60-208), one thread per "
+                      "graph, on graphs 0..%d of the mix in a %.0f s budget, extrapolated by their share of "
+                      "the mix's entries; all_cores_value: %d graphs counted at once (independent graphs)"
+                      % (done - 1, seconds, cores)}
 
 
 def per_vertex_count(ctx, g, triangles, steps=3):
@@ -233,6 +319,15 @@ def batch_workload(ctx, n_graphs=4096, runs=5, tune=True):
     out["vertices"] = t["n_vertices"]
     out["directed_entries"] = t["n_edges"]
     out["triangles_total"] = int(res["hipgraph"].sum())
+    # HBM roofline of the batched count: the per-graph algorithmic bytes
+    # 8(n+1) + 4|E+| + 4 sum min(d+(u), d+(v)), summed over the graphs, over the
+    # hipGraph replay's device time
+    st = b.stats()
+    ach = st["alg_bytes"] / (out["hipgraph_ms"] * 1e-3) / 1e9
+    out["roofline"] = {"bound": "hbm", "alg_bytes": st["alg_bytes"], "achieved": round(ach, 1),
+                       "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                       "sum_min": st["sum_min"], "ne_plus": st["ne_plus"], "heavy_edges": st["heavy_edges"],
+                       "group_bytes": st["group_bytes"]}
     b.close()
     return out
 
@@ -302,6 +397,27 @@ def copy_calibration(ctx, nbytes=1 << 30, reps=3):
                       "best_s": min(ts), "GBps_read_plus_write": 2 * nbytes / min(ts) / 1e9}))
 
 
+def memory_ceilings(ctx, reps=3):
+    """The read paths the counting kernels are bound by, on this GPU: a streaming
+    read (the group kernel's lists) and random gathers of 2-, 4- and 8-byte
+    elements (the orientation and light passes' neighbour keys and records),
+    from arrays the size of R-MAT s26's, and one that fits the Infinity Cache."""
+    import ctypes
+    out = {"streaming_read": [], "random_gather": []}
+    for nbytes in (128 << 20, 8 << 30):
+        ms = ctypes.c_double()
+        ctx._check(ctx._L.xt_mem_ceiling(ctx.h, 0, nbytes // 16, 16, 0, reps, ctypes.byref(ms)))
+        out["streaming_read"].append({"bytes": nbytes, "ms": round(ms.value, 3),
+                                      "GBps": round(nbytes / (ms.value * 1e-3) / 1e9, 1)})
+    ng = 1 << 31
+    for eb, n in ((2, 1 << 26), (4, 1 << 26), (8, 1 << 26), (2, 1 << 22)):
+        ms = ctypes.c_double()
+        ctx._check(ctx._L.xt_mem_ceiling(ctx.h, 1, n, eb, ng, reps, ctypes.byref(ms)))
+        out["random_gather"].append({"elem_bytes": eb, "array_bytes": n * eb, "gathers": ng,
+                                     "ms": round(ms.value, 3), "Ggathers_per_s": round(ng / (ms.value * 1e-3) / 1e9, 2)})
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -319,6 +435,8 @@ def main():
                     help="only run the serving workload at 1k..50k graphs/s and print one line per rate")
     ap.add_argument("--batch-only", action="store_true",
                     help="only run the batched 4096-graph workload")
+    ap.add_argument("--ceilings", action="store_true",
+                    help="only measure the streaming-read and random-gather ceilings of this GPU")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the library's default knobs instead of tuning them on each workload first")
     args = ap.parse_args()
@@ -330,6 +448,9 @@ def main():
     ctx = xxtric_amd.Context(0)
     if args.copy_calibration:
         copy_calibration(ctx)
+        return
+    if args.ceilings:
+        print(json.dumps(memory_ceilings(ctx)), flush=True)
         return
     if args.serve_sweep:
         for rate in (1000, 2000, 5000, 10000, 20000, 50000):
@@ -376,8 +497,9 @@ def main():
             line["small_configs"] = {"error": str(e)}
         try:
             line["batch_4096"] = batch_workload(ctx, tune=tune)
+            line["batch_4096"]["cpu_baseline"] = batch_cpu_baseline(ctx)
         except Exception as e:  # reported, not hidden
-            line["batch_4096"] = {"error": str(e)}
+            line["batch_4096"] = dict(line.get("batch_4096") or {}, error=str(e))
         try:
             line["serve"] = serve_workload(ctx, args.serve_rate)
         except Exception as e:

@@ -107,6 +107,13 @@ int xt_dev_alloc(xt_ctx* ctx, size_t bytes, void** out);
 int xt_dev_free(xt_ctx* ctx, void* p);
 int xt_memcpy(xt_ctx* ctx, void* dst, const void* src, size_t bytes, int kind);
 int xt_dev_sync(xt_ctx* ctx);
+/* No counterpart in the reference: the memory read paths the counting kernels
+ * are bound by, measured on this device (HIP events, best of `reps` after one
+ * warm-up, into *ms).  kind 0: a streaming read of n 16-byte elements (every
+ * byte once, coalesced); kind 1: ngather loads of elem_bytes (2, 4 or 8) from
+ * pseudo-random positions of an array of n elements (n a power of two). */
+int xt_mem_ceiling(xt_ctx* ctx, int kind, int64_t n, int elem_bytes, int64_t ngather, int reps,
+                   double* ms);
 
 /* ---- single-device count of a host CSR ---------------------------------
  * Replaces TriangleCountingSingle(Graph*) + count() (xxtric-gpu.hpp:27-156,
@@ -308,6 +315,12 @@ int xt_batch_add_generated(xt_batch* b, uint64_t base_seed, int64_t first, int64
 int xt_batch_run(xt_batch* b, int mode, uint32_t flags);
 int xt_batch_results(xt_batch* b, uint64_t* totals, int64_t n);
 int xt_batch_timing_get(const xt_batch* b, xt_batch_timing* out);
+/* Algorithmic byte figures of the whole batch under `flags` orientation, as
+ * xt_graph_stats_get gives for one graph, summed over its graphs
+ * (8(n+1) + 4|E+| + 4 * sum of min(d+(u), d+(v)) each); heavy edges are those
+ * whose longer list exceeds the batch light cut.  Orients the batch again on
+ * its stream; results of an earlier run stay readable. */
+int xt_batch_stats_get(xt_batch* b, uint32_t flags, xt_graph_stats* out);
 int xt_batch_clear(xt_batch* b);
 int xt_batch_destroy(xt_batch* b);
 /* Parameters of batch graph g (log-uniform n in [nmin,nmax], RGG even / ER odd). */

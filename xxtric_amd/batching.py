@@ -49,6 +49,13 @@ class Batch:
         self.ctx._check(self._L.xt_batch_timing_get(self.h, ctypes.byref(t)))
         return {k: getattr(t, k) for k, _ in t._fields_}
 
+    def stats(self, orient="degree"):
+        """Algorithmic byte figures of the whole batch (sums over its graphs)."""
+        st = capi.XtGraphStats()
+        flags = capi.XT_ORIENT_ID if orient == "id" else capi.XT_ORIENT_DEGREE
+        self.ctx._check(self._L.xt_batch_stats_get(self.h, flags, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in st._fields_}
+
     def clear(self):
         self.ctx._check(self._L.xt_batch_clear(self.h))
         self.n = 0

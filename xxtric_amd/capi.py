@@ -91,6 +91,7 @@ _SIGS = {
     "xt_dev_free": (_int, [_p, _p]),
     "xt_memcpy": (_int, [_p, _p, _p, ctypes.c_size_t, _int]),
     "xt_dev_sync": (_int, [_p]),
+    "xt_mem_ceiling": (_int, [_p, _int, _i64, _int, _i64, _int, _p]),
     "xt_count_csr": (_int, [_p, _i64, _i64, _p, _p, _u32, _p, _p]),
     "xt_single_gpu_count": (_int, [_p, _i64, _p, _p, _p, _p]),
     "xt_single_gpu_count_noalloc": (_int, [_p, _i64, _p, _p, _p, _p]),
@@ -123,6 +124,7 @@ _SIGS = {
     "xt_batch_run": (_int, [_p, _int, _u32]),
     "xt_batch_results": (_int, [_p, _p, _i64]),
     "xt_batch_timing_get": (_int, [_p, ctypes.POINTER(XtBatchTiming)]),
+    "xt_batch_stats_get": (_int, [_p, _u32, ctypes.POINTER(XtGraphStats)]),
     "xt_batch_clear": (_int, [_p]),
     "xt_batch_destroy": (_int, [_p]),
     "xt_batch_params": (_int, [_u64, _i64, _i64, _i64, _p, _p, _p, _p]),

@@ -5,6 +5,7 @@
 #include <cstring>
 #include <vector>
 
+#include "xt_device.hpp"
 #include "xt_internal.hpp"
 
 namespace xt {
@@ -28,6 +29,40 @@ static int launch_copy16(xt_ctx* ctx, void* dst, const void* src, size_t bytes, 
                      static_cast<int4*>(dst), static_cast<const int4*>(src), n);
   XT_HIP(ctx, hipGetLastError());
   return XT_OK;
+}
+
+// ---- memory ceilings: the read paths the counting kernels are bound by ------
+// Streaming read: every 16-byte element once, coalesced (the group kernel's
+// streamed lists).  Random gather: one elem-byte load per index from a
+// pseudo-random position (the orientation and light passes' neighbour keys
+// and records).  Each thread folds what it loads into one word, written once,
+// so no load is dead.
+__global__ __launch_bounds__(256) void k_stream_read(const int4* __restrict__ a, int64_t n,
+                                                     uint32_t* __restrict__ sink) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  uint32_t acc = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const int4 v = a[i];
+    acc ^= (uint32_t)(v.x ^ v.y ^ v.z ^ v.w);
+  }
+  sink[(int64_t)blockIdx.x * blockDim.x + threadIdx.x] = acc;
+}
+
+template <typename E>
+__global__ __launch_bounds__(256) void k_gather(const E* __restrict__ a, int64_t n, int64_t ngather,
+                                                uint32_t* __restrict__ sink) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  uint32_t acc = 0;
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i + 7 * stride < ngather; i += 8 * stride) {  // eight independent gathers in flight
+    E v[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) v[j] = a[mix64_dev((uint64_t)(i + j * stride)) & (uint64_t)(n - 1)];
+#pragma unroll
+    for (int j = 0; j < 8; j++) acc ^= (uint32_t)v[j];
+  }
+  for (; i < ngather; i += stride) acc ^= (uint32_t)a[mix64_dev((uint64_t)i) & (uint64_t)(n - 1)];
+  sink[(int64_t)blockIdx.x * blockDim.x + threadIdx.x] = acc;
 }
 
 int set_err(xt_ctx* ctx, int code, const std::string& msg) {
@@ -357,6 +392,53 @@ int xt_memcpy(xt_ctx* ctx, void* dst, const void* src, size_t bytes, int kind) {
     XT_HIP(ctx, hipDeviceSynchronize());
     XT_HIP(ctx, hipMemcpy(dst, src, bytes, k));
     return XT_OK;
+  });
+}
+
+int xt_mem_ceiling(xt_ctx* ctx, int kind, int64_t n, int elem_bytes, int64_t ngather, int reps,
+                   double* ms_out) {
+  return xt_guarded([&]() -> int {
+    if (!ctx || !ms_out || n <= 0 || reps < 1) return XT_ERR_ARG;
+    if (kind == 1 && elem_bytes != 2 && elem_bytes != 4 && elem_bytes != 8)
+      return set_err(ctx, XT_ERR_ARG, "gather elements are 2, 4 or 8 bytes");
+    if (kind == 1 && (n & (n - 1)) != 0) return set_err(ctx, XT_ERR_ARG, "gather arrays hold a power of two elements");
+    if (kind != 0 && kind != 1) return set_err(ctx, XT_ERR_ARG, "kind 0 (streaming read) or 1 (gather)");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    XT_HIP(ctx, hipSetDevice(ctx->device));
+    const size_t bytes = kind == 0 ? (size_t)n * 16 : (size_t)n * (size_t)elem_bytes;
+    const unsigned grid = (unsigned)(ctx->num_cus * 16);
+    void* a = nullptr;
+    uint32_t* sink = nullptr;
+    XT_HIP(ctx, hipMalloc(&a, bytes));
+    hipError_t e = hipMalloc(&sink, (size_t)grid * 256 * sizeof(uint32_t));
+    int rc = XT_OK;
+    if (e == hipSuccess) e = hipMemsetAsync(a, 0x5a, bytes, ctx->stream);
+    float best = 0.f;
+    for (int r = 0; e == hipSuccess && r <= reps; r++) {  // run 0 warms up
+      XT_HIP(ctx, hipEventRecord(ctx->ev[kEvStart], ctx->stream));
+      if (kind == 0)
+        hipLaunchKernelGGL(k_stream_read, dim3(grid), dim3(256), 0, ctx->stream, (const int4*)a, n, sink);
+      else if (elem_bytes == 2)
+        hipLaunchKernelGGL(k_gather<uint16_t>, dim3(grid), dim3(256), 0, ctx->stream, (const uint16_t*)a, n,
+                           ngather, sink);
+      else if (elem_bytes == 4)
+        hipLaunchKernelGGL(k_gather<uint32_t>, dim3(grid), dim3(256), 0, ctx->stream, (const uint32_t*)a, n,
+                           ngather, sink);
+      else
+        hipLaunchKernelGGL(k_gather<uint64_t>, dim3(grid), dim3(256), 0, ctx->stream, (const uint64_t*)a, n,
+                           ngather, sink);
+      e = hipGetLastError();
+      if (e == hipSuccess) e = hipEventRecord(ctx->ev[kEvCounted], ctx->stream);
+      if (e == hipSuccess) e = hipEventSynchronize(ctx->ev[kEvCounted]);
+      float ms = 0.f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx->ev[kEvStart], ctx->ev[kEvCounted]);
+      if (r > 0 && (best == 0.f || ms < best)) best = ms;
+    }
+    if (e != hipSuccess) rc = hip_err(ctx, e, "memory ceiling kernels");
+    (void)hipFree(a);
+    if (sink) (void)hipFree(sink);
+    *ms_out = best;
+    return rc;
   });
 }
 

@@ -567,6 +567,41 @@ int xt_batch_results(xt_batch* b, uint64_t* totals, int64_t n) {
   });
 }
 
+int xt_batch_stats_get(xt_batch* b, uint32_t flags, xt_graph_stats* out) {
+  return xt_guarded([&]() -> int {
+    if (!b || !out) return XT_ERR_ARG;
+    xt_ctx* ctx = b->ctx;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    XT_HIP(ctx, hipSetDevice(ctx->device));
+    XT_TRY(batch_wait(b));
+    XT_TRY(batch_flush(b));
+    *out = xt_graph_stats{};
+    const int64_t G = (int64_t)b->graphs.size();
+    if (G == 0) return XT_OK;
+    hipStream_t st = b->stream;
+    const int mode = (flags & XT_ORIENT_ID) ? kOrientId : kOrientDegree;
+    // The union's orientation is each graph's own (graphs share no edge).
+    XT_TRY(fill_u64(ctx, b->s.counters.as<uint64_t>(), kNumSlots, 0, st));
+    XT_TRY(orient<int32_t>(ctx, b->s, b->V, b->E, b->goff, b->gtails, mode, st));
+    XT_TRY(alg_stats(ctx, b->s, b->V, b->E, st, ctx->small_cut > 0 ? ctx->small_cut : kBatchLightCut));
+    uint64_t h[kNumSlots];
+    int64_t nep = 0;
+    XT_HIP(ctx, hipMemcpyAsync(h, b->s.counters.p, sizeof(h), hipMemcpyDeviceToHost, st));
+    XT_HIP(ctx, hipMemcpyAsync(&nep, b->s.ooff.as<int64_t>() + b->V, sizeof(nep), hipMemcpyDeviceToHost, st));
+    XT_HIP(ctx, hipStreamSynchronize(st));
+    xt_graph_stats r = {};
+    r.sum_min = h[kStatMin];
+    r.ne_plus = nep;
+    // per graph 8(n+1) + 4|E+| + 4 sum_min, summed over the graphs
+    r.alg_bytes = 8ULL * (uint64_t)(b->V + G) + 4ULL * (uint64_t)nep + 4ULL * r.sum_min;
+    r.sum_min_heavy = h[kStatMinHeavy];
+    r.heavy_edges = (int64_t)h[kStatHeavyEdges];
+    r.group_bytes = 4ULL * (r.sum_min_heavy + (uint64_t)r.heavy_edges);
+    *out = r;
+    return XT_OK;
+  });
+}
+
 int xt_batch_timing_get(const xt_batch* b, xt_batch_timing* out) {
   return xt_guarded([&]() -> int {
     if (!b || !out) return XT_ERR_ARG;

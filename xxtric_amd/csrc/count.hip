@@ -1280,7 +1280,7 @@ template int count_oriented<int64_t>(xt_ctx*, Scratch&, int64_t, int64_t, const 
                                      uint64_t*, const int32_t*, uint64_t*, hipStream_t, hipEvent_t*,
                                      const OwnedRange*);
 
-int alg_stats(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne_bound, hipStream_t st) {
+int alg_stats(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne_bound, hipStream_t st, int cut) {
   if (nv <= 0 || ne_bound <= 0) return XT_OK;
   const int64_t nchunks = (ne_bound + kChunk - 1) / kChunk;
   uint64_t* counters = s.counters.as<uint64_t>();
@@ -1288,7 +1288,7 @@ int alg_stats(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne_bound, hipStream_t
   XT_TRY(fill_u64(ctx, counters + kStatMinHeavy, 2, 0, st));
   hipLaunchKernelGGL(k_alg_stats, dim3((unsigned)nchunks), dim3(kBlock), 0, st, nv,
                      (const int64_t*)s.ooff.as<int64_t>(), (const int32_t*)s.otails.as<int32_t>(),
-                     ctx->small_cut > 0 ? ctx->small_cut : kLightCut, counters);
+                     cut > 0 ? cut : ctx->small_cut > 0 ? ctx->small_cut : kLightCut, counters);
   XT_HIP(ctx, hipGetLastError());
   return XT_OK;
 }

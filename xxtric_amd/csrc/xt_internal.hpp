@@ -143,7 +143,9 @@ template <typename T>
 int count_oriented(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne_bound, const int64_t* in_off,
                    const T* in_tails, uint64_t* pv, const int32_t* gid, uint64_t* gtotal,
                    hipStream_t st, hipEvent_t* evs, const OwnedRange* range = nullptr);
-int alg_stats(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne_bound, hipStream_t st);
+// Sums of min(d+(u), d+(v)) over the oriented edges of s (all, and those whose
+// longer list exceeds `cut`: 0 = the context's light cut for one graph).
+int alg_stats(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne_bound, hipStream_t st, int cut = 0);
 // Pre-sizes every scratch buffer for graphs up to (nv, ne).
 int reserve(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne, bool per_vertex);
 // Bytes a single-graph count of (nv, ne) needs when every scratch buffer is a
