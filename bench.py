@@ -332,50 +332,54 @@ def batch_workload(ctx, n_graphs=4096, runs=5, tune=True):
     return out
 
 
-def serve_workload(ctx, rate, n_req=4000, pool=400, seed=7, borrow=True):
+_SERVE_POOL = {}
+
+
+def _serve_pool(ctx, pool):
+    """The request pool: `pool` graphs of the batch mix plus R-MAT s18, s19 and
+    s20 (the 1 % large requests), each with its solo count on this GPU."""
+    import xxtric_amd
+    if pool not in _SERVE_POOL:
+        graphs = [xxtric_amd.batch_graph_csr(2026, g) for g in range(pool)]
+        for s in (18, 19, 20):
+            dg = ctx.rmat(s, 16, 100 + s, True)
+            graphs.append(dg.download())
+            dg.close()
+        graphs = [(np.ascontiguousarray(o, np.int64), np.ascontiguousarray(t, np.int32)) for o, t in graphs]
+        solo = np.array([ctx.count_csr(o, t) for o, t in graphs], np.uint64)
+        _SERVE_POOL[pool] = (graphs, solo)
+    return _SERVE_POOL[pool]
+
+
+def serve_workload(ctx, rate, n_req=4000, pool=400, seed=7):
+    """Poisson arrivals at `rate` graphs/s of the batch mix plus 1 % R-MAT
+    s18-s20 requests, submitted by the library's native request generator
+    (xt_serve_replay: borrowed submissions at their scheduled times, every
+    result checked against the graph's solo count)."""
     import xxtric_amd
     rng = np.random.default_rng(seed)
-    graphs = [xxtric_amd.batch_graph_csr(2026, g) for g in range(pool)]
-    for s in (18, 19, 20):                       # 1% large R-MAT requests
-        dg = ctx.rmat(s, 16, 100 + s, True)
-        o, t = dg.download()
-        graphs.append((o, t))
-        dg.close()
-    # the request arrays in the layout the C ABI takes (no conversion per submit)
-    graphs = [(np.ascontiguousarray(o, np.int64), np.ascontiguousarray(t, np.int32)) for o, t in graphs]
+    graphs, solo = _serve_pool(ctx, pool)
     kinds = rng.integers(0, pool, n_req)
     big = rng.random(n_req) < 0.01
     kinds[big] = pool + rng.integers(0, 3, big.sum())
     arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req)) * 1e6  # us
     srv = xxtric_amd.Server(ctx, hbm_budget_bytes=16 << 30, max_batch_graphs=4096, max_inflight=2)
     base = srv.now_us() + 1000.0
-    tickets = []
-    for k, a in zip(kinds, arrivals):
-        while srv.now_us() < base + a:
-            pass
-        o, t = graphs[k]
-        tickets.append((srv.submit(o, t, base + a, borrow=borrow), k))
-    srv.drain(120.0)
-    solo = {}
-    ok = True
-    lat_small, lat_large = [], []
-    for tk, k in tickets:
-        tot, lat = srv.poll(tk)
-        (lat_large if k >= pool else lat_small).append(lat)
-        if k not in solo:
-            solo[k] = ctx.count_csr(*graphs[k])
-        ok &= tot == solo[k]
+    bad, lat = srv.replay(graphs, solo, kinds, base + arrivals)
     st = srv.stats()
     srv.close()
-    pct = lambda v, q: round(float(np.percentile(v, q)), 1) if v else None  # noqa: E731
+    ok = lat >= 0
+    small, large = lat[ok & (kinds < pool)], lat[ok & (kinds >= pool)]
+    pct = lambda v, q: round(float(np.percentile(v, q)), 1) if len(v) else None  # noqa: E731
     return {"rate_graphs_per_s": rate, "requests": n_req, "graphs_per_s": round(st["graphs_per_s"], 1),
             "p50_us": round(st["p50_us"], 1), "p99_us": round(st["p99_us"], 1),
-            "batches": st["batches"], "counts_equal_solo": bool(ok),
+            "batches": st["batches"], "isolated_batches": st["isolated_batches"],
+            "rejected": st["rejected"], "counts_equal_solo": bad == 0 and st["rejected"] == 0,
+            "submitter": "native (xt_serve_replay)",
             # the same latencies split by request class: the pool's small graphs and the
             # 1 % R-MAT s18-s20 requests
-            "small": {"requests": len(lat_small), "p50_us": pct(lat_small, 50), "p99_us": pct(lat_small, 99)},
-            "rmat_s18_s20": {"requests": len(lat_large), "p50_us": pct(lat_large, 50),
-                             "p99_us": pct(lat_large, 99)}}
+            "small": {"requests": int(len(small)), "p50_us": pct(small, 50), "p99_us": pct(small, 99)},
+            "rmat_s18_s20": {"requests": int(len(large)), "p50_us": pct(large, 50), "p99_us": pct(large, 99)}}
 
 
 def copy_calibration(ctx, nbytes=1 << 30, reps=3):
@@ -453,8 +457,8 @@ def main():
         print(json.dumps(memory_ceilings(ctx)), flush=True)
         return
     if args.serve_sweep:
-        for rate in (1000, 2000, 5000, 10000, 20000, 50000):
-            print(json.dumps(serve_workload(ctx, float(rate), n_req=max(2000, min(20000, rate)))),
+        for rate in (1000, 2000, 5000, 10000, 20000, 50000, 100000):
+            print(json.dumps(serve_workload(ctx, float(rate), n_req=max(2000, min(40000, rate)))),
                   flush=True)
         return
     tune = not args.no_autotune

@@ -93,6 +93,10 @@ int xt_ctx_tune_tiles(xt_ctx* ctx, int items_per_tile, int tile_run);
  * out-degrees for classification) -- for tests and comparisons on graphs of
  * any size.  Changes no result. */
 int xt_ctx_force_wide(xt_ctx* ctx, int on);
+/* Failure injection for tests: the next n device allocations the context
+ * makes for its scratch (counts, batches, servers) fail with XT_ERR_NOMEM as
+ * hipMalloc running out would.  0 turns it off. */
+int xt_ctx_fail_allocs(xt_ctx* ctx, int n);
 
 /* ---- device memory plumbing ------------------------------------------------
  * The CudaMalloc / CudaMemcpyHtoD / CudaMemcpyDtoH / CudaFree /
@@ -344,6 +348,7 @@ typedef struct xt_serve_stats {
   double p50_us, p99_us, mean_us, max_us; /* arrival -> result latency */
   double graphs_per_s;                    /* completed / (last done - first arrival) */
   int64_t peak_resident_bytes;
+  int64_t isolated_batches;               /* batches holding one large request alone */
 } xt_serve_stats;
 int xt_serve_create(xt_ctx* ctx, const xt_serve_config* cfg, xt_server** out);
 /* Non-blocking; the CSR is copied into server-owned host memory before return.
@@ -357,9 +362,34 @@ int xt_serve_submit(xt_server* s, int64_t nv, int64_t ne, const int64_t* offsets
  * that fails the check is reported by xt_serve_poll (XT_ERR_INPUT). */
 int xt_serve_submit_borrowed(xt_server* s, int64_t nv, int64_t ne, const int64_t* offsets,
                              const int32_t* tails, double arrival_us, uint64_t* ticket);
-/* XT_OK with the count when done, XT_PENDING while queued or running. */
+/* XT_OK with the count when done, XT_PENDING while queued or running.  A
+ * request's own failure (XT_ERR_INPUT: a borrowed request failed its check;
+ * XT_ERR_INTERNAL: it could not be staged) retires its ticket like a result.
+ * A server-wide error is returned for a request that has not finished, whose
+ * ticket stays live: borrowed arrays are released only once
+ * xt_serve_ticket_state reports XT_ERR_NOTFOUND. */
 int xt_serve_poll(xt_server* s, uint64_t ticket, uint64_t* total, double* latency_us);
+#define XT_SERVE_QUEUED 0
+#define XT_SERVE_RUNNING 1
+#define XT_SERVE_DONE 2
+/* XT_SERVE_QUEUED / _RUNNING / _DONE (not yet polled), or XT_ERR_NOTFOUND for a
+ * ticket that was never issued or whose result has been reported. */
+int xt_serve_ticket_state(xt_server* s, uint64_t ticket);
+/* paused != 0 holds admission (queued requests wait, submissions still queue
+ * up to max_queue); running batches finish.  For tests and controlled
+ * shutdowns. */
+int xt_serve_pause(xt_server* s, int paused);
 int xt_serve_drain(xt_server* s, double timeout_s);
+/* Native request generator for load tests: submits request i (borrowed) of
+ * pool graph order[i] at arrival_us[i] on the server clock (waiting until
+ * then), drains, polls every ticket and counts results different from
+ * expected[graph] (nullable) into *mismatches; latency_us (nullable, nreq)
+ * gets each latency, -1 for a request rejected at submit (XT_ERR_CAPACITY,
+ * counted in the stats).  The pool arrays stay valid for the call. */
+int xt_serve_replay(xt_server* s, int64_t npool, const int64_t* nv, const int64_t* ne,
+                    const int64_t* const* offsets, const int32_t* const* tails, const uint64_t* expected,
+                    int64_t nreq, const int64_t* order, const double* arrival_us, double timeout_s,
+                    int64_t* mismatches, double* latency_us);
 int xt_serve_stats_get(xt_server* s, xt_serve_stats* out);
 double xt_serve_now_us(const xt_server* s);
 int xt_serve_destroy(xt_server* s);

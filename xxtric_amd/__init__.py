@@ -8,6 +8,7 @@ the seeded random-geometric generator).  All counting happens in the HIP
 kernels of the library; this module only moves arrays and pointers.
 """
 import ctypes
+import weakref
 
 import numpy as np
 
@@ -49,6 +50,13 @@ class Context:
             raise XtError(rc, "xt_ctx_create(%d) failed (no HIP device?)" % device)
         self.h = h
         self.device = device
+        # library objects made on this context (graphs, buffers, batches,
+        # servers): closed before the context itself, since each uses it
+        self._children = weakref.WeakSet()
+
+    def _adopt(self, obj):
+        self._children.add(obj)
+        return obj
 
     def _check(self, rc):
         if rc != capi.XT_OK:
@@ -57,6 +65,11 @@ class Context:
 
     def close(self):
         if self.h:
+            for obj in list(self._children):
+                try:
+                    obj.close()
+                except Exception:
+                    pass
             self._L.xt_ctx_destroy(self.h)
             self.h = None
 
@@ -80,6 +93,10 @@ class Context:
         self._check(self._L.xt_ctx_tune(self.h, small_cut, group_blocks, group_round))
         if items_per_tile or tile_run:
             self._check(self._L.xt_ctx_tune_tiles(self.h, items_per_tile, tile_run))
+
+    def fail_allocs(self, n):
+        """Failure injection: the next n scratch allocations fail with XT_ERR_NOMEM."""
+        self._check(self._L.xt_ctx_fail_allocs(self.h, int(n)))
 
     def force_wide(self, on=True):
         """Count with the layout of graphs of 2^32 or more entries (tests, comparisons)."""
@@ -196,6 +213,7 @@ class DeviceBuffer:
         p = ctypes.c_void_p()
         ctx._check(ctx._L.xt_dev_alloc(ctx.h, self.nbytes, ctypes.byref(p)))
         self.ptr = p.value
+        ctx._adopt(self)
 
     def upload(self, arr):
         arr = np.ascontiguousarray(arr)
@@ -233,6 +251,7 @@ class DeviceGraph:
         ctx._L.xt_graph_info(h, ctypes.byref(nv), ctypes.byref(ne))
         self.nv = nv.value
         self.ne = ne.value
+        ctx._adopt(self)
 
     def count(self, per_vertex=False, orient="degree"):
         total = ctypes.c_uint64()

@@ -21,6 +21,7 @@ class Batch:
                                            ctypes.byref(h)))
         self.h = h
         self.n = 0
+        ctx._adopt(self)
 
     def add(self, off, tails):
         off = np.ascontiguousarray(off, np.int64)
@@ -61,9 +62,9 @@ class Batch:
         self.n = 0
 
     def close(self):
-        if self.h:
+        if self.h and self.ctx.h:  # the context closes its batches before itself
             self._L.xt_batch_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:
@@ -85,6 +86,7 @@ class Server:
         ctx._check(self._L.xt_serve_create(ctx.h, ctypes.byref(cfg), ctypes.byref(h)))
         self.h = h
         self._borrowed = {}  # ticket -> arrays the server reads in place
+        ctx._adopt(self)
 
     def submit(self, off, tails, arrival_us=-1.0, borrow=False):
         """Queue a graph.  borrow=True skips the copy: the server reads the arrays
@@ -106,10 +108,47 @@ class Server:
         rc = self._L.xt_serve_poll(self.h, ticket, ctypes.byref(total), ctypes.byref(lat))
         if rc == capi.XT_PENDING:
             return None
-        self._borrowed.pop(ticket, None)
+        # A borrowed request's arrays are let go only once its ticket is retired
+        # (its result or its own failure reported); after a server-wide error the
+        # request may still be read.
+        if rc == capi.XT_OK or self._L.xt_serve_ticket_state(self.h, ticket) == capi.XT_ERR_NOTFOUND:
+            self._borrowed.pop(ticket, None)
         if rc != capi.XT_OK:
             raise XtError(rc, "xt_serve_poll")
         return total.value, lat.value
+
+    def ticket_state(self, ticket):
+        """0 queued, 1 running, 2 done (not yet polled); XT_ERR_NOTFOUND when retired."""
+        return self._L.xt_serve_ticket_state(self.h, ticket)
+
+    def pause(self, paused=True):
+        """Hold admission (queued requests wait; running batches finish)."""
+        rc = self._L.xt_serve_pause(self.h, 1 if paused else 0)
+        if rc != capi.XT_OK:
+            raise XtError(rc, "xt_serve_pause")
+
+    def replay(self, pool, expected, order, arrival_us, timeout_s=300.0):
+        """Native load generator (xt_serve_replay): request i is pool graph order[i],
+        submitted at arrival_us[i] on the server clock.  pool: [(off int64, tails
+        int32)], expected: per-graph counts (or None).  Returns (mismatches,
+        latencies in us, -1 for requests rejected at submit)."""
+        pool = [(np.ascontiguousarray(o, np.int64), np.ascontiguousarray(t, np.int32)) for o, t in pool]
+        n = len(pool)
+        nv = np.array([len(o) - 1 for o, _ in pool], np.int64)
+        ne = np.array([len(t) for _, t in pool], np.int64)
+        offs = (ctypes.c_void_p * n)(*[o.ctypes.data for o, _ in pool])
+        tls = (ctypes.c_void_p * n)(*[t.ctypes.data if len(t) else 0 for _, t in pool])
+        exp = None if expected is None else np.ascontiguousarray(expected, np.uint64)
+        order = np.ascontiguousarray(order, np.int64)
+        arrival_us = np.ascontiguousarray(arrival_us, np.float64)
+        lat = np.empty(len(order), np.float64)
+        bad = ctypes.c_int64()
+        rc = self._L.xt_serve_replay(self.h, n, capi.ptr(nv), capi.ptr(ne), ctypes.addressof(offs),
+                                     ctypes.addressof(tls), capi.ptr(exp), len(order), capi.ptr(order),
+                                     capi.ptr(arrival_us), timeout_s, ctypes.byref(bad), capi.ptr(lat))
+        if rc != capi.XT_OK:
+            raise XtError(rc, "xt_serve_replay")
+        return bad.value, lat
 
     def drain(self, timeout_s=60.0):
         rc = self._L.xt_serve_drain(self.h, timeout_s)
@@ -127,9 +166,10 @@ class Server:
         return {k: getattr(s, k) for k, _ in s._fields_}
 
     def close(self):
-        if self.h:
+        if self.h and self.ctx.h:  # the context closes its servers before itself
             self._L.xt_serve_destroy(self.h)
-            self.h = None
+        self.h = None
+        self._borrowed.clear()
         self._borrowed.clear()
 
     def __del__(self):

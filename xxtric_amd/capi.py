@@ -75,7 +75,7 @@ class XtServeConfig(ctypes.Structure):
 class XtServeStats(ctypes.Structure):
     _fields_ = [("submitted", _i64), ("completed", _i64), ("rejected", _i64), ("batches", _i64),
                 ("p50_us", _dbl), ("p99_us", _dbl), ("mean_us", _dbl), ("max_us", _dbl),
-                ("graphs_per_s", _dbl), ("peak_resident_bytes", _i64)]
+                ("graphs_per_s", _dbl), ("peak_resident_bytes", _i64), ("isolated_batches", _i64)]
 
 
 _SIGS = {
@@ -87,6 +87,7 @@ _SIGS = {
     "xt_ctx_tune": (_int, [_p, _int, _int, _int]),
     "xt_ctx_tune_tiles": (_int, [_p, _int, _int]),
     "xt_ctx_force_wide": (_int, [_p, _int]),
+    "xt_ctx_fail_allocs": (_int, [_p, _int]),
     "xt_dev_alloc": (_int, [_p, ctypes.c_size_t, _pp]),
     "xt_dev_free": (_int, [_p, _p]),
     "xt_memcpy": (_int, [_p, _p, _p, ctypes.c_size_t, _int]),
@@ -134,8 +135,11 @@ _SIGS = {
     "xt_serve_submit_borrowed": (_int, [_p, _i64, _i64, _p, _p, _dbl, _p]),
     "xt_serve_poll": (_int, [_p, _u64, _p, _p]),
     "xt_serve_drain": (_int, [_p, _dbl]),
+    "xt_serve_replay": (_int, [_p, _i64, _p, _p, _p, _p, _p, _i64, _p, _p, _dbl, _p, _p]),
     "xt_serve_stats_get": (_int, [_p, ctypes.POINTER(XtServeStats)]),
     "xt_serve_now_us": (_dbl, [_p]),
+    "xt_serve_ticket_state": (_int, [_p, _u64]),
+    "xt_serve_pause": (_int, [_p, _int]),
     "xt_serve_destroy": (_int, [_p]),
 }
 

@@ -41,7 +41,15 @@ __global__ __launch_bounds__(256) void k_stream_read(const int4* __restrict__ a,
                                                      uint32_t* __restrict__ sink) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   uint32_t acc = 0;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i + 7 * stride < n; i += 8 * stride) {  // eight coalesced 16-byte loads in flight
+    int4 v[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) v[j] = a[i + j * stride];
+#pragma unroll
+    for (int j = 0; j < 8; j++) acc ^= (uint32_t)(v[j].x ^ v[j].y ^ v[j].z ^ v[j].w);
+  }
+  for (; i < n; i += stride) {
     const int4 v = a[i];
     acc ^= (uint32_t)(v.x ^ v.y ^ v.z ^ v.w);
   }
@@ -92,6 +100,11 @@ int ensure(xt_ctx* ctx, DevBuf& b, size_t bytes) {
     b.bytes = 0;
   }
   const size_t want = (bytes + ((size_t)1 << 21) - 1) & ~(((size_t)1 << 21) - 1);
+  if (ctx && ctx->fail_allocs > 0) {
+    ctx->fail_allocs--;
+    return set_err(ctx, XT_ERR_NOMEM, std::string("device allocation of ") + std::to_string(want) +
+                                          " bytes failed (injected)");
+  }
   hipError_t e = hipMalloc(&b.p, want);
   if (e != hipSuccess) {
     b.p = nullptr;
@@ -334,6 +347,15 @@ int xt_ctx_tune_tiles(xt_ctx* ctx, int items_per_tile, int tile_run) {
     if (items_per_tile == -1) ctx->items_per_tile = 256;
     if (tile_run > 0) ctx->tile_run = tile_run;
     if (tile_run == -1) ctx->tile_run = 16;
+    return XT_OK;
+  });
+}
+
+int xt_ctx_fail_allocs(xt_ctx* ctx, int n) {
+  return xt_guarded([&]() -> int {
+    if (!ctx || n < 0) return XT_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    ctx->fail_allocs = n;
     return XT_OK;
   });
 }

@@ -8,7 +8,9 @@
 // uploads them and launches the persistent batch pipeline on the slot's
 // stream; slots in flight run concurrently on the device.  Finished slots are
 // read back, their requests completed (latency = completion - arrival) and
-// their footprint released, which admits the next requests.
+// their footprint released, which admits the next requests.  A large
+// admission is copied into its slot's staging memory by a pool of host
+// threads while the worker goes on serving the other slots.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -101,10 +103,25 @@ void run_copy(const CopyTask& t) {
   }
 }
 
-// A run hands out task indices from one counter; a thread takes part in a run
-// only after registering (under the lock, while the run's task list is
-// current), and run() returns once every task is done and every registered
-// thread has left, so no thread keeps a run's task list past it.
+// A copy job: its tasks are handed out from one counter to the pool's
+// threads (and to the worker when it helps); `done` counts finished tasks.
+struct CopyJob {
+  std::vector<CopyTask> tasks;
+  std::atomic<size_t> next{0};
+  std::atomic<size_t> done{0};
+  bool finished() const { return done.load(std::memory_order_acquire) == tasks.size(); }
+  // Runs unclaimed tasks in the calling thread; false once none is left.
+  bool help() {
+    const size_t i = next.fetch_add(1);
+    if (i >= tasks.size()) return false;
+    run_copy(tasks[i]);
+    done.fetch_add(1, std::memory_order_release);
+    return true;
+  }
+};
+
+// Host threads that run copy jobs in submission order.  A job stays listed
+// until all its tasks are claimed; its owner keeps it alive until finished().
 class CopyPool {
  public:
   explicit CopyPool(int n) {
@@ -118,66 +135,65 @@ class CopyPool {
     cv_.notify_all();
     for (std::thread& t : th_) t.join();
   }
-  // Runs every task (the calling thread helps) and returns when all are done.
-  void run(const std::vector<CopyTask>& tasks) {
-    if (tasks.empty()) return;
+  void submit(const std::shared_ptr<CopyJob>& job) {
+    if (job->tasks.empty()) return;
     {
       std::lock_guard<std::mutex> lk(mu_);
-      tasks_ = &tasks;
-      next_.store(0);
-      left_ = tasks.size();
-      active_ = 1;  // this thread
-      gen_++;
+      jobs_.push_back(job);
     }
     cv_.notify_all();
-    work();
-    std::unique_lock<std::mutex> lk(mu_);
-    done_cv_.wait(lk, [&] { return left_ == 0 && active_ == 0; });
-    tasks_ = nullptr;
+  }
+  // Submits the job, helps with it, and returns when every task is done.
+  void run(const std::shared_ptr<CopyJob>& job) {
+    submit(job);
+    while (job->help()) {
+    }
+    while (!job->finished()) std::this_thread::yield();
   }
 
  private:
-  void work() {  // the caller is registered in active_
-    const std::vector<CopyTask>& ts = *tasks_;
-    size_t finished = 0;
-    for (size_t i = next_.fetch_add(1); i < ts.size(); i = next_.fetch_add(1)) {
-      run_copy(ts[i]);
-      finished++;
-    }
-    std::lock_guard<std::mutex> lk(mu_);
-    left_ -= finished;
-    active_--;
-    if (left_ == 0 && active_ == 0) done_cv_.notify_all();
-  }
   void loop() {
-    uint64_t seen = 0;
     while (true) {
+      std::shared_ptr<CopyJob> job;
       {
         std::unique_lock<std::mutex> lk(mu_);
-        cv_.wait(lk, [&] { return stop_ || (gen_ != seen && tasks_ != nullptr); });
+        cv_.wait(lk, [&] { return stop_ || !jobs_.empty(); });
         if (stop_) return;
-        seen = gen_;
-        active_++;
+        job = jobs_.front();
       }
-      work();
+      if (!job->help()) {  // every task of this job is claimed: unlist it
+        std::lock_guard<std::mutex> lk(mu_);
+        if (!jobs_.empty() && jobs_.front() == job) jobs_.pop_front();
+        continue;
+      }
+      while (job->help()) {
+      }
     }
   }
   std::vector<std::thread> th_;
   std::mutex mu_;
-  std::condition_variable cv_, done_cv_;
-  const std::vector<CopyTask>* tasks_ = nullptr;
-  std::atomic<size_t> next_{0};
-  size_t left_ = 0;
-  int active_ = 0;
-  uint64_t gen_ = 0;
+  std::condition_variable cv_;
+  std::deque<std::shared_ptr<CopyJob>> jobs_;
   bool stop_ = false;
 };
+
+// A large request admitted alone (below saturation) with at least this many
+// directed entries is copied into its slot's staging mirror by the pool while
+// the worker goes on serving the other slots; every other admission is copied
+// by the pool with the worker's help before its launch.
+constexpr int64_t kAsyncStage = int64_t(1) << 20;
 
 struct ServeSlot {
   xt_batch* b = nullptr;
   std::vector<Req*> reqs;
   int64_t V = 0, E = 0;
   bool inflight = false;
+  // staging by the pool: the job, and per request its destination and check flag
+  bool staging = false;
+  std::shared_ptr<CopyJob> job;
+  std::vector<int64_t*> odst;
+  std::vector<int32_t*> tdst;
+  std::unique_ptr<std::atomic<int>[]> bad;
 };
 
 }  // namespace
@@ -195,9 +211,10 @@ struct xt_server {
   int64_t resident = 0, peak_resident = 0;
   std::vector<double> latencies;
   double first_arrival = -1, last_done = 0;
-  int64_t submitted = 0, completed = 0, rejected = 0, batches = 0;
+  int64_t submitted = 0, completed = 0, rejected = 0, batches = 0, isolated = 0;
   int worker_err = XT_OK;
   bool stop = false;
+  bool paused = false;  // admission held (xt_serve_pause); running batches still finish
   std::unique_ptr<CopyPool> pool;
   std::thread worker;
   std::chrono::steady_clock::time_point t0;
@@ -233,6 +250,34 @@ void complete_slot(xt_server* s, ServeSlot& sl) {
   s->cv_done.notify_all();
 }
 
+// The slot's requests are in its staging mirror: requests that failed their
+// check become harmless graphs (reported at poll), host copies are released,
+// and the slot's batch is launched.
+void launch_staged(xt_server* s, ServeSlot& sl) {
+  for (size_t i = 0; i < sl.reqs.size(); i++) {
+    Req* r = sl.reqs[i];
+    if (sl.bad[i].load()) {
+      // Keep the slot's layout: the graph becomes rows of vertex 0 ids
+      // (valid, never reported) and the request fails at poll.
+      for (int64_t v = 0; v < r->nv; v++) sl.odst[i][v] = 0;
+      sl.odst[i][r->nv] = r->ne;
+      if (r->ne > 0) std::memset(sl.tdst[i], 0, (size_t)r->ne * sizeof(int32_t));
+      std::lock_guard<std::mutex> lk(s->mu);
+      r->err = XT_ERR_INPUT;
+      s->rejected++;
+    }
+    r->off.reset();  // host copies no longer needed
+    r->tails.reset();
+  }
+  sl.staging = false;
+  sl.job.reset();
+  int rc = batch_launch(sl.b, XT_BATCH_PERSISTENT, XT_ORIENT_DEGREE);
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (rc != XT_OK) s->worker_err = rc;
+  sl.inflight = true;
+  s->batches++;
+}
+
 void worker_main(xt_server* s) {
   (void)hipSetDevice(s->ctx->device);
   while (true) {
@@ -246,15 +291,30 @@ void worker_main(xt_server* s) {
     // Fill one free slot from the queue (arrival order), then launch it.
     ServeSlot* free_slot = nullptr;
     for (ServeSlot& sl : s->slots)
-      if (!sl.inflight) {
+      if (!sl.inflight && !sl.staging) {
         free_slot = &sl;
         break;
       }
-    if (free_slot) {
+    bool held = false;
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      held = s->paused && !s->stop;
+    }
+    if (free_slot && !held) {
       std::vector<Req*> take;
+      bool alone = false;  // one large request admitted by itself (below saturation)
       {
         std::unique_lock<std::mutex> lk(s->mu);
-        if (s->stop && s->queue.empty()) break;
+        if (s->stop && s->queue.empty()) {
+          // Background copies still read request arrays and write the slots'
+          // staging memory: they end before the worker does.
+          lk.unlock();
+          for (ServeSlot& o : s->slots)
+            if (o.staging)
+              while (!o.job->finished())
+                if (!o.job->help()) std::this_thread::yield();
+          break;
+        }
         if (!s->queue.empty() && s->cfg.max_wait_us > 0) {
           // batching window: let a short queue fill unless a slot's worth is waiting
           const double wait_until = s->queue.front()->arrival_us + s->cfg.max_wait_us;
@@ -284,67 +344,88 @@ void worker_main(xt_server* s) {
           V += r->nv;
           E += r->ne;
           G++;
-          if (large) break;
+          if (large) {
+            s->isolated++;
+            alone = true;
+            break;
+          }
         }
       }
       // Lay the admitted requests out in the slot's pinned mirror, then copy
-      // (and check the borrowed ones) with the copy pool.
-      std::vector<CopyTask> tasks;
-      std::vector<int64_t*> odst(take.size());
-      std::vector<int32_t*> tdst(take.size());
-      std::unique_ptr<std::atomic<int>[]> bad(new std::atomic<int>[take.size()]);
+      // (and check the borrowed ones) with the copy pool: inline for small
+      // admissions, in the background for large ones.
+      ServeSlot& sl = *free_slot;
+      sl.job = std::make_shared<CopyJob>();
+      sl.odst.assign(take.size(), nullptr);
+      sl.tdst.assign(take.size(), nullptr);
+      sl.bad.reset(new std::atomic<int>[take.size()]);
+      int64_t entries = 0;
       for (size_t i = 0; i < take.size(); i++) {
         Req* r = take[i];
-        bad[i].store(0);
-        int rc = batch_stage_layout(free_slot->b, r->nv, r->ne, &odst[i], &tdst[i]);
+        sl.bad[i].store(0);
+        int rc = batch_stage_layout(sl.b, r->nv, r->ne, &sl.odst[i], &sl.tdst[i]);
         if (rc != XT_OK) {
+          // Admission keeps a slot's graphs within its arena, so this is an
+          // internal failure: this request and the rest of the admitted ones
+          // end with an error and give their footprint back.
           std::lock_guard<std::mutex> lk(s->mu);
-          s->worker_err = rc;
+          const double t = s->now();
+          for (size_t j = i; j < take.size(); j++) {
+            Req* q = take[j];
+            q->err = XT_ERR_INTERNAL;
+            q->state = kDone;
+            q->done_us = t;
+            q->off.reset();
+            q->tails.reset();
+            s->resident -= q->footprint;
+            s->rejected++;
+          }
+          take.resize(i);
+          s->cv_done.notify_all();
           break;
         }
-        free_slot->reqs.push_back(r);
-        free_slot->V += r->nv;
-        free_slot->E += r->ne;
+        sl.reqs.push_back(r);
+        sl.V += r->nv;
+        sl.E += r->ne;
+        entries += r->ne;
         const bool borrowed = r->b_off != nullptr;
         const int64_t* off = borrowed ? r->b_off : r->off.get();
         const int32_t* tails = borrowed ? r->b_tails : r->tails.get();
-        if (borrowed && (off[0] != 0 || off[r->nv] != r->ne)) bad[i].store(1);
+        if (borrowed && (off[0] != 0 || off[r->nv] != r->ne)) sl.bad[i].store(1);
         for (int64_t lo = 0; lo <= r->nv; lo += kCopyRange)
-          tasks.push_back(CopyTask{off, nullptr, odst[i], nullptr, r->nv, lo, std::min(r->nv + 1, lo + kCopyRange),
-                                   borrowed, &bad[i]});
+          sl.job->tasks.push_back(CopyTask{off, nullptr, sl.odst[i], nullptr, r->nv, lo,
+                                           std::min(r->nv + 1, lo + kCopyRange), borrowed, &sl.bad[i]});
         for (int64_t lo = 0; lo < r->ne; lo += kCopyRange)
-          tasks.push_back(CopyTask{nullptr, tails, nullptr, tdst[i], r->nv, lo, std::min(r->ne, lo + kCopyRange),
-                                   borrowed, &bad[i]});
+          sl.job->tasks.push_back(CopyTask{nullptr, tails, nullptr, sl.tdst[i], r->nv, lo,
+                                           std::min(r->ne, lo + kCopyRange), borrowed, &sl.bad[i]});
       }
-      s->pool->run(tasks);
-      for (size_t i = 0; i < free_slot->reqs.size() && i < take.size(); i++) {
-        Req* r = take[i];
-        if (bad[i].load()) {
-          // Keep the slot's layout: the graph becomes rows of vertex 0 ids
-          // (valid, never reported) and the request fails at poll.
-          for (int64_t v = 0; v < r->nv; v++) odst[i][v] = 0;
-          odst[i][r->nv] = r->ne;
-          if (r->ne > 0) std::memset(tdst[i], 0, (size_t)r->ne * sizeof(int32_t));
-          std::lock_guard<std::mutex> lk(s->mu);
-          r->err = XT_ERR_INPUT;
-          s->rejected++;
+      if (!sl.reqs.empty()) {
+        if (alone && entries >= kAsyncStage) {
+          // Below saturation: the large request is copied in the background
+          // while the worker keeps launching small graphs in the other slots.
+          s->pool->submit(sl.job);
+          sl.staging = true;
+        } else {
+          // Near saturation everything is batched, and the worker helps the
+          // pool copy, for throughput.
+          s->pool->run(sl.job);
+          launch_staged(s, sl);
         }
-        r->off.reset();  // host copies no longer needed
-        r->tails.reset();
-      }
-      if (!free_slot->reqs.empty()) {
-        int rc = batch_launch(free_slot->b, XT_BATCH_PERSISTENT, XT_ORIENT_DEGREE);
-        std::lock_guard<std::mutex> lk(s->mu);
-        if (rc != XT_OK) s->worker_err = rc;
-        free_slot->inflight = true;
-        s->batches++;
         progressed = true;
+      } else {
+        sl.job.reset();
       }
     }
+    // Slots whose background staging has finished are launched.
+    for (ServeSlot& sl : s->slots)
+      if (sl.staging && sl.job->finished()) {
+        launch_staged(s, sl);
+        progressed = true;
+      }
     if (!progressed) {
       std::unique_lock<std::mutex> lk(s->mu);
       bool any_inflight = false;
-      for (ServeSlot& sl : s->slots) any_inflight |= sl.inflight;
+      for (ServeSlot& sl : s->slots) any_inflight |= sl.inflight || sl.staging;
       if (s->stop && s->queue.empty() && !any_inflight) break;
       s->cv.wait_for(lk, std::chrono::microseconds(any_inflight ? 20 : 200));
     }
@@ -471,21 +552,83 @@ int xt_serve_poll(xt_server* s, uint64_t ticket, uint64_t* total, double* latenc
   return xt_guarded([&]() -> int {
     if (!s) return XT_ERR_ARG;
     std::lock_guard<std::mutex> lk(s->mu);
-    if (s->worker_err != XT_OK) return s->worker_err;
     auto it = s->live.find(ticket);
     if (it == s->live.end()) return XT_ERR_NOTFOUND;
     Req* r = it->second;
-    if (r->state != kDone) return XT_PENDING;
-    if (r->err != XT_OK) {  // a borrowed request that failed its check
+    // A request that has not finished stays live (and a borrowed one keeps
+    // being read) whatever the server-wide state; its own result, once there,
+    // is reported and the ticket retired.
+    if (r->state != kDone) return s->worker_err != XT_OK ? s->worker_err : XT_PENDING;
+    if (r->err != XT_OK) {  // failed its check, or could not be staged
       const int err = r->err;
       s->live.erase(it);
       delete r;
-      return set_err(s->ctx, err, "request rejected: adjacency out of range or offsets not monotone");
+      return set_err(s->ctx, err, err == XT_ERR_INPUT
+                                      ? "request rejected: adjacency out of range or offsets not monotone"
+                                      : "request failed: it could not be staged into its batch slot");
     }
     if (total) *total = r->total;
     if (latency_us) *latency_us = r->done_us - r->arrival_us;
     s->live.erase(it);
     delete r;
+    return XT_OK;
+  });
+}
+
+int xt_serve_ticket_state(xt_server* s, uint64_t ticket) {
+  return xt_guarded([&]() -> int {
+    if (!s) return XT_ERR_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->live.find(ticket);
+    if (it == s->live.end()) return XT_ERR_NOTFOUND;
+    return it->second->state == kDone ? XT_SERVE_DONE : it->second->state == kRunning ? XT_SERVE_RUNNING
+                                                                                        : XT_SERVE_QUEUED;
+  });
+}
+
+int xt_serve_pause(xt_server* s, int paused) {
+  return xt_guarded([&]() -> int {
+    if (!s) return XT_ERR_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->paused = paused != 0;
+    s->cv.notify_all();
+    return XT_OK;
+  });
+}
+
+int xt_serve_replay(xt_server* s, int64_t npool, const int64_t* nv, const int64_t* ne,
+                    const int64_t* const* offsets, const int32_t* const* tails, const uint64_t* expected,
+                    int64_t nreq, const int64_t* order, const double* arrival_us, double timeout_s,
+                    int64_t* mismatches, double* latency_us) {
+  return xt_guarded([&]() -> int {
+    if (!s || npool < 1 || !nv || !ne || !offsets || !tails || nreq < 0 || (nreq > 0 && (!order || !arrival_us)))
+      return XT_ERR_ARG;
+    for (int64_t i = 0; i < nreq; i++)
+      if (order[i] < 0 || order[i] >= npool) return set_err(s->ctx, XT_ERR_ARG, "replay: pool index out of range");
+    std::vector<uint64_t> tk((size_t)nreq, 0);
+    // Each request is submitted (borrowed) at its scheduled arrival time on
+    // the server clock, which is also the arrival its latency is measured from.
+    for (int64_t i = 0; i < nreq; i++) {
+      while (s->now() < arrival_us[i]) std::this_thread::yield();
+      const int64_t k = order[i];
+      const int rc = xt_serve_submit_borrowed(s, nv[k], ne[k], offsets[k], tails[k], arrival_us[i], &tk[i]);
+      if (rc != XT_OK && rc != XT_ERR_CAPACITY) return rc;  // rejected requests keep ticket 0
+    }
+    XT_TRY(xt_serve_drain(s, timeout_s));
+    int64_t bad = 0;
+    for (int64_t i = 0; i < nreq; i++) {
+      if (tk[i] == 0) {
+        if (latency_us) latency_us[i] = -1.0;
+        continue;
+      }
+      uint64_t tot = 0;
+      double lat = 0;
+      const int rc = xt_serve_poll(s, tk[i], &tot, &lat);
+      if (rc != XT_OK) return rc;
+      if (expected && tot != expected[order[i]]) bad++;
+      if (latency_us) latency_us[i] = lat;
+    }
+    if (mismatches) *mismatches = bad;
     return XT_OK;
   });
 }
@@ -512,6 +655,7 @@ int xt_serve_stats_get(xt_server* s, xt_serve_stats* out) {
     st.rejected = s->rejected;
     st.batches = s->batches;
     st.peak_resident_bytes = s->peak_resident;
+    st.isolated_batches = s->isolated;
     if (!s->latencies.empty()) {
       std::vector<double> v = s->latencies;
       std::sort(v.begin(), v.end());

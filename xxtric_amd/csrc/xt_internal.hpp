@@ -98,6 +98,7 @@ struct xt_ctx {
   int items_per_tile = 256;  // items of one group tile (64, 128 or 256)
   int tile_run = 16;         // consecutive tiles a group-kernel workgroup draws at once (1..64)
   bool force_wide = false;  // use the 64-bit offset layout of graphs with >= 2^32 entries everywhere
+  int fail_allocs = 0;      // failure injection: scratch allocations left to fail
 };
 
 struct xt_graph {
