@@ -73,8 +73,8 @@ const char* xt_last_error(const xt_ctx* ctx);
 int xt_last_timing(const xt_ctx* ctx, xt_timing* out);
 /* Tuning knobs (set by xxtric_amd.autotune from measurements; 0 keeps the
  * current value): light_cut -- edges whose two out-lists are both at most this
- * long are intersected one per thread, the rest by the group kernel (default
- * 32 for one graph, 64 for batches and the server; a value set here applies
+ * long are intersected one per thread, the rest by the group kernel or wave
+ * tiles (default 32 for one graph, 8 for batches and the server; a value set here applies
  * to both, -1 restores the defaults); group_blocks -- persistent group-kernel workgroups per CU (1..4,
  * default 4); group_round -- 16-byte list chunks each group-kernel thread has
  * in flight per streaming round (1, 2 or 4, default 2).  None of them changes
@@ -88,6 +88,15 @@ int xt_ctx_tune(xt_ctx* ctx, int light_cut, int group_blocks, int group_round);
  * changes a result.  The counterpart of the reference's compile-time tuning
  * macros (counting.cuh:150-176). */
 int xt_ctx_tune_tiles(xt_ctx* ctx, int items_per_tile, int tile_run);
+/* Wave tiles for plain counts (0 keeps the current value, -1 restores the
+ * defaults, -2 turns them off): owners whose out-list holds at most wave_owner
+ * ids (1..128) are counted one wave per tile of at most 64 items, with a
+ * 1 KB LDS table per wave, instead of by the 512-thread group kernel (default
+ * 128 for one graph, 64 for batches and the server).  Per-vertex and
+ * owned-range counts always use the group kernel.  Does not change a result.
+ * No counterpart in the reference, whose launch shapes are compile-time
+ * macros (counting.cuh:150-176). */
+int xt_ctx_tune_waves(xt_ctx* ctx, int wave_owner);
 /* on != 0: count every graph with the layout used for graphs of 2^32 or more
  * entries (64-bit list offsets gathered by the light pass, 16-bit saturated
  * out-degrees for classification) -- for tests and comparisons on graphs of
@@ -150,8 +159,8 @@ int xt_single_gpu_count(xt_ctx* ctx, int64_t nv, const int64_t* d_indices,
  * nothing (it still synchronises with `stream`, so it is not for stream
  * capture).  Its scratch is one statically sized 64 MiB device array per GPU,
  * held from the first context created on that GPU until the process ends; a
- * graph needing more (at most 68 bytes per vertex plus 20.3 per entry: e.g.
- * 65,536 vertices and 3 M entries fit) is rejected with XT_ERR_CAPACITY
+ * graph needing more (at most 80 bytes per vertex plus 20.7 per entry: e.g.
+ * 65,536 vertices and 2.9 M entries fit) is rejected with XT_ERR_CAPACITY
  * before any kernel runs.  All calls on one GPU share that array, so calls
  * that overlapped would race: the library serialises them per device and
  * each returns only after its last kernel has finished. */
@@ -265,6 +274,19 @@ int xt_rgg_generate(int64_t nv, uint32_t seed, double radius, int64_t* offsets,
 int xt_rgg_generate_ranks(int64_t nv, int nprocs, uint32_t seed, double radius, int64_t* offsets,
                           int64_t* tails, int64_t cap, int64_t* ne);
 double xt_rgg_radius(int64_t nv);
+/* xt_rgg_generate_ranks with options (flags, OR-ed):
+ *   XT_RGG_STRICT   edge iff sqrt(dx*dx+dy*dy) < r (the reference tests <=,
+ *                   graph.hpp:927,982,1014);
+ *   XT_RGG_CONNECT  radius <= 0 selects sqrt(2 ln nv / (pi nv)), the connectivity
+ *                   radius (xt_rgg_radius_connect), instead of graph.hpp:789-791.
+ * flags = 0 is xt_rgg_generate_ranks.  No counterpart in the reference: it
+ * builds the random geometric graph that the workload description "edge when
+ * Euclidean distance < sqrt(2 ln(n)/(pi n))" names, from the same points. */
+#define XT_RGG_STRICT 0x1u
+#define XT_RGG_CONNECT 0x2u
+int xt_rgg_generate_opts(int64_t nv, int nprocs, uint32_t seed, double radius, uint32_t flags,
+                         int64_t* offsets, int64_t* tails, int64_t cap, int64_t* ne);
+double xt_rgg_radius_connect(int64_t nv);
 /* vite binary file (graph.hpp:294-395); layout32 selects the USE_32_BIT_GRAPH
  * layout.  Header only when offsets == NULL; header and offsets (rebased to
  * start at 0, graph.hpp:386-388) when tails == NULL. */

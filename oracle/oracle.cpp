@@ -61,8 +61,14 @@ extern "C" void or_rgg_lcg_coords(int64_t n, uint32_t seed, double* X, double* Y
 
 extern "C" int64_t or_rgg_lcg(int64_t n, uint32_t seed, double radius, int64_t* off,
                               int64_t* tails, int64_t cap, int nthreads) {
+  return or_rgg_lcg_opts(n, seed, radius, 0, off, tails, cap, nthreads);
+}
+
+extern "C" int64_t or_rgg_lcg_opts(int64_t n, uint32_t seed, double radius, int strict, int64_t* off,
+                                   int64_t* tails, int64_t cap, int nthreads) {
   // graph.hpp:919-948: every pair i<j with sqrt(dx*dx+dy*dy) <= rn_ becomes
   // the two directed entries (i,j) and (j,i); 1289-1341: CSR sorted by (src,dst).
+  // strict != 0 tests < instead of <= (no counterpart in the reference).
   std::vector<double> X(n), Y(n);
   or_rgg_lcg_coords(n, seed, X.data(), Y.data());
   const double rn = radius > 0 ? radius : or_rgg_radius_ref(n);
@@ -74,7 +80,7 @@ extern "C" int64_t or_rgg_lcg(int64_t n, uint32_t seed, double radius, int64_t* 
       double dx = X[i] - X[j];
       double dy = Y[i] - Y[j];
       double ed = std::sqrt(dx * dx + dy * dy);
-      if (ed <= rn) up[i].push_back(j);
+      if (strict ? ed < rn : ed <= rn) up[i].push_back(j);
     }
   }
   std::vector<int64_t> deg(n, 0);

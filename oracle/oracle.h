@@ -51,6 +51,9 @@ void or_rgg_lcg_coords(int64_t n, uint32_t seed, double* X, double* Y);
  * entries ne; fills off[n+1] always and tails[0..ne) when ne <= cap. */
 int64_t or_rgg_lcg(int64_t n, uint32_t seed, double radius, int64_t* off,
                    int64_t* tails, int64_t cap, int nthreads);
+/* or_rgg_lcg with edge iff sqrt(dx*dx+dy*dy) < radius when strict != 0. */
+int64_t or_rgg_lcg_opts(int64_t n, uint32_t seed, double radius, int strict, int64_t* off,
+                        int64_t* tails, int64_t cap, int nthreads);
 
 /* LCG coordinates of a P-rank run (utils.hpp:245-430, graph.hpp:836-872): the
  * parallel prefix in wrapping int64 arithmetic, rank r's points at

@@ -35,6 +35,7 @@ def lib():
             "or_rgg_radius_ref": (_dbl, [_i64]),
             "or_rgg_lcg_coords": (None, [_i64, _u32, _p, _p]),
             "or_rgg_lcg": (_i64, [_i64, _u32, _dbl, _p, _p, _i64, _int]),
+            "or_rgg_lcg_opts": (_i64, [_i64, _u32, _dbl, _int, _p, _p, _i64, _int]),
             "or_rgg_lcg_coords_ranks": (None, [_i64, _int, _u32, _p, _p]),
             "or_rgg_lcg_ranks": (_i64, [_i64, _int, _u32, _dbl, _p, _p, _i64, _int]),
             "or_count_reference": (_u64, [_i64, _p, _p, _p, _int, _i64, _i64]),
@@ -84,13 +85,14 @@ def rgg_lcg_coords(n, seed=1):
     return X, Y
 
 
-def rgg_lcg(n, seed=1, radius=0.0, nthreads=None):
-    """(off int64[n+1], tails int64[ne]) of the O(n^2) reference RGG."""
+def rgg_lcg(n, seed=1, radius=0.0, nthreads=None, strict=False):
+    """(off int64[n+1], tails int64[ne]) of the O(n^2) reference RGG; strict
+    tests distance < radius instead of the reference's <=."""
     nt = nthreads or nthreads_default()
     off = np.empty(n + 1, np.int64)
-    ne = lib().or_rgg_lcg(n, seed, radius, _ptr(off), None, 0, nt)
+    ne = lib().or_rgg_lcg_opts(n, seed, radius, int(strict), _ptr(off), None, 0, nt)
     tails = np.empty(ne, np.int64)
-    lib().or_rgg_lcg(n, seed, radius, _ptr(off), _ptr(tails), ne, nt)
+    lib().or_rgg_lcg_opts(n, seed, radius, int(strict), _ptr(off), _ptr(tails), ne, nt)
     return off, tails
 
 

@@ -167,3 +167,25 @@ def test_edgelist_converter(tmp_path):
     off, tails = xxtric_amd.edgelist_to_csr(str(p))
     assert list(off) == [0, 0, 2, 4, 7, 8]
     assert list(tails) == [2, 3, 1, 3, 1, 2, 4, 3]
+
+
+@pytest.mark.parametrize("strict,connect", [(True, True), (True, False), (False, True)])
+def test_rgg_strict_and_connect_options_match_oracle(strict, connect):
+    """xt_rgg_generate_opts (cell grid) equals the oracle's all-pairs loop with the
+    same test (< or <=) and radius (sqrt(2 ln n / (pi n)) or graph.hpp:789-791)."""
+    n = 6000
+    r = xxtric_amd.rgg_radius_connect(n) if connect else 0.0
+    o1, t1 = xxtric_amd.rgg_generate(n, seed=9, strict=strict, connect=connect)
+    o2, t2 = O.rgg_lcg(n, seed=9, radius=r, strict=strict)
+    assert (o1 == o2).all() and (t1 == t2).all()
+
+
+def test_rgg_strict_drops_a_pair_at_exactly_the_radius():
+    n = 3000
+    X, Y = O.rgg_lcg_coords(n, 4)
+    dx, dy = X[7] - X, Y[7] - Y
+    d = np.sqrt(dx * dx + dy * dy)
+    j = int(np.argsort(d)[5])
+    for strict in (False, True):
+        o, t = xxtric_amd.rgg_generate(n, seed=4, radius=float(d[j]), strict=strict)
+        assert (j in set(t[o[7]:o[8]].tolist())) == (not strict)

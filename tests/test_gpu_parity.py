@@ -514,3 +514,26 @@ def test_wide_layout_gives_the_same_counts(ctx, small_cut):
     finally:
         ctx.force_wide(False)
         ctx.tune(small_cut=xxtric_amd.DEFAULT_LIGHT_CUT, group_blocks=4, group_round=2)
+
+
+def test_rgg_connectivity_radius_strict_1e5(ctx):
+    """The workload's radius variant: 100,000 points of the reference's LCG stream,
+    edge iff distance < sqrt(2 ln n / (pi n)).  The cell-grid CSR equals the
+    oracle's all-pairs loop with the strict test; global and per-vertex counts
+    equal the oracle's in both orientations; `tric -n 100000 -l -C` prints the
+    same edge and triangle counts."""
+    import os
+    import subprocess
+    n = 100000
+    r = xxtric_amd.rgg_radius_connect(n)
+    off, tails = xxtric_amd.rgg_generate(n, seed=1, strict=True, connect=True)
+    oo, ot = O.rgg_lcg(n, seed=1, radius=r, strict=True)
+    assert (off == oo).all() and (tails == ot).all()
+    exp = _check_csr(ctx, off, tails, "rgg1e5 connect strict")
+    assert exp > 0
+    tric = os.path.join(os.path.dirname(xxtric_amd.__file__), "bin", "tric")
+    _say("tric -n 100000 -l -C")
+    out = subprocess.run([tric, "-n", str(n), "-l", "-C"], capture_output=True, text=True,
+                         check=True, timeout=120).stdout
+    assert "Number of edges: %d" % len(tails) in out
+    assert "Number of triangles: %d" % exp in out

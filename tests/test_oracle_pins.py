@@ -120,3 +120,37 @@ def test_degree_ordered_missing_apex_credit_is_caught():
             Nv = {x for x in tails[off[v]:off[v + 1]] if key(x) > key(v)}
             wrong[min(u, v)] += len(Nu & Nv)
     assert int(wrong.sum()) == bt and not (wrong == bpv).all()
+
+
+def _rgg_brute(X, Y, r, strict):
+    """All pairs in numpy: distance sqrt(dx*dx + dy*dy), rounded like the C code (no FMA)."""
+    n = len(X)
+    i, j = np.triu_indices(n, 1)
+    dx = X[i] - X[j]
+    dy = Y[i] - Y[j]
+    d = np.sqrt(dx * dx + dy * dy)
+    keep = d < r if strict else d <= r
+    return csr_from_edges(n, np.stack([i[keep], j[keep]], 1))
+
+
+@pytest.mark.parametrize("strict", [False, True])
+def test_rgg_strict_option_pinned_by_a_tie(strict):
+    """or_rgg_lcg_opts: with the radius set to the exact distance of one pair, the
+    strict test drops that pair and the reference's <= keeps it; both equal an
+    all-pairs numpy enumeration of the same points."""
+    n = 1500
+    X, Y = O.rgg_lcg_coords(n, 5)
+    dx, dy = X[10] - X, Y[10] - Y
+    d = np.sqrt(dx * dx + dy * dy)
+    j = int(np.argsort(d)[6])          # the 6th-nearest point to point 10: about 6 neighbours each
+    r = float(d[j])
+    o, t = O.rgg_lcg(n, seed=5, radius=r, strict=strict)
+    bo, bt = _rgg_brute(X, Y, r, strict)
+    assert (o == bo).all() and (t == bt).all()
+    assert (j in set(t[o[10]:o[11]].tolist())) == (not strict)
+
+
+def test_rgg_connect_radius_formula():
+    import xxtric_amd
+    for n in (1000, 100000):
+        assert xxtric_amd.rgg_radius_connect(n) == sqrt(2 * log(n) / (np.pi * n))

@@ -78,7 +78,7 @@ def test_noalloc_rejects_what_does_not_fit(ctx):
     """A graph whose scratch exceeds the static array fails with XT_ERR_CAPACITY
     before any kernel runs (the counter stays untouched); bad input is still
     rejected; the next call that fits works."""
-    nv = 1 << 21                                   # 72 MiB of per-vertex scratch alone (36 B each)
+    nv = 1 << 21                                   # 96 MiB of per-vertex scratch alone (48 B each)
     d_ind = ctx.to_device(np.zeros(nv + 1, np.int64))
     d_el = ctx.to_device(np.zeros(1, np.int64))
     d_cnt = ctx.to_device(np.array([7], np.int64))

@@ -17,9 +17,9 @@ from .capi import (XT_BATCH_HIPGRAPH, XT_BATCH_PER_GRAPH, XT_BATCH_PERSISTENT,  
                    XT_CHECK_SYMMETRY, XT_ORIENT_DEGREE, XT_ORIENT_ID, XT_PER_VERTEX,
                    XT_VALIDATE, XtError)
 
-DEFAULT_LIGHT_CUT = -1  # tune() value that restores the defaults: light = both out-lists <= 32 (64 in batches)
+DEFAULT_LIGHT_CUT = -1  # tune() value that restores the defaults: light = both out-lists <= 32 (8 in batches)
 
-__all__ = ["Context", "DeviceGraph", "TriangleCountingSingle", "rgg_generate", "rgg_radius",
+__all__ = ["Context", "DeviceGraph", "TriangleCountingSingle", "rgg_generate", "rgg_radius", "rgg_radius_connect",
            "vite_read", "vite_write", "edgelist_to_csr", "batch_params", "batch_graph_csr",
            "Batch", "Server", "XtError"]
 
@@ -85,14 +85,17 @@ class Context:
         except Exception:
             pass
 
-    def tune(self, small_cut=0, group_blocks=0, group_round=0, items_per_tile=0, tile_run=0):
+    def tune(self, small_cut=0, group_blocks=0, group_round=0, items_per_tile=0, tile_run=0, wave_owner=0):
         """Set the light-edge cut, the group kernel's workgroups per CU, its chunks in
-        flight per thread, the items per group tile and/or the tiles a workgroup draws
-        at once (0 = keep; -1 restores the default of small_cut, items_per_tile,
-        tile_run).  No setting changes a result."""
+        flight per thread, the items per group tile, the tiles a workgroup draws at
+        once and/or the longest owner list counted in wave tiles (0 = keep; -1
+        restores the default of small_cut, items_per_tile, tile_run, wave_owner;
+        wave_owner -2 turns wave tiles off).  No setting changes a result."""
         self._check(self._L.xt_ctx_tune(self.h, small_cut, group_blocks, group_round))
         if items_per_tile or tile_run:
             self._check(self._L.xt_ctx_tune_tiles(self.h, items_per_tile, tile_run))
+        if wave_owner:
+            self._check(self._L.xt_ctx_tune_waves(self.h, wave_owner))
 
     def fail_allocs(self, n):
         """Failure injection: the next n scratch allocations fail with XT_ERR_NOMEM."""
@@ -341,20 +344,28 @@ def rgg_radius(n):
     return capi.lib().xt_rgg_radius(n)
 
 
-def rgg_generate(n, seed=1, radius=0.0, nprocs=1):
+def rgg_radius_connect(n):
+    """sqrt(2 ln n / (pi n)): the connectivity radius."""
+    return capi.lib().xt_rgg_radius_connect(n)
+
+
+def rgg_generate(n, seed=1, radius=0.0, nprocs=1, strict=False, connect=False):
     """Reference RGG (graph.hpp:745-1374, LCG mode): (off int64[n+1], tails int64).
-    nprocs > 1: the graph a run on that many MPI ranks generates, all rows in one CSR."""
+    nprocs > 1: the graph a run on that many MPI ranks generates, all rows in one CSR.
+    strict: edge iff distance < r (the reference tests <=); connect: r = sqrt(2 ln n /
+    (pi n)) when radius <= 0 (the reference's radius is graph.hpp:789-791)."""
     L = capi.lib()
+    flags = (capi.XT_RGG_STRICT if strict else 0) | (capi.XT_RGG_CONNECT if connect else 0)
     off = np.empty(n + 1, np.int64)
     ne = ctypes.c_int64()
-    rc = L.xt_rgg_generate_ranks(n, nprocs, seed, radius, capi.ptr(off), None, 0, ctypes.byref(ne))
+    rc = L.xt_rgg_generate_opts(n, nprocs, seed, radius, flags, capi.ptr(off), None, 0, ctypes.byref(ne))
     if rc != capi.XT_OK:
-        raise XtError(rc, "xt_rgg_generate_ranks")
+        raise XtError(rc, "xt_rgg_generate_opts")
     tails = np.empty(ne.value, np.int64)
-    rc = L.xt_rgg_generate_ranks(n, nprocs, seed, radius, capi.ptr(off), capi.ptr(tails), ne.value,
-                                 ctypes.byref(ne))
+    rc = L.xt_rgg_generate_opts(n, nprocs, seed, radius, flags, capi.ptr(off), capi.ptr(tails), ne.value,
+                                ctypes.byref(ne))
     if rc != capi.XT_OK:
-        raise XtError(rc, "xt_rgg_generate_ranks")
+        raise XtError(rc, "xt_rgg_generate_opts")
     return off, tails
 
 

@@ -1,9 +1,12 @@
 """Autotuner for the counting pipeline's launch knobs.
 
-Five knobs decide how the work is split; none changes a result:
+Six knobs decide how the work is split; none changes a result:
   * light_cut      -- edges whose out-lists are both at most this long are
                       intersected in the light pass; longer ones go to the
-                      group kernel (LDS-staged list, streamed partner lists);
+                      group kernel (LDS-staged list, streamed partner lists)
+                      or to wave tiles;
+  * wave_owner     -- owners whose out-list is at most this long are counted
+                      one wave per tile with a small per-wave table (-2: none);
   * group_round    -- 16-byte list chunks each group-kernel thread keeps in
                       flight per round (1, 2 or 4: more loads in flight against
                       more registers, hence fewer resident workgroups);
@@ -14,7 +17,7 @@ Five knobs decide how the work is split; none changes a result:
                       keep its staged table more often).
 The reference fixes its launch shapes with compile-time macros
 (counting.cuh:150-176); here they are context knobs (xt_ctx_tune,
-xt_ctx_tune_tiles) chosen by measurement.
+xt_ctx_tune_tiles, xt_ctx_tune_waves) chosen by measurement.
 
 The search is coordinate descent from the library defaults: each knob in turn
 is swept with the others held at the best setting so far, for at most
@@ -36,14 +39,16 @@ import numpy as np
 
 DEFAULT_CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
 SPACE = {
-    "light_cut": (16, 24, 32, 48, 64, 96),
+    "light_cut": (4, 8, 16, 24, 32, 48, 64, 96),
+    "wave_owner": (-2, 32, 64, 128),
     "group_round": (1, 2, 4),
     "group_blocks": (2, 3, 4),
     "items_per_tile": (64, 128, 256),
     "tile_run": (4, 8, 16, 32, 64),
 }
-DEFAULTS = {"light_cut": 32, "group_round": 2, "group_blocks": 4, "items_per_tile": 256, "tile_run": 16}
-BATCH_DEFAULTS = dict(DEFAULTS, light_cut=64)
+DEFAULTS = {"light_cut": 32, "wave_owner": 128, "group_round": 2, "group_blocks": 4, "items_per_tile": 256,
+            "tile_run": 16}
+BATCH_DEFAULTS = dict(DEFAULTS, light_cut=8, wave_owner=64)
 
 
 def _time_graph(ctx, g, reps):
@@ -78,7 +83,7 @@ def _time_batch(b, reps):
 def apply(ctx, knobs):
     ctx.tune(small_cut=knobs["light_cut"], group_blocks=knobs["group_blocks"],
              group_round=knobs["group_round"], items_per_tile=knobs["items_per_tile"],
-             tile_run=knobs["tile_run"])
+             tile_run=knobs["tile_run"], wave_owner=knobs.get("wave_owner", -1))
 
 
 def search(ctx, runner, start, space=SPACE, passes=2, log=print, label=""):

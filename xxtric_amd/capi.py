@@ -42,6 +42,9 @@ XT_D2H = 2
 XT_D2D = 3
 XT_D2D_KERNEL = 4
 
+XT_RGG_STRICT = 0x1
+XT_RGG_CONNECT = 0x2
+
 _i64 = ctypes.c_int64
 _u64 = ctypes.c_uint64
 _u32 = ctypes.c_uint32
@@ -86,6 +89,7 @@ _SIGS = {
     "xt_last_timing": (_int, [_p, ctypes.POINTER(XtTiming)]),
     "xt_ctx_tune": (_int, [_p, _int, _int, _int]),
     "xt_ctx_tune_tiles": (_int, [_p, _int, _int]),
+    "xt_ctx_tune_waves": (_int, [_p, _int]),
     "xt_ctx_force_wide": (_int, [_p, _int]),
     "xt_ctx_fail_allocs": (_int, [_p, _int]),
     "xt_dev_alloc": (_int, [_p, ctypes.c_size_t, _pp]),
@@ -115,6 +119,8 @@ _SIGS = {
     "xt_rgg_generate": (_int, [_i64, _u32, _dbl, _p, _p, _i64, _p]),
     "xt_rgg_generate_ranks": (_int, [_i64, _int, _u32, _dbl, _p, _p, _i64, _p]),
     "xt_rgg_radius": (_dbl, [_i64]),
+    "xt_rgg_generate_opts": (_int, [_i64, _int, _u32, _dbl, _u32, _p, _p, _i64, _p]),
+    "xt_rgg_radius_connect": (_dbl, [_i64]),
     "xt_vite_read": (_int, [ctypes.c_char_p, _int, _p, _p, _p, _p]),
     "xt_vite_write": (_int, [ctypes.c_char_p, _int, _i64, _p, _p]),
     "xt_edgelist_to_csr": (_int, [ctypes.c_char_p, _p, _p, _p, _p]),

@@ -263,11 +263,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8))) 
   __shared__ uint16_t s_rank[kChunk + 1];    // owned entries before each entry of the chunk
   __shared__ uint32_t s_bb[2];               // reserved starts of the first and last row's items
   __shared__ unsigned long long s_g[BATCH ? kGraphSlots : 1];
-  const int64_t eb = (int64_t)blockIdx.x * kChunk;
+  const int64_t cb = xcd_block((ne + kChunk - 1) / kChunk);
+  const int64_t eb = cb * kChunk;
   if (eb >= ne) return;  // whole workgroup leaves together
   const int64_t ee = min(ne, eb + (int64_t)kChunk);
   if (BATCH && threadIdx.x < kGraphSlots) s_g[threadIdx.x] = 0;
-  const TileRows r = tile_rows_at(off, crows, nv, (int64_t)blockIdx.x, s_off);
+  const TileRows r = tile_rows_at(off, crows, nv, cb, s_off);
   const int32_t g0 = BATCH ? gid[r.ulo] : 0;
 
   uint64_t hits_all = 0;
@@ -576,26 +577,48 @@ __global__ __launch_bounds__(256) void k_vertex_ranges(int64_t nv, const int64_t
   }
 }
 
-// Tiles of each owner: its items cut into tiles of ipt (<= kItemsPerTile).
+// Wave tiles: an owner whose out-list has at most wave_owner ids (a cut of at
+// most kWaveOwnerMax) is counted by k_count_waves, one wave per tile of at
+// most kWaveItems items, instead of by the group kernel.
+constexpr int kWaveItems = 64;
+
+// Tiles of each owner: its items cut into tiles of ipt (<= kItemsPerTile) for
+// the group kernel, counted at tcnt[x]; with wave tiles (wave_owner > 0) the
+// owners with d+(x) <= wave_owner have tiles of kWaveItems counted at
+// tcnt[nv + x] instead, so one scan over 2 nv counts puts every group tile
+// before every wave tile.
 __global__ __launch_bounds__(256) void k_tile_counts(int64_t nv, const uint32_t* __restrict__ itcnt,
-                                                     uint32_t* __restrict__ tcnt, int ipt) {
+                                                     const int64_t* __restrict__ ooff,
+                                                     uint32_t* __restrict__ tcnt, int ipt, int wave_owner) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < nv; x += stride)
-    tcnt[x] = (itcnt[x] + (uint32_t)ipt - 1) / (uint32_t)ipt;
+  for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < nv; x += stride) {
+    const uint32_t it = itcnt[x];
+    if (wave_owner > 0) {
+      const bool wv = ooff[x + 1] - ooff[x] <= (int64_t)wave_owner;
+      tcnt[x] = wv ? 0u : (it + (uint32_t)ipt - 1) / (uint32_t)ipt;
+      tcnt[nv + x] = wv ? (it + (uint32_t)kWaveItems - 1) / (uint32_t)kWaveItems : 0u;
+    } else {
+      tcnt[x] = (it + (uint32_t)ipt - 1) / (uint32_t)ipt;
+    }
+  }
 }
 
-// One descriptor per tile; x's items start at off[x].
+// One descriptor per tile; x's items start at off[x].  Group tiles are
+// [tileoff[x], tileoff[x+1]), wave tiles [tileoff[nv+x], tileoff[nv+x+1]).
 __global__ __launch_bounds__(256) void k_tile_desc(int64_t nv, const int64_t* __restrict__ off,
                                                    const int64_t* __restrict__ ooff,
                                                    const uint32_t* __restrict__ itcnt,
                                                    const int64_t* __restrict__ tileoff,
-                                                   TileDesc* __restrict__ desc, int ipt) {
+                                                   TileDesc* __restrict__ desc, int ipt_group, int wave_owner) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < nv; x += stride) {
-    const int64_t t0 = tileoff[x], t1 = tileoff[x + 1];
-    if (t0 == t1) continue;
-    const int64_t ib = off[x], cnt = itcnt[x], a0 = ooff[x];
+    const int64_t a0 = ooff[x];
     const int32_t dx = (int32_t)(ooff[x + 1] - a0);
+    const bool wv = wave_owner > 0 && dx <= wave_owner;
+    const int64_t t0 = wv ? tileoff[nv + x] : tileoff[x], t1 = wv ? tileoff[nv + x + 1] : tileoff[x + 1];
+    if (t0 == t1) continue;
+    const int ipt = wv ? kWaveItems : ipt_group;
+    const int64_t ib = off[x], cnt = itcnt[x];
     for (int64_t t = t0; t < t1; t++) {
       const int64_t k = (t - t0) * ipt;
       const int64_t n = cnt - k < ipt ? cnt - k : ipt;
@@ -1112,6 +1135,298 @@ __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(W
   flush_slots();  // the last table's counters (the loop ends on a barrier)
 }
 
+// ---------------------------------------------------------------------------
+// Wave tiles (plain counts).  An owner x with a short out-list gets one wave,
+// not a 512-thread workgroup with a 2048-bucket table: the wave hashes N+(x)
+// (at most kWaveOwnerMax ids) into its own 64-bucket LDS table (1 KB) and
+// streams its tile's items -- at most 64, one per lane -- through it.  The
+// items' lists are concatenated into 16-byte chunks and dealt out 64 at a
+// time, one per lane, so lanes never wait on a longer list than their own
+// chunk.  Small sparse graphs (batches) have almost only such owners, and a
+// workgroup-wide tile per owner cost more in set-up than its lists.
+// Chunk c of a round belongs to the item k with pre[k] <= c < pre[k+1]
+// (pre: exclusive scan of the items' chunk counts, lane k holding pre[k]):
+// the items starting at or before the round's first chunk are counted with
+// one ballot, those starting inside the round are bits of a 64-bit mask
+// OR-ed across the wave, and each lane adds the bits up to its own position.
+constexpr int kWaveBuckets = 64;  // per-wave LDS hash: 64 buckets of four ids
+constexpr int kWaveOwnerMax = 2 * kWaveBuckets;  // owner lists fill at most half the slots
+constexpr int kWaveThreads = 256;  // four independent waves per workgroup
+
+// Home bucket of w: a multiplicative hash, so ids sharing their low bits
+// (strided or clustered labels) still spread.
+__device__ __forceinline__ uint32_t wave_bucket(int32_t w) {
+  return ((uint32_t)w * 0x9E3779B1u) >> (32 - 6);
+}
+static_assert(kWaveBuckets == 64, "wave_bucket yields 6 bits");
+
+// Inserts key; returns whether a full bucket had to be passed (and was
+// marked spilled, as in tab_insert).
+__device__ __forceinline__ bool wave_tab_insert(int4* tab, int32_t key) {
+  uint32_t b = wave_bucket(key);
+  bool passed = false;
+  for (;;) {
+    int* s = reinterpret_cast<int*>(tab + b);
+    if (atomicCAS(s, kEmpty, key) == kEmpty) return passed;
+    if (atomicCAS(s + 1, kEmpty, key) == kEmpty) return passed;
+    if (atomicCAS(s + 2, kEmpty, key) == kEmpty) return passed;
+    if (atomicCAS(s + 3, kEmpty, key) == kEmpty) return passed;
+    atomicOr(s + 3, (int)0x80000000);
+    passed = true;
+    b = (b + 1) & (kWaveBuckets - 1);
+  }
+}
+
+// Continues a lookup past the spilled home bucket b; ends because at most half
+// the slots are used, so some bucket is not full and therefore not spilled.
+__device__ __forceinline__ bool wave_tab_continue(const int4* tab, uint32_t b, int32_t w) {
+  for (;;) {
+    b = (b + 1) & (kWaveBuckets - 1);
+    const int4 t = tab[b];
+    if (bucket_miss(t, w) == 0) return true;
+    if (!bucket_spilled(t)) return false;
+  }
+}
+
+// Orders this wave's LDS accesses before and after it (one wave owns the
+// table; LDS instructions of a wave execute in order).
+__device__ __forceinline__ void wave_lds_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Wave tiles are tdesc[*t_begin .. *t_end).  Each wave draws runs of
+// tile_run consecutive tiles from a 64-bit queue and keeps its table while
+// the owner stays the same.  Loads are issued one round ahead: while round r
+// of a tile is probed, the chunks of the next round -- of the same tile, or
+// the first round of the next tile together with that tile's owner list --
+// are in flight, and the items of the tile after and the descriptor after
+// that are loaded earlier still.  Hits are counted per wave (ballot
+// population counts) and added once per wave, or, in a batch, once per run of
+// tiles of one graph.
+struct WaveTile {
+  int64_t i0, a0;  // first item; ooff[x]
+  int32_t x, n, dx;
+  bool valid;
+};
+
+// One lane's chunk of a round: four ids and how many of them lie inside the list.
+struct WaveChunk {
+  int4 v;
+  int32_t r;
+};
+
+template <bool BATCH>
+__global__ __launch_bounds__(kWaveThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_count_waves(
+    const int32_t* __restrict__ ot, const Item* __restrict__ items, const TileDesc* __restrict__ tdesc,
+    const int64_t* __restrict__ t_begin, const int64_t* __restrict__ t_end,
+    uint64_t* __restrict__ counters, const int32_t* __restrict__ gid,
+    unsigned long long* __restrict__ gtotal, uint32_t tile_run) {
+  __shared__ int4 s_tab[kWaveThreads / kWave][kWaveBuckets];
+  const int lane = threadIdx.x & (kWave - 1);
+  int4* tab = s_tab[threadIdx.x / kWave];
+  const int64_t tb = *t_begin;
+  const uint64_t nt = (uint64_t)(*t_end - tb);
+  // One queue per XCD over an eighth of the tiles (consecutive tiles: the same
+  // owners, and in a batch the same graphs, so they stay in that XCD's L2);
+  // a wave whose XCD's share is drawn goes on to the next XCD's queue.
+  unsigned long long* queues = reinterpret_cast<unsigned long long*>(&counters[kWaveQueue]);
+  int q = (int)(blockIdx.x % kXcds), qleft = kXcds;
+  uint64_t run_next = 0, run_end = 0;
+  auto draw = [&]() -> uint64_t {
+    while (run_next >= run_end) {
+      if (qleft == 0) return nt;  // every queue drawn: no tile
+      unsigned long long v = 0;
+      if (lane == 0) v = atomicAdd(&queues[q], (unsigned long long)tile_run);
+      const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+      const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+      const uint64_t got = ((uint64_t)hi << 32) | lo;
+      const uint64_t qlo = nt * (uint64_t)q / kXcds, qhi = nt * (uint64_t)(q + 1) / kXcds;
+      if (qlo + got < qhi) {
+        run_next = qlo + got;
+        run_end = min(run_next + tile_run, qhi);
+      } else {
+        q = (q + 1) % kXcds;
+        qleft--;
+      }
+    }
+    return run_next++;
+  };
+  const WaveTile none{0, 0, 0, 0, 0, false};
+  const int4* __restrict__ td4 = reinterpret_cast<const int4*>(tdesc);
+  auto load_desc = [&](uint64_t t) -> WaveTile {
+    WaveTile d = none;
+    if (t >= nt) return d;
+    const int4 u0 = td4[2 * (tb + (int64_t)t)], u1 = td4[2 * (tb + (int64_t)t) + 1];
+    auto uni64 = [](int32_t lo32, int32_t hi32) -> int64_t {
+      return (int64_t)(((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)hi32) << 32) |
+                       __builtin_amdgcn_readfirstlane((uint32_t)lo32));
+    };
+    d.i0 = uni64(u0.x, u0.y);
+    d.a0 = uni64(u0.z, u0.w);
+    d.x = __builtin_amdgcn_readfirstlane(u1.x);
+    d.n = __builtin_amdgcn_readfirstlane(u1.y);
+    d.dx = __builtin_amdgcn_readfirstlane(u1.z);
+    d.valid = true;
+    return d;
+  };
+  // Lane k holds item k of a tile: its list's offset and length.
+  auto load_items = [&](const WaveTile& d, int64_t& ib, int32_t& il) {
+    ib = 0;
+    il = 0;
+    if (d.valid && lane < d.n) {
+      const Item it = items[d.i0 + lane];
+      ib = it.base;
+      il = it.len;
+    }
+  };
+  // Where each item's chunks start in the tile's run of chunks (lane k: item k).
+  auto chunk_starts = [&](int32_t il, int32_t& pre, int32_t& total) {
+    const int32_t nc = (il + 3) >> 2;
+    int32_t inc = nc;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+      const int32_t y = __shfl_up(inc, o, kWave);
+      if (lane >= o) inc += y;
+    }
+    pre = inc - nc;
+    total = __builtin_amdgcn_readlane(inc, kWave - 1);
+  };
+  // Chunk c0 + lane of a tile: its item kk has pre[kk] <= c < pre[kk+1].
+  auto load_round = [&](int64_t ib, int32_t il, int32_t pre, int n, int32_t total, int32_t c0) -> WaveChunk {
+    const int32_t c = c0 + lane;
+    const int k0 = __popcll(__builtin_amdgcn_ballot_w64(lane < n && pre <= c0)) - 1;
+    uint64_t sb = (lane < n && pre > c0 && pre < c0 + kWave) ? (1ull << (pre - c0)) : 0ull;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) sb |= __shfl_xor(sb, o, kWave);
+    const int kk = k0 + __popcll(sb & ((2ull << lane) - 1ull));
+    const int64_t kb = __shfl(ib, kk, kWave);
+    const int32_t kp = __shfl(pre, kk, kWave), kl = __shfl(il, kk, kWave);
+    const int32_t q = c - kp;  // chunk of item kk
+    WaveChunk ch{make_int4(-1, -1, -1, -1), kl - 4 * q};  // ids inside the list: <= 0 past the run's end
+    if (c < total) {
+      const v4i_a4 tq = *reinterpret_cast<const gv4i_a4*>(reinterpret_cast<uint64_t>(ot + kb + 4 * (int64_t)q));
+      ch.v = make_int4(tq.x, tq.y, tq.z, tq.w);
+    }
+    return ch;
+  };
+  // An owner list of at most kWaveOwnerMax ids: two ids per lane.
+  auto load_owner = [&](const WaveTile& d, int32_t& o0, int32_t& o1) {
+    o0 = lane < d.dx ? ot[d.a0 + lane] : 0;
+    o1 = lane + kWave < d.dx ? ot[d.a0 + kWave + lane] : 0;
+  };
+  bool spl = false;  // some bucket of the table spilled (uniform)
+  auto build_table = [&](const WaveTile& d, int32_t o0, int32_t o1) {
+    tab[lane] = make_int4(kEmpty, kEmpty, kEmpty, kEmpty);
+    wave_lds_fence();
+    bool passed = false;
+    if (lane < d.dx) passed |= wave_tab_insert(tab, o0);
+    if (lane + kWave < d.dx) passed |= wave_tab_insert(tab, o1);
+    spl = __builtin_amdgcn_ballot_w64(passed) != 0;
+    wave_lds_fence();
+  };
+  uint64_t hits = 0;   // uniform: sums of ballot population counts
+  auto probe = [&](const WaveChunk& ch) {
+    const int32_t w[4] = {ch.v.x, ch.v.y, ch.v.z, ch.v.w};
+    int4 tt[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) tt[j] = tab[wave_bucket(w[j])];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const uint64_t hit = (__builtin_amdgcn_ballot_w64(tt[j].x == w[j]) | __builtin_amdgcn_ballot_w64(tt[j].y == w[j]) |
+                            __builtin_amdgcn_ballot_w64(tt[j].z == w[j]) | __builtin_amdgcn_ballot_w64(tt[j].w == w[j])) &
+                           __builtin_amdgcn_ballot_w64(j < ch.r);
+      hits += (uint64_t)__popcll(hit);
+    }
+    // A key in slot 3 of a spilled bucket carries the mark (the raw compare
+    // missed it), and keys that passed a full bucket sit further on.
+    if (spl && __builtin_amdgcn_ballot_w64((tt[0].w | tt[1].w | tt[2].w | tt[3].w) < 0)) {
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        bool h = false;
+        if (j < ch.r && tt[j].w < 0) {
+          h = (tt[j].w & 0x7fffffff) == w[j];
+          if (!h && bucket_miss(tt[j], w[j]) != 0) h = wave_tab_continue(tab, wave_bucket(w[j]), w[j]);
+        }
+        hits += (uint64_t)__popcll(__builtin_amdgcn_ballot_w64(h));
+      }
+    }
+  };
+  int32_t g_cur = -1;  // BATCH: graph of `hits`
+  auto enter_graph = [&](const WaveTile& d) {
+    if constexpr (BATCH) {
+      const int32_t g = gid[d.x];
+      if (g != g_cur) {
+        if (lane == 0 && hits) atomicAdd(&gtotal[g_cur], (unsigned long long)hits);
+        hits = 0;
+        g_cur = g;
+      }
+    }
+  };
+
+  // Prologue: the first tile's owner list, items and first round; the next
+  // tile's items; the descriptor after it.
+  WaveTile dc = load_desc(draw());
+  if (dc.valid) {
+    int64_t ib_c, ib_n;
+    int32_t il_c, il_n, o0, o1;
+    load_items(dc, ib_c, il_c);
+    load_owner(dc, o0, o1);
+    WaveTile dn = load_desc(draw());
+    load_items(dn, ib_n, il_n);
+    WaveTile dnn = dn.valid ? load_desc(draw()) : none;
+    enter_graph(dc);
+    build_table(dc, o0, o1);
+    int64_t table_a0 = dc.a0;
+    int32_t pre_c, total_c;
+    chunk_starts(il_c, pre_c, total_c);
+    WaveChunk cur = load_round(ib_c, il_c, pre_c, dc.n, total_c, 0);
+    int32_t c0 = 0;
+    for (;;) {
+      const bool same = c0 + kWave < total_c;  // uniform: the tile has another round
+      WaveChunk nxt{make_int4(-1, -1, -1, -1), 0};
+      int32_t pre_n = 0, total_n = 0;
+      const bool fresh = !same && dn.valid && dn.a0 != table_a0;  // the next tile needs a new table
+      if (same) {
+        nxt = load_round(ib_c, il_c, pre_c, dc.n, total_c, c0 + kWave);
+      } else if (dn.valid) {
+        chunk_starts(il_n, pre_n, total_n);
+        nxt = load_round(ib_n, il_n, pre_n, dn.n, total_n, 0);
+        if (fresh) load_owner(dn, o0, o1);
+      }
+      probe(cur);
+      cur = nxt;
+      if (same) {
+        c0 += kWave;
+        continue;
+      }
+      if (!dn.valid) break;  // uniform
+      // Tile boundary: dn becomes the current tile.
+      enter_graph(dn);
+      if (fresh) {
+        build_table(dn, o0, o1);
+        table_a0 = dn.a0;
+      }
+      dc = dn;
+      ib_c = ib_n;
+      il_c = il_n;
+      pre_c = pre_n;
+      total_c = total_n;
+      c0 = 0;
+      dn = dnn;
+      load_items(dn, ib_n, il_n);
+      dnn = dn.valid ? load_desc(draw()) : none;
+    }
+  }
+  if (lane == 0 && hits) {
+    if constexpr (BATCH)
+      atomicAdd(&gtotal[g_cur], (unsigned long long)hits);
+    else
+      atomicAdd((unsigned long long*)&counters[kTotal], (unsigned long long)hits);
+  }
+}
+
 // Sums over oriented edges of min(d+(u), d+(v)) (all edges, and the heavy ones)
 // and the number of heavy edges -- the algorithmic byte figures.
 __global__ __launch_bounds__(kBlock) void k_alg_stats(int64_t nv, const int64_t* __restrict__ ooff,
@@ -1157,6 +1472,12 @@ static int64_t tiles_bound(int64_t nv, int64_t ne) {
   return b < ne + 1 ? b : ne + 1;
 }
 
+// Largest owner out-list counted in wave tiles, 0 for none.
+static int wave_owner_cut(const xt_ctx* ctx, bool batch) {
+  const int w = ctx->wave_owner == -1 ? (batch ? kBatchWaveOwner : kWaveOwner) : ctx->wave_owner;
+  return w < 0 ? 0 : (w > kWaveOwnerMax ? kWaveOwnerMax : w);
+}
+
 template <typename T>
 int count_oriented(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne_bound, const int64_t* in_off,
                    const T* in_tails, uint64_t* pv, const int32_t* gid, uint64_t* gtotal,
@@ -1174,11 +1495,14 @@ int count_oriented(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne_bound, const 
   const int64_t nchunks = (ne_bound + kChunk - 1) / kChunk;
   XT_TRY(ensure(ctx, s.itcnt, (size_t)nv * sizeof(uint32_t)));
   XT_TRY(ensure(ctx, s.items, (size_t)ne_bound * sizeof(Item)));
-  XT_TRY(ensure(ctx, s.tileoff, (size_t)(nv + 1) * sizeof(int64_t)));
   XT_TRY(ensure(ctx, s.tiledesc, (size_t)tiles_bound(nv, ne_bound) * sizeof(TileDesc)));
-  XT_TRY(ensure(ctx, s.deg, (size_t)nv * sizeof(uint32_t)));  // reused as per-owner tile counts
+  // Owners with out-lists of at most wown ids go to wave tiles (plain counts).
+  const int wown = (range || pv) ? 0 : wave_owner_cut(ctx, batch);
+  XT_TRY(ensure(ctx, s.deg, (size_t)(wown ? 2 : 1) * (size_t)nv * sizeof(uint32_t)));
+  XT_TRY(ensure(ctx, s.tileoff, (size_t)((wown ? 2 : 1) * nv + 1) * sizeof(int64_t)));
   uint64_t* counters = s.counters.as<uint64_t>();
   XT_TRY(fill_u64(ctx, counters + kTileQueue, 1, 0, st));
+  if (wown) XT_TRY(fill_u64(ctx, counters + kWaveQueue, kXcds, 0, st));
   XT_TRY(fill_u32(ctx, s.itcnt.as<uint32_t>(), nv, 0u, st));
   const int64_t* ooff = s.ooff.as<int64_t>();
   const int32_t* ot = s.otails.as<int32_t>();
@@ -1209,7 +1533,7 @@ int count_oriented(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne_bound, const 
   }
   XT_HIP(ctx, hipGetLastError());
 #define XT_LIGHT(N, C, B)                                                                           \
-  hipLaunchKernelGGL((k_light_items<T, N, C, B>), dim3((unsigned)nchunks), dim3(kBlock), 0, st, nv,  \
+  hipLaunchKernelGGL((k_light_items<T, N, C, B>), dim3(xcd_grid(nchunks)), dim3(kBlock), 0, st, nv,  \
                      ne_bound, in_off, in_tails, oflags, ooff, (const uint64_t*)vrec,              \
                      (const uint4*)vrange, (const uint16_t*)dp16, ot, (const int64_t*)s.crows.as<int64_t>(), cut, \
                      counters, s.itcnt.as<uint32_t>(), s.items.as<Item>(), pvp, gid, gt, rlo, rhi)
@@ -1235,12 +1559,12 @@ int count_oriented(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne_bound, const 
   if (evs) XT_HIP(ctx, hipEventRecord(evs[0], st));
   const int ipt = ctx->items_per_tile;
   hipLaunchKernelGGL(k_tile_counts, dim3((unsigned)g), dim3(256), 0, st, nv,
-                     (const uint32_t*)s.itcnt.as<uint32_t>(), s.deg.as<uint32_t>(), ipt);
+                     (const uint32_t*)s.itcnt.as<uint32_t>(), ooff, s.deg.as<uint32_t>(), ipt, wown);
   XT_HIP(ctx, hipGetLastError());
-  XT_TRY(scan_u32_to_i64(ctx, s, s.deg.as<uint32_t>(), nv, s.tileoff.as<int64_t>(), st));
+  XT_TRY(scan_u32_to_i64(ctx, s, s.deg.as<uint32_t>(), (wown ? 2 : 1) * nv, s.tileoff.as<int64_t>(), st));
   hipLaunchKernelGGL(k_tile_desc, dim3((unsigned)g), dim3(256), 0, st, nv, in_off, ooff,
                      (const uint32_t*)s.itcnt.as<uint32_t>(), (const int64_t*)s.tileoff.as<int64_t>(),
-                     s.tiledesc.as<TileDesc>(), ipt);
+                     s.tiledesc.as<TileDesc>(), ipt, wown);
   XT_HIP(ctx, hipGetLastError());
   if (evs) XT_HIP(ctx, hipEventRecord(evs[1], st));
   const unsigned groups = (unsigned)(ctx->num_cus * ctx->group_blocks);
@@ -1270,6 +1594,19 @@ int count_oriented(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne_bound, const 
 #undef XT_GROUPS_R
 #undef XT_GROUPS
   XT_HIP(ctx, hipGetLastError());
+  if (wown) {
+    const int64_t* tof = s.tileoff.as<int64_t>();
+    const unsigned wblocks = (unsigned)(ctx->num_cus * kWaveBlocksPerCU);
+    if (batch)
+      hipLaunchKernelGGL(k_count_waves<true>, dim3(wblocks), dim3(kWaveThreads), 0, st, ot,
+                         (const Item*)s.items.as<Item>(), (const TileDesc*)s.tiledesc.as<TileDesc>(), tof + nv,
+                         tof + 2 * nv, counters, gid, gt, (uint32_t)ctx->tile_run);
+    else
+      hipLaunchKernelGGL(k_count_waves<false>, dim3(wblocks), dim3(kWaveThreads), 0, st, ot,
+                         (const Item*)s.items.as<Item>(), (const TileDesc*)s.tiledesc.as<TileDesc>(), tof + nv,
+                         tof + 2 * nv, counters, gid, gt, (uint32_t)ctx->tile_run);
+    XT_HIP(ctx, hipGetLastError());
+  }
   return XT_OK;
 }
 
@@ -1300,7 +1637,7 @@ template <typename F>
 static void for_each_need(Scratch& s, int64_t nv, int64_t ne, bool per_vertex, F&& f) {
   const int64_t nvv = nv > 0 ? nv : 1, nee = ne > 0 ? ne : 1;
   const int64_t nchunks = (nee + kChunk - 1) / kChunk;
-  f(s.deg, (size_t)nvv * sizeof(uint32_t));
+  f(s.deg, (size_t)(2 * nvv) * sizeof(uint32_t));  // degree keys, then tile counts (group and wave)
   f(s.dplus, (size_t)nvv * sizeof(uint32_t));
   f(s.tailcnt, (size_t)nchunks * sizeof(uint32_t));
   f(s.qscan, (size_t)(nchunks + 1) * sizeof(int64_t));
@@ -1311,10 +1648,10 @@ static void for_each_need(Scratch& s, int64_t nv, int64_t ne, bool per_vertex, F
   f(s.crows, (size_t)(2 * nchunks) * sizeof(int64_t));
   f(s.itcnt, (size_t)nvv * sizeof(uint32_t));
   f(s.items, (size_t)nee * sizeof(Item));
-  f(s.tileoff, (size_t)(nvv + 1) * sizeof(int64_t));
+  f(s.tileoff, (size_t)(2 * nvv + 1) * sizeof(int64_t));
   f(s.tiledesc, (size_t)tiles_bound(nvv, nee) * sizeof(TileDesc));
   if (nee < (1ll << 32)) f(s.vrec, (size_t)nvv * sizeof(uint64_t));
-  f(s.scan_tmp, scan_tmp_bytes(nvv) > scan_tmp_bytes(nchunks) ? scan_tmp_bytes(nvv) : scan_tmp_bytes(nchunks));
+  f(s.scan_tmp, scan_tmp_bytes(2 * nvv) > scan_tmp_bytes(nchunks) ? scan_tmp_bytes(2 * nvv) : scan_tmp_bytes(nchunks));
   if (per_vertex) f(s.pv, (size_t)nvv * sizeof(uint64_t));
 }
 

@@ -7,6 +7,8 @@
 //    (graph.hpp:922-926).  The reference tests all n^2/2 pairs; here points
 //    are binned into square cells of side slightly above r and only the 3x3
 //    neighbouring cells are tested, which yields the identical edge set.
+//    Options outside the reference (xt_rgg_generate_opts): the strict test
+//    distance < r, and the connectivity radius sqrt(2 ln n / (pi n)).
 //    This file is compiled with -ffp-contract=off so dx*dx + dy*dy rounds
 //    exactly like the reference's host build (x86-64 baseline, no FMA).
 //  * vite binary edge lists (graph.hpp:294-395).
@@ -135,13 +137,21 @@ double xt_rgg_radius(int64_t nv) {
   return (rc + rt) / (double)2.0;
 }
 
-int xt_rgg_generate_ranks(int64_t nv, int nprocs, uint32_t seed, double radius, int64_t* offsets,
-                          int64_t* tails, int64_t cap, int64_t* ne_out) {
+double xt_rgg_radius_connect(int64_t nv) {
+  return std::sqrt(2.0 * std::log((double)nv) / (M_PI * (double)nv));
+}
+
+int xt_rgg_generate_opts(int64_t nv, int nprocs, uint32_t seed, double radius, uint32_t flags,
+                         int64_t* offsets, int64_t* tails, int64_t cap, int64_t* ne_out) {
   return xt_guarded([&]() -> int {
     if (nv <= 0 || !offsets || !ne_out) return XT_ERR_ARG;
+    if (flags & ~(XT_RGG_STRICT | XT_RGG_CONNECT)) return XT_ERR_ARG;
     // graph.hpp:765-793: a power-of-two rank count dividing nv, strips wider than r
     if (nprocs < 1 || (nprocs & (nprocs - 1)) != 0 || nv % nprocs != 0) return XT_ERR_ARG;
-    const double r = radius > 0 ? radius : xt_rgg_radius(nv);
+    const bool strict = (flags & XT_RGG_STRICT) != 0;
+    const double r = radius > 0                  ? radius
+                     : (flags & XT_RGG_CONNECT) ? xt_rgg_radius_connect(nv)
+                                                : xt_rgg_radius(nv);
     if (!((1.0 / (double)nprocs) > r)) return XT_ERR_ARG;
     const int64_t nl = nv / nprocs;
     std::vector<double> X, Y;
@@ -196,7 +206,7 @@ int xt_rgg_generate_ranks(int64_t nv, int nprocs, uint32_t seed, double radius, 
               const double dx = X[p] - X[q];
               const double dy = Y[p] - Y[q];
               const double ed = std::sqrt(dx * dx + dy * dy);
-              if (ed <= r) row.push_back(j);
+              if (strict ? ed < r : ed <= r) row.push_back(j);
             }
           }
         std::sort(row.begin(), row.end());
@@ -209,6 +219,11 @@ int xt_rgg_generate_ranks(int64_t nv, int nprocs, uint32_t seed, double radius, 
     for (int64_t i = 0; i < nv; i++) std::copy(rows[i].begin(), rows[i].end(), tails + offsets[i]);
   return XT_OK;
   });
+}
+
+int xt_rgg_generate_ranks(int64_t nv, int nprocs, uint32_t seed, double radius, int64_t* offsets,
+                          int64_t* tails, int64_t cap, int64_t* ne_out) {
+  return xt_rgg_generate_opts(nv, nprocs, seed, radius, 0, offsets, tails, cap, ne_out);
 }
 
 int xt_rgg_generate(int64_t nv, uint32_t seed, double radius, int64_t* offsets, int64_t* tails,

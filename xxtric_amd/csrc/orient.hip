@@ -145,7 +145,9 @@ __global__ __launch_bounds__(kBlock) void k_orient_count(
   __shared__ int64_t s_off[kOffCap + 1];
   __shared__ uint32_t s_cnt[kOffCap];
   __shared__ uint32_t s_tail;
-  const int64_t c = blockIdx.x;
+  const int64_t nchunks = (ne + kChunk - 1) / kChunk;
+  const int64_t c = xcd_block(nchunks);
+  if (c >= nchunks) return;  // whole workgroup
   const int64_t eb = c * kChunk;
   const int64_t ee = min(ne, eb + (int64_t)kChunk);
   if (threadIdx.x == 0) s_tail = 0;
@@ -233,7 +235,9 @@ __global__ __launch_bounds__(kBlock) void k_orient_fill(
     const uint8_t* __restrict__ oflags, const int64_t* __restrict__ ooff,
     const int64_t* __restrict__ q, const int64_t* __restrict__ crows, int32_t* __restrict__ otails) {
   __shared__ int64_t s_scan[kBlock / kWave];
-  const int64_t c = blockIdx.x;
+  const int64_t nchunks = (ne + kChunk - 1) / kChunk;
+  const int64_t c = xcd_block(nchunks);
+  if (c >= nchunks) return;  // whole workgroup
   const int64_t eb = c * kChunk;
   const int64_t ee = min(ne, eb + (int64_t)kChunk);
   const int64_t e0 = eb + (int64_t)threadIdx.x * kPerThread;
@@ -425,17 +429,17 @@ int orient(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne, const int64_t* off, 
   uint64_t* counters = s.counters.as<uint64_t>();
   uint8_t* oflags = s.oflags.as<uint8_t>();
   if (mode == kOrientDegree)
-    hipLaunchKernelGGL((k_orient_count<T, kOrientDegree>), dim3((unsigned)nchunks), dim3(kBlock), 0,
+    hipLaunchKernelGGL((k_orient_count<T, kOrientDegree>), dim3(xcd_grid(nchunks)), dim3(kBlock), 0,
                        st, nv, ne, off, tails, (const uint16_t*)deg, (const int64_t*)crows,
                        s.dplus.as<uint32_t>(), s.tailcnt.as<uint32_t>(), counters, oflags);
   else
-    hipLaunchKernelGGL((k_orient_count<T, kOrientId>), dim3((unsigned)nchunks), dim3(kBlock), 0, st,
+    hipLaunchKernelGGL((k_orient_count<T, kOrientId>), dim3(xcd_grid(nchunks)), dim3(kBlock), 0, st,
                        nv, ne, off, tails, (const uint16_t*)deg, (const int64_t*)crows,
                        s.dplus.as<uint32_t>(), s.tailcnt.as<uint32_t>(), counters, oflags);
   XT_HIP(ctx, hipGetLastError());
   XT_TRY(scan_u32_to_i64(ctx, s, s.dplus.as<uint32_t>(), nv, s.ooff.as<int64_t>(), st));
   XT_TRY(scan_u32_to_i64(ctx, s, s.tailcnt.as<uint32_t>(), nchunks, s.qscan.as<int64_t>(), st));
-  hipLaunchKernelGGL((k_orient_fill<T>), dim3((unsigned)nchunks), dim3(kBlock), 0, st, nv, ne, off,
+  hipLaunchKernelGGL((k_orient_fill<T>), dim3(xcd_grid(nchunks)), dim3(kBlock), 0, st, nv, ne, off,
                      tails, (const uint8_t*)oflags, (const int64_t*)s.ooff.as<int64_t>(),
                      (const int64_t*)s.qscan.as<int64_t>(), (const int64_t*)crows, s.otails.as<int32_t>());
   XT_HIP(ctx, hipGetLastError());

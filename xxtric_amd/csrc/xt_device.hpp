@@ -9,6 +9,22 @@ namespace xt {
 
 constexpr int kWave = 64;  // CDNA wavefront width
 
+// XCD-aware block order.  The chip's eight XCDs (each with its own L2) receive
+// workgroups round-robin by workgroup index, so logical block
+// (b mod 8) * per + b / 8 hands XCD x the contiguous blocks [x per, (x+1) per):
+// neighbouring chunks of the input -- rows of the same small graph in a
+// batch -- are read through one L2 instead of eight.  Launch xcd_grid(n)
+// workgroups; a logical block >= n does nothing.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int64_t xcd_block(int64_t nblocks) {
+  const int64_t b = blockIdx.x;
+  const int64_t per = (nblocks + kXcds - 1) / kXcds;
+  return (b % kXcds) * per + b / kXcds;
+}
+inline unsigned xcd_grid(int64_t nblocks) {
+  return (unsigned)(((nblocks + kXcds - 1) / kXcds) * kXcds);
+}
+
 // Index of the row containing entry e: the largest r in [lo, hi] with
 // off[r] <= e (rows may be empty, so equal offsets are skipped to the last).
 __device__ __forceinline__ int64_t row_of(const int64_t* __restrict__ off, int64_t lo, int64_t hi,

@@ -22,8 +22,13 @@ constexpr int kChunk = kBlock * kPerThread;  // 2048
 // last chunk of the last list may read up to three ids past |E+|.
 constexpr int64_t kOtailsPad = 4;
 constexpr int kLightCut = 32;       // default light cut, one graph
-constexpr int kBatchLightCut = 64;  // default light cut, batches and the server
+constexpr int kBatchLightCut = 8;   // default light cut, batches and the server
 constexpr int kOffCap = 2048;
+// Wave tiles (k_count_waves): owners whose out-list has at most this many ids
+// are counted one wave per tile; 0 turns them off.
+constexpr int kWaveOwner = 128;      // default, one graph
+constexpr int kBatchWaveOwner = 64;  // default, batches and the server
+constexpr int kWaveBlocksPerCU = 8;  // persistent 256-thread workgroups per CU
 
 // Counter slots in Scratch::counters (uint64).
 enum Slot : int {
@@ -39,7 +44,8 @@ enum Slot : int {
   kHeavyItems = 11,    // heavy edges listed under their owners by the counting pass
   kOwned = 12,         // owned-range count: triangles whose minimum vertex is owned
   kLocal = 13,         // ... and whose middle vertex is owned too
-  kNumSlots = 16
+  kWaveQueue = 14,     // next wave tile of k_count_waves, one queue per XCD (14..21)
+  kNumSlots = 24
 };
 
 // Orientation rules.
@@ -58,7 +64,7 @@ struct DevBuf {
 enum Ev : int { kEvStart = 0, kEvOriented, kEvLight, kEvItems, kEvCounted, kNumEv };
 
 struct Scratch {
-  DevBuf deg;       // uint32[nv]          degree of each vertex; later tiles per group owner
+  DevBuf deg;       // uint32[2nv]         degree of each vertex; later tiles per owner (group, wave)
   DevBuf dplus;     // uint32[nv]          out-degree of each vertex
   DevBuf tailcnt;   // uint32[nchunks]     flagged entries of each chunk's last vertex
   DevBuf qscan;     // int64[nchunks+1]    exclusive scan of tailcnt
@@ -70,7 +76,7 @@ struct Scratch {
   DevBuf crows;     // int64[2*nchunks]    first and last row of each kChunk-entry input chunk
   DevBuf itcnt;     // uint32[nv]          heavy edges owned by each vertex
   DevBuf items;     // 16 B[ne]            item lists: owner x's {ooff[y], d+(y), y} records from items[off[x]]
-  DevBuf tileoff;   // int64[nv+1]         exclusive scan of tiles per owner
+  DevBuf tileoff;   // int64[2nv+1]        exclusive scan of tiles per owner (group tiles, then wave tiles)
   DevBuf tiledesc;  // 32 B[nv+ne/256+1]  {first item, ooff[x], x, items, d+(x)} of each tile
   DevBuf pv;        // uint64[nv]          per-vertex counts
   DevBuf scan_tmp;  // int64[..]           block sums of the scans
@@ -90,13 +96,13 @@ struct xt_ctx {
   hipEvent_t ev[xt::kNumEv] = {};
   xt_timing timing = {};
   // Light edges: both out-lists at most this long; 0 picks kLightCut for one
-  // graph and kBatchLightCut for batches (lists of small sparse graphs, where
-  // per-thread intersections beat the group kernel up to ~64 ids).
+  // graph and kBatchLightCut for batches (where the rest goes to wave tiles).
   int small_cut = 0;
   int group_blocks = 4;   // persistent group-kernel workgroups per CU
   int group_round = 2;    // 16-byte chunks per thread per group-kernel streaming round (1, 2, 4)
   int items_per_tile = 256;  // items of one group tile (64, 128 or 256)
   int tile_run = 16;         // consecutive tiles a group-kernel workgroup draws at once (1..64)
+  int wave_owner = -1;       // wave tiles for owner lists up to this long; -1 default, 0 off
   bool force_wide = false;  // use the 64-bit offset layout of graphs with >= 2^32 entries everywhere
   int fail_allocs = 0;      // failure injection: scratch allocations left to fail
 };

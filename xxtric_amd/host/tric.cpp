@@ -19,6 +19,8 @@
 //   -N P      with -n: generate the graph the reference builds on P MPI ranks
 //             (`mpiexec -n P`: per-rank LCG streams and strips, cross-rank
 //             edges as its exchange leaves them), all of it counted here
+//   -C        with -n: radius sqrt(2 ln N / (pi N)) and edges at distance < r
+//             (the reference uses graph.hpp:789-791 and <=)
 #include <unistd.h>
 
 #include <chrono>
@@ -42,6 +44,7 @@ struct Opts {
   int balanced = 0;   // 1: -b (greedy bins), 2: -d (linear partitioning)
   int bin_ranks = 0;  // -B K
   int gen_ranks = 1;  // -N P
+  uint32_t rgg_flags = 0;  // -C
   uint64_t seed = 1;
   bool seed_set = false;
   uint32_t flags = XT_ORIENT_DEGREE;
@@ -55,7 +58,7 @@ struct Opts {
 Opts parse(int argc, char* argv[]) {
   Opts o;
   int c;
-  while ((c = getopt(argc, argv, "f:r:n:p:olbds:R:e:S:Pm:g:v:w:xLjt:B:N:")) != -1) {
+  while ((c = getopt(argc, argv, "f:r:n:p:olbds:R:e:S:Pm:g:v:w:xLjt:B:N:C")) != -1) {
     switch (c) {
       case 'f': o.file = optarg; break;
       case 'b': o.balanced = 1; break;
@@ -63,6 +66,7 @@ Opts parse(int argc, char* argv[]) {
       case 'o': break;
       case 'B': o.bin_ranks = atoi(optarg); break;
       case 'N': o.gen_ranks = atoi(optarg); break;
+      case 'C': o.rgg_flags = XT_RGG_STRICT | XT_RGG_CONNECT; break;
       case 'r': case 's': break;             // ranks per node / buffer size: no meaning at one process
       case 'n': o.nv = atol(optarg); break;
       case 'l': o.lcg = true; break;
@@ -109,7 +113,7 @@ int main(int argc, char* argv[]) {
       if (xt_graph_rmat(ctx, o.rmat, o.ef, 0.57, 0.19, 0.19, o.seed, o.permute, &dg) != XT_OK)
         die(std::string("R-MAT generation failed: ") + xt_last_error(ctx));
     } else if (o.nv > 0) {
-      GenerateRGG gr(o.nv, 0.0, o.gen_ranks);
+      GenerateRGG gr(o.nv, 0.0, o.gen_ranks, o.rgg_flags);
       g = gr.generate(o.lcg, true, o.pct, o.seed_set ? (uint32_t)o.seed : 0);
     } else {
       BinaryEdgeList rm;

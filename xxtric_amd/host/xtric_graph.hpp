@@ -146,9 +146,11 @@ class GenerateRGG {            // graph.hpp:745-1374
  public:
   // nprocs > 1: the graph the reference generates on that many MPI ranks
   // (its constructor's checks, graph.hpp:765-793), all ranks' rows here.
-  explicit GenerateRGG(GraphElem nv, double radius = 0.0, int nprocs = 1)
-      : nv_(nv), radius_(radius), nprocs_(nprocs) {
-    rn_ = radius > 0 ? radius : xt_rgg_radius(nv);
+  // rgg_flags: XT_RGG_STRICT (distance < r) and XT_RGG_CONNECT (r = sqrt(2 ln n /
+  // (pi n)) when radius <= 0); 0 is the reference's generator.
+  explicit GenerateRGG(GraphElem nv, double radius = 0.0, int nprocs = 1, uint32_t rgg_flags = 0)
+      : nv_(nv), radius_(radius), nprocs_(nprocs), rgg_flags_(rgg_flags) {
+    rn_ = radius > 0 ? radius : (rgg_flags & XT_RGG_CONNECT) ? xt_rgg_radius_connect(nv) : xt_rgg_radius(nv);
     if (nprocs_ < 1 || (nprocs_ & (nprocs_ - 1)) != 0)
       throw std::runtime_error("[ERROR] Number of processes must be a power of 2.");
     if (nv_ % nprocs_ != 0)
@@ -166,10 +168,10 @@ class GenerateRGG {            // graph.hpp:745-1374
     const uint32_t s = seed ? seed : (isLCG ? 1u : 2u);
     int64_t ne = 0;
     std::vector<int64_t> off(nv_ + 1);
-    if (xt_rgg_generate_ranks(nv_, nprocs_, s, radius_, off.data(), nullptr, 0, &ne) != XT_OK)
+    if (xt_rgg_generate_opts(nv_, nprocs_, s, radius_, rgg_flags_, off.data(), nullptr, 0, &ne) != XT_OK)
       throw std::runtime_error("RGG generation failed");
     std::vector<int64_t> tails(ne);
-    xt_rgg_generate_ranks(nv_, nprocs_, s, radius_, off.data(), tails.data(), ne, &ne);
+    xt_rgg_generate_opts(nv_, nprocs_, s, radius_, rgg_flags_, off.data(), tails.data(), ne, &ne);
     if (randomEdgePercent > 0.0) add_random_edges(off, tails, randomEdgePercent, s);
     return graph_from_csr(off, tails);
   }
@@ -204,4 +206,5 @@ class GenerateRGG {            // graph.hpp:745-1374
   GraphElem nv_;
   double radius_, rn_;
   int nprocs_;
+  uint32_t rgg_flags_;
 };
