@@ -351,7 +351,7 @@ def _serve_pool(ctx, pool):
     return _SERVE_POOL[pool]
 
 
-def serve_workload(ctx, rate, n_req=4000, pool=400, seed=7):
+def serve_workload(ctx, rate, n_req=4000, pool=400, seed=7, inflight=4):
     """Poisson arrivals at `rate` graphs/s of the batch mix plus 1 % R-MAT
     s18-s20 requests, submitted by the library's native request generator
     (xt_serve_replay: borrowed submissions at their scheduled times, every
@@ -363,7 +363,7 @@ def serve_workload(ctx, rate, n_req=4000, pool=400, seed=7):
     big = rng.random(n_req) < 0.01
     kinds[big] = pool + rng.integers(0, 3, big.sum())
     arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req)) * 1e6  # us
-    srv = xxtric_amd.Server(ctx, hbm_budget_bytes=16 << 30, max_batch_graphs=4096, max_inflight=2)
+    srv = xxtric_amd.Server(ctx, hbm_budget_bytes=16 << 30, max_batch_graphs=4096, max_inflight=inflight)
     base = srv.now_us() + 1000.0
     bad, lat = srv.replay(graphs, solo, kinds, base + arrivals)
     st = srv.stats()
@@ -371,11 +371,12 @@ def serve_workload(ctx, rate, n_req=4000, pool=400, seed=7):
     ok = lat >= 0
     small, large = lat[ok & (kinds < pool)], lat[ok & (kinds >= pool)]
     pct = lambda v, q: round(float(np.percentile(v, q)), 1) if len(v) else None  # noqa: E731
-    return {"rate_graphs_per_s": rate, "requests": n_req, "graphs_per_s": round(st["graphs_per_s"], 1),
+    return {"rate_graphs_per_s": rate, "requests": n_req, "slots": inflight, "graphs_per_s": round(st["graphs_per_s"], 1),
             "p50_us": round(st["p50_us"], 1), "p99_us": round(st["p99_us"], 1),
             "batches": st["batches"], "isolated_batches": st["isolated_batches"],
             "rejected": st["rejected"], "counts_equal_solo": bad == 0 and st["rejected"] == 0,
             "submitter": "native (xt_serve_replay)",
+            "worker_s": {k: round(st[k], 4) for k in ("admit_s", "stage_s", "launch_s", "complete_s", "device_s")},
             # the same latencies split by request class: the pool's small graphs and the
             # 1 % R-MAT s18-s20 requests
             "small": {"requests": int(len(small)), "p50_us": pct(small, 50), "p99_us": pct(small, 99)},

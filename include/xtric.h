@@ -371,6 +371,11 @@ typedef struct xt_serve_stats {
   double graphs_per_s;                    /* completed / (last done - first arrival) */
   int64_t peak_resident_bytes;
   int64_t isolated_batches;               /* batches holding one large request alone */
+  /* Where the worker thread's time went (seconds): admitting and laying out
+   * requests, copying them into pinned staging memory (with the copy pool),
+   * launching batches (upload + kernels enqueued) and completing them (result
+   * read-back, latencies); device_s sums the batches' kernel time on the GPU. */
+  double admit_s, stage_s, launch_s, complete_s, device_s;
 } xt_serve_stats;
 int xt_serve_create(xt_ctx* ctx, const xt_serve_config* cfg, xt_server** out);
 /* Non-blocking; the CSR is copied into server-owned host memory before return.

@@ -78,7 +78,9 @@ class XtServeConfig(ctypes.Structure):
 class XtServeStats(ctypes.Structure):
     _fields_ = [("submitted", _i64), ("completed", _i64), ("rejected", _i64), ("batches", _i64),
                 ("p50_us", _dbl), ("p99_us", _dbl), ("mean_us", _dbl), ("max_us", _dbl),
-                ("graphs_per_s", _dbl), ("peak_resident_bytes", _i64), ("isolated_batches", _i64)]
+                ("graphs_per_s", _dbl), ("peak_resident_bytes", _i64), ("isolated_batches", _i64),
+                ("admit_s", _dbl), ("stage_s", _dbl), ("launch_s", _dbl), ("complete_s", _dbl),
+                ("device_s", _dbl)]
 
 
 _SIGS = {

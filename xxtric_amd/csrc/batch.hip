@@ -122,6 +122,7 @@ struct xt_batch {
   int64_t* h_loff = nullptr;
   int32_t* h_ltails = nullptr;
   int64_t* h_rec = nullptr;  // kRecWords per staged graph
+  uint64_t* h_totals = nullptr;  // pinned per-graph totals copied back by each launch (batch_readback_reserve)
   int64_t* d_rec = nullptr;
   int64_t pend_g = 0;         // graphs [pend_g, graphs.size()) are staged, not yet on the device
   hipEvent_t ev_stage = nullptr;
@@ -140,7 +141,7 @@ static void batch_free(xt_batch* b) {
   void* ps[] = {b->goff, b->gtails, b->gid, b->loff, b->ltails, b->gtotal, b->ptotal, b->d_rec};
   for (void* p : ps)
     if (p) (void)hipFree(p);
-  void* hs[] = {b->h_loff, b->h_ltails, b->h_rec};
+  void* hs[] = {b->h_loff, b->h_ltails, b->h_rec, b->h_totals};
   for (void* p : hs)
     if (p) (void)hipHostFree(p);
   if (b->ev_stage) (void)hipEventDestroy(b->ev_stage);
@@ -253,6 +254,8 @@ int batch_launch(xt_batch* b, int mode, uint32_t flags) {
   if (mode == XT_BATCH_PERSISTENT) {
     XT_HIP(ctx, hipEventRecord(b->ev0, st));
     XT_TRY(batch_pipeline(b, flags));
+    if (b->h_totals)  // the totals follow the kernels: done with the launch's last event
+      XT_HIP(ctx, hipMemcpyAsync(b->h_totals, b->gtotal, (size_t)G * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
     XT_HIP(ctx, hipEventRecord(b->ev1, st));
   } else if (mode == XT_BATCH_HIPGRAPH) {
     if (!b->exec || b->exec_V != b->V || b->exec_E != b->E || b->exec_G != G ||
@@ -330,10 +333,24 @@ int batch_wait(xt_batch* b) {
   return batch_poll(b) < 0 ? XT_ERR_HIP : XT_OK;
 }
 
+// Pinned memory that each persistent launch copies the per-graph totals into
+// (after its kernels, on its stream), so a finished launch needs no further
+// device call to read them (batch_read).
+int batch_readback_reserve(xt_batch* b) {
+  if (b->h_totals) return XT_OK;
+  hipError_t e = hipHostMalloc((void**)&b->h_totals, (size_t)(b->max_g > 0 ? b->max_g : 1) * sizeof(uint64_t));
+  if (e != hipSuccess) return hip_err(b->ctx, e, "batch readback buffer");
+  return XT_OK;
+}
+
 int batch_read(xt_batch* b, uint64_t* totals, int64_t n) {
   const int64_t G = (int64_t)b->graphs.size();
   if (n > G) n = G;
   if (n <= 0) return XT_OK;
+  if (b->h_totals && b->last_mode == XT_BATCH_PERSISTENT && !b->running) {  // copied back by the launch
+    std::memcpy(totals, b->h_totals, (size_t)n * sizeof(uint64_t));
+    return XT_OK;
+  }
   const uint64_t* src = b->last_mode == XT_BATCH_PER_GRAPH ? b->ptotal : b->gtotal;
   XT_HIP(b->ctx, hipMemcpyAsync(totals, src, (size_t)n * sizeof(uint64_t), hipMemcpyDeviceToHost, b->stream));
   XT_HIP(b->ctx, hipStreamSynchronize(b->stream));

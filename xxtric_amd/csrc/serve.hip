@@ -33,6 +33,7 @@ int batch_read(xt_batch* b, uint64_t* totals, int64_t n);
 const char* check_host_graph(int64_t nv, int64_t ne, const int64_t* off, const int32_t* tails,
                              int64_t* off_out, int32_t* tails_out);
 int batch_stage_reserve(xt_batch* b);
+int batch_readback_reserve(xt_batch* b);
 int batch_stage_host(xt_batch* b, int64_t nv, int64_t ne, const int64_t* off, const int32_t* tails);
 const char* batch_stage_host_checked(xt_batch* b, int64_t nv, int64_t ne, const int64_t* off,
                                      const int32_t* tails, int* rc);
@@ -212,6 +213,7 @@ struct xt_server {
   std::vector<double> latencies;
   double first_arrival = -1, last_done = 0;
   int64_t submitted = 0, completed = 0, rejected = 0, batches = 0, isolated = 0;
+  double t_admit = 0, t_stage = 0, t_launch = 0, t_complete = 0, t_device = 0;  // seconds (under mu)
   int worker_err = XT_OK;
   bool stop = false;
   bool paused = false;  // admission held (xt_serve_pause); running batches still finish
@@ -227,11 +229,14 @@ struct xt_server {
 namespace {
 
 void complete_slot(xt_server* s, ServeSlot& sl) {
+  const double t_in = s->now();
   std::vector<uint64_t> tot(sl.reqs.size());
   int rc = batch_read(sl.b, tot.data(), (int64_t)tot.size());
   const double t = s->now();
   std::lock_guard<std::mutex> lk(s->mu);
   if (rc != XT_OK) s->worker_err = rc;
+  xt_batch_timing bt = {};
+  if (xt_batch_timing_get(sl.b, &bt) == XT_OK) s->t_device += bt.run_ms * 1e-3;
   for (size_t i = 0; i < sl.reqs.size(); i++) {
     Req* r = sl.reqs[i];
     r->total = tot[i];
@@ -248,6 +253,7 @@ void complete_slot(xt_server* s, ServeSlot& sl) {
   sl.inflight = false;
   (void)xt_batch_clear(sl.b);
   s->cv_done.notify_all();
+  s->t_complete += (s->now() - t_in) * 1e-6;
 }
 
 // The slot's requests are in its staging mirror: requests that failed their
@@ -271,8 +277,11 @@ void launch_staged(xt_server* s, ServeSlot& sl) {
   }
   sl.staging = false;
   sl.job.reset();
+  const double t_in = s->now();
   int rc = batch_launch(sl.b, XT_BATCH_PERSISTENT, XT_ORIENT_DEGREE);
+  const double t_out = s->now();
   std::lock_guard<std::mutex> lk(s->mu);
+  s->t_launch += (t_out - t_in) * 1e-6;
   if (rc != XT_OK) s->worker_err = rc;
   sl.inflight = true;
   s->batches++;
@@ -301,6 +310,7 @@ void worker_main(xt_server* s) {
       held = s->paused && !s->stop;
     }
     if (free_slot && !held) {
+      const double t_admit0 = s->now();
       std::vector<Req*> take;
       bool alone = false;  // one large request admitted by itself (below saturation)
       {
@@ -399,6 +409,11 @@ void worker_main(xt_server* s) {
           sl.job->tasks.push_back(CopyTask{nullptr, tails, nullptr, sl.tdst[i], r->nv, lo,
                                            std::min(r->ne, lo + kCopyRange), borrowed, &sl.bad[i]});
       }
+      const double t_admit1 = s->now();
+      {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->t_admit += (t_admit1 - t_admit0) * 1e-6;
+      }
       if (!sl.reqs.empty()) {
         if (alone && entries >= kAsyncStage) {
           // Below saturation: the large request is copied in the background
@@ -409,6 +424,10 @@ void worker_main(xt_server* s) {
           // Near saturation everything is batched, and the worker helps the
           // pool copy, for throughput.
           s->pool->run(sl.job);
+          {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->t_stage += (s->now() - t_admit1) * 1e-6;
+          }
           launch_staged(s, sl);
         }
         progressed = true;
@@ -457,6 +476,7 @@ int xt_serve_create(xt_ctx* ctx, const xt_serve_config* cfg, xt_server** out) {
     for (ServeSlot& sl : s->slots) {
       int rc = xt_batch_create(ctx, s->slot_v, s->slot_e, s->slot_g, &sl.b);
       if (rc == XT_OK) rc = batch_stage_reserve(sl.b);  // pinned staging now, not on the first request
+      if (rc == XT_OK) rc = batch_readback_reserve(sl.b);  // results come back with each launch
       if (rc != XT_OK) {
         for (ServeSlot& o : s->slots)
           if (o.b) xt_batch_destroy(o.b);
@@ -656,6 +676,11 @@ int xt_serve_stats_get(xt_server* s, xt_serve_stats* out) {
     st.batches = s->batches;
     st.peak_resident_bytes = s->peak_resident;
     st.isolated_batches = s->isolated;
+    st.admit_s = s->t_admit;
+    st.stage_s = s->t_stage;
+    st.launch_s = s->t_launch;
+    st.complete_s = s->t_complete;
+    st.device_s = s->t_device;
     if (!s->latencies.empty()) {
       std::vector<double> v = s->latencies;
       std::sort(v.begin(), v.end());
