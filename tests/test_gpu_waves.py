@@ -153,3 +153,31 @@ def test_batch_wave_settings_agree(ctx):
         _restore(ctx)
         b.close()
 
+
+
+@pytest.mark.parametrize("wave_owner", [-2, 16, 128])
+@pytest.mark.parametrize("small_cut", [1, 32])
+def test_wave_tiles_owned_ranges(ctx, wave_owner, small_cut):
+    """Owned-range counts with wave tiles (minimum and middle vertex split by
+    lane masks in k_count_waves): the owned count equals the oracle's
+    per-vertex counts summed over the range, the local count a brute-force
+    enumeration of (minimum, middle) pairs, for the ranges of an 8-way split."""
+    from helpers import min_mid_pairs
+    off, tails = O.rmat_csr(12, 16, 9, True)
+    off = np.asarray(off, np.int64)
+    tails = np.asarray(tails, np.int64)
+    n = len(off) - 1
+    total, pv = O.count_reference(off, tails, per_vertex=True)
+    us, vs, cs = min_mid_pairs(off, tails)
+    g = ctx.graph(off, tails)
+    ctx.tune(small_cut=small_cut, wave_owner=wave_owner)
+    try:
+        for r in range(8):
+            lo, hi = n * r // 8, n * (r + 1) // 8
+            _say("waves owned", wave_owner, "cut", small_cut, "range", lo, hi)
+            owned, local, _ = g.count_owned(lo, hi, total=False)
+            assert owned == int(pv[lo:hi].sum())
+            assert local == int(cs[(us >= lo) & (us < hi) & (vs < hi)].sum())
+    finally:
+        _restore(ctx)
+        g.close()

@@ -148,6 +148,32 @@ __device__ __forceinline__ uint32_t light_intersect(const int32_t* __restrict__ 
   return h;
 }
 
+// The same for two lists of at most eight ids each (a light cut of 8 or
+// less): all four 16-byte pieces are loaded at once and every pair of ids is
+// compared, so the edge waits on one round of loads instead of a walk.
+// Padding ids (-1 and -2) never match; loads stay inside each list's first
+// eight ids or the ids that follow it (kOtailsPad at the end).
+__device__ __forceinline__ uint32_t light_intersect8(const int32_t* __restrict__ ot, const EdgeLists& L) {
+  auto piece = [&](int64_t i) -> v4i_a4 {
+    return *reinterpret_cast<const gv4i_a4*>(reinterpret_cast<uint64_t>(ot + i));
+  };
+  const int64_t ns = L.s1 - L.s0, nl = L.l1 - L.l0;
+  const v4i_a4 p = piece(L.s0), t = piece(L.l0);
+  v4i_a4 q = {-1, -1, -1, -1}, u = {-2, -2, -2, -2};
+  if (ns > 4) q = piece(L.s0 + 4);
+  if (nl > 4) u = piece(L.l0 + 4);
+  const int32_t a[8] = {p.x,               ns > 1 ? p.y : -1, ns > 2 ? p.z : -1, ns > 3 ? p.w : -1,
+                        ns > 4 ? q.x : -1, ns > 5 ? q.y : -1, ns > 6 ? q.z : -1, ns > 7 ? q.w : -1};
+  const int32_t b[8] = {t.x,               nl > 1 ? t.y : -2, nl > 2 ? t.z : -2, nl > 3 ? t.w : -2,
+                        nl > 4 ? u.x : -2, nl > 5 ? u.y : -2, nl > 6 ? u.z : -2, nl > 7 ? u.w : -2};
+  uint32_t h = 0;
+#pragma unroll
+  for (int i = 0; i < 8; i++)
+#pragma unroll
+    for (int k = 0; k < 8; k++) h += a[i] == b[k] ? 1u : 0u;
+  return h;
+}
+
 // Lane l's value of v, in every lane (l uniform).
 __device__ __forceinline__ int64_t readlane64(int64_t v, int l) {
   const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
@@ -452,7 +478,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8))) 
           const EdgeLists L = edge_lists(a0, a1, (int64_t)gb[j], (int64_t)gb[j] + (int64_t)gl[j]);
           uint32_t h = 0;
           if constexpr (CM == kCreditTotal) {
-            h = light_intersect(ot, L);
+            h = light_cut <= 8 ? light_intersect8(ot, L) : light_intersect(ot, L);  // uniform choice
           } else {
             // Credit modes need each common id: each id of the shorter list is
             // searched in the rest of the longer one (fewer registers than the
@@ -1215,14 +1241,19 @@ struct WaveTile {
 struct WaveChunk {
   int4 v;
   int32_t r;
+  int32_t y;  // the item's partner (owned-range credit)
 };
 
-template <bool BATCH>
+// CM: kCreditTotal, or kCreditRange (owned-range counts: each hit {x, y, w}
+// is split by its minimum and middle vertex, as in the group kernel).
+template <int CM, bool BATCH>
 __global__ __launch_bounds__(kWaveThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_count_waves(
     const int32_t* __restrict__ ot, const Item* __restrict__ items, const TileDesc* __restrict__ tdesc,
     const int64_t* __restrict__ t_begin, const int64_t* __restrict__ t_end,
     uint64_t* __restrict__ counters, const int32_t* __restrict__ gid,
-    unsigned long long* __restrict__ gtotal, uint32_t tile_run) {
+    unsigned long long* __restrict__ gtotal, uint32_t tile_run, int64_t rlo, int64_t rhi) {
+  static_assert(CM == kCreditTotal || CM == kCreditRange, "plain or owned-range counts");
+  constexpr bool RANGE = CM == kCreditRange;
   __shared__ int4 s_tab[kWaveThreads / kWave][kWaveBuckets];
   const int lane = threadIdx.x & (kWave - 1);
   int4* tab = s_tab[threadIdx.x / kWave];
@@ -1271,14 +1302,16 @@ __global__ __launch_bounds__(kWaveThreads) __attribute__((amdgpu_waves_per_eu(8,
     d.valid = true;
     return d;
   };
-  // Lane k holds item k of a tile: its list's offset and length.
-  auto load_items = [&](const WaveTile& d, int64_t& ib, int32_t& il) {
+  // Lane k holds item k of a tile: its list's offset and length (and partner).
+  auto load_items = [&](const WaveTile& d, int64_t& ib, int32_t& il, int32_t& iy) {
     ib = 0;
     il = 0;
+    iy = 0;
     if (d.valid && lane < d.n) {
       const Item it = items[d.i0 + lane];
       ib = it.base;
       il = it.len;
+      iy = it.y;
     }
   };
   // Where each item's chunks start in the tile's run of chunks (lane k: item k).
@@ -1294,7 +1327,8 @@ __global__ __launch_bounds__(kWaveThreads) __attribute__((amdgpu_waves_per_eu(8,
     total = __builtin_amdgcn_readlane(inc, kWave - 1);
   };
   // Chunk c0 + lane of a tile: its item kk has pre[kk] <= c < pre[kk+1].
-  auto load_round = [&](int64_t ib, int32_t il, int32_t pre, int n, int32_t total, int32_t c0) -> WaveChunk {
+  auto load_round = [&](int64_t ib, int32_t il, int32_t iy, int32_t pre, int n, int32_t total,
+                        int32_t c0) -> WaveChunk {
     const int32_t c = c0 + lane;
     const int k0 = __popcll(__builtin_amdgcn_ballot_w64(lane < n && pre <= c0)) - 1;
     uint64_t sb = (lane < n && pre > c0 && pre < c0 + kWave) ? (1ull << (pre - c0)) : 0ull;
@@ -1304,7 +1338,8 @@ __global__ __launch_bounds__(kWaveThreads) __attribute__((amdgpu_waves_per_eu(8,
     const int64_t kb = __shfl(ib, kk, kWave);
     const int32_t kp = __shfl(pre, kk, kWave), kl = __shfl(il, kk, kWave);
     const int32_t q = c - kp;  // chunk of item kk
-    WaveChunk ch{make_int4(-1, -1, -1, -1), kl - 4 * q};  // ids inside the list: <= 0 past the run's end
+    WaveChunk ch{make_int4(-1, -1, -1, -1), kl - 4 * q, 0};  // ids inside the list: <= 0 past the run's end
+    if constexpr (RANGE) ch.y = __shfl(iy, kk, kWave);
     if (c < total) {
       const v4i_a4 tq = *reinterpret_cast<const gv4i_a4*>(reinterpret_cast<uint64_t>(ot + kb + 4 * (int64_t)q));
       ch.v = make_int4(tq.x, tq.y, tq.z, tq.w);
@@ -1327,6 +1362,16 @@ __global__ __launch_bounds__(kWaveThreads) __attribute__((amdgpu_waves_per_eu(8,
     wave_lds_fence();
   };
   uint64_t hits = 0;   // uniform: sums of ballot population counts
+  uint64_t owned = 0, local = 0;  // RANGE, uniform: hits whose minimum / minimum and middle are in [rlo, rhi)
+  int32_t cur_x = 0;   // owner of the tile being probed
+  // Owned-range split of the hits in mask `hit` (ids w, partner y, owner cur_x).
+  auto range_split = [&](uint64_t hit, int32_t w, int32_t y) {
+    const int32_t ya = min(cur_x, y), yb = max(cur_x, y);
+    const int32_t m = min(ya, w), mid = max(ya, min(yb, w));
+    const uint64_t own = hit & __builtin_amdgcn_ballot_w64((uint32_t)(m - (int32_t)rlo) < (uint32_t)(rhi - rlo));
+    owned += (uint64_t)__popcll(own);
+    local += (uint64_t)__popcll(own & __builtin_amdgcn_ballot_w64(mid < (int32_t)rhi));
+  };
   auto probe = [&](const WaveChunk& ch) {
     const int32_t w[4] = {ch.v.x, ch.v.y, ch.v.z, ch.v.w};
     int4 tt[4];
@@ -1338,6 +1383,7 @@ __global__ __launch_bounds__(kWaveThreads) __attribute__((amdgpu_waves_per_eu(8,
                             __builtin_amdgcn_ballot_w64(tt[j].z == w[j]) | __builtin_amdgcn_ballot_w64(tt[j].w == w[j])) &
                            __builtin_amdgcn_ballot_w64(j < ch.r);
       hits += (uint64_t)__popcll(hit);
+      if constexpr (RANGE) range_split(hit, w[j], ch.y);
     }
     // A key in slot 3 of a spilled bucket carries the mark (the raw compare
     // missed it), and keys that passed a full bucket sit further on.
@@ -1349,7 +1395,9 @@ __global__ __launch_bounds__(kWaveThreads) __attribute__((amdgpu_waves_per_eu(8,
           h = (tt[j].w & 0x7fffffff) == w[j];
           if (!h && bucket_miss(tt[j], w[j]) != 0) h = wave_tab_continue(tab, wave_bucket(w[j]), w[j]);
         }
-        hits += (uint64_t)__popcll(__builtin_amdgcn_ballot_w64(h));
+        const uint64_t hm = __builtin_amdgcn_ballot_w64(h);
+        hits += (uint64_t)__popcll(hm);
+        if constexpr (RANGE) range_split(hm, w[j], ch.y);
       }
     }
   };
@@ -1370,29 +1418,30 @@ __global__ __launch_bounds__(kWaveThreads) __attribute__((amdgpu_waves_per_eu(8,
   WaveTile dc = load_desc(draw());
   if (dc.valid) {
     int64_t ib_c, ib_n;
-    int32_t il_c, il_n, o0, o1;
-    load_items(dc, ib_c, il_c);
+    int32_t il_c, il_n, iy_c, iy_n, o0, o1;
+    load_items(dc, ib_c, il_c, iy_c);
     load_owner(dc, o0, o1);
     WaveTile dn = load_desc(draw());
-    load_items(dn, ib_n, il_n);
+    load_items(dn, ib_n, il_n, iy_n);
     WaveTile dnn = dn.valid ? load_desc(draw()) : none;
     enter_graph(dc);
     build_table(dc, o0, o1);
     int64_t table_a0 = dc.a0;
+    cur_x = dc.x;
     int32_t pre_c, total_c;
     chunk_starts(il_c, pre_c, total_c);
-    WaveChunk cur = load_round(ib_c, il_c, pre_c, dc.n, total_c, 0);
+    WaveChunk cur = load_round(ib_c, il_c, iy_c, pre_c, dc.n, total_c, 0);
     int32_t c0 = 0;
     for (;;) {
       const bool same = c0 + kWave < total_c;  // uniform: the tile has another round
-      WaveChunk nxt{make_int4(-1, -1, -1, -1), 0};
+      WaveChunk nxt{make_int4(-1, -1, -1, -1), 0, 0};
       int32_t pre_n = 0, total_n = 0;
       const bool fresh = !same && dn.valid && dn.a0 != table_a0;  // the next tile needs a new table
       if (same) {
-        nxt = load_round(ib_c, il_c, pre_c, dc.n, total_c, c0 + kWave);
+        nxt = load_round(ib_c, il_c, iy_c, pre_c, dc.n, total_c, c0 + kWave);
       } else if (dn.valid) {
         chunk_starts(il_n, pre_n, total_n);
-        nxt = load_round(ib_n, il_n, pre_n, dn.n, total_n, 0);
+        nxt = load_round(ib_n, il_n, iy_n, pre_n, dn.n, total_n, 0);
         if (fresh) load_owner(dn, o0, o1);
       }
       probe(cur);
@@ -1409,13 +1458,15 @@ __global__ __launch_bounds__(kWaveThreads) __attribute__((amdgpu_waves_per_eu(8,
         table_a0 = dn.a0;
       }
       dc = dn;
+      cur_x = dc.x;
       ib_c = ib_n;
       il_c = il_n;
+      iy_c = iy_n;
       pre_c = pre_n;
       total_c = total_n;
       c0 = 0;
       dn = dnn;
-      load_items(dn, ib_n, il_n);
+      load_items(dn, ib_n, il_n, iy_n);
       dnn = dn.valid ? load_desc(draw()) : none;
     }
   }
@@ -1424,6 +1475,10 @@ __global__ __launch_bounds__(kWaveThreads) __attribute__((amdgpu_waves_per_eu(8,
       atomicAdd(&gtotal[g_cur], (unsigned long long)hits);
     else
       atomicAdd((unsigned long long*)&counters[kTotal], (unsigned long long)hits);
+  }
+  if constexpr (RANGE) {
+    if (lane == 0 && owned) atomicAdd((unsigned long long*)&counters[kOwned], (unsigned long long)owned);
+    if (lane == 0 && local) atomicAdd((unsigned long long*)&counters[kLocal], (unsigned long long)local);
   }
 }
 
@@ -1497,7 +1552,7 @@ int count_oriented(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne_bound, const 
   XT_TRY(ensure(ctx, s.items, (size_t)ne_bound * sizeof(Item)));
   XT_TRY(ensure(ctx, s.tiledesc, (size_t)tiles_bound(nv, ne_bound) * sizeof(TileDesc)));
   // Owners with out-lists of at most wown ids go to wave tiles (plain counts).
-  const int wown = (range || pv) ? 0 : wave_owner_cut(ctx, batch);
+  const int wown = pv ? 0 : wave_owner_cut(ctx, batch);
   XT_TRY(ensure(ctx, s.deg, (size_t)(wown ? 2 : 1) * (size_t)nv * sizeof(uint32_t)));
   XT_TRY(ensure(ctx, s.tileoff, (size_t)((wown ? 2 : 1) * nv + 1) * sizeof(int64_t)));
   uint64_t* counters = s.counters.as<uint64_t>();
@@ -1597,14 +1652,17 @@ int count_oriented(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne_bound, const 
   if (wown) {
     const int64_t* tof = s.tileoff.as<int64_t>();
     const unsigned wblocks = (unsigned)(ctx->num_cus * kWaveBlocksPerCU);
-    if (batch)
-      hipLaunchKernelGGL(k_count_waves<true>, dim3(wblocks), dim3(kWaveThreads), 0, st, ot,
-                         (const Item*)s.items.as<Item>(), (const TileDesc*)s.tiledesc.as<TileDesc>(), tof + nv,
-                         tof + 2 * nv, counters, gid, gt, (uint32_t)ctx->tile_run);
+#define XT_WAVES(C, B)                                                                                    \
+  hipLaunchKernelGGL((k_count_waves<C, B>), dim3(wblocks), dim3(kWaveThreads), 0, st, ot,                      \
+                     (const Item*)s.items.as<Item>(), (const TileDesc*)s.tiledesc.as<TileDesc>(), tof + nv,     \
+                     tof + 2 * nv, counters, gid, gt, (uint32_t)ctx->tile_run, rlo, rhi)
+    if (range)
+      XT_WAVES(kCreditRange, false);
+    else if (batch)
+      XT_WAVES(kCreditTotal, true);
     else
-      hipLaunchKernelGGL(k_count_waves<false>, dim3(wblocks), dim3(kWaveThreads), 0, st, ot,
-                         (const Item*)s.items.as<Item>(), (const TileDesc*)s.tiledesc.as<TileDesc>(), tof + nv,
-                         tof + 2 * nv, counters, gid, gt, (uint32_t)ctx->tile_run);
+      XT_WAVES(kCreditTotal, false);
+#undef XT_WAVES
     XT_HIP(ctx, hipGetLastError());
   }
   return XT_OK;
