@@ -155,7 +155,7 @@ def test_tile_knobs_do_not_change_counts(ctx, ipt, run):
     off, tails = O.rmat_csr(14, 16, 3, True)
     exp, exp_pv = O.count_degree_ordered(off, tails, per_vertex=True)
     n = len(off) - 1
-    ctx.tune(small_cut=1, items_per_tile=ipt, tile_run=run)
+    ctx.tune(small_cut=1, items_per_tile=ipt, tile_run=run, wave_owner=-2)  # group tiles only
     try:
         _say("items_per_tile", ipt, "tile_run", run)
         tot, pv = ctx.count_csr(off, tails.astype(np.int64), per_vertex=True)
@@ -167,7 +167,7 @@ def test_tile_knobs_do_not_change_counts(ctx, ipt, run):
         assert owned == int(exp_pv[n // 4: n // 2].sum())
         g.close()
     finally:
-        ctx.tune(small_cut=xxtric_amd.DEFAULT_LIGHT_CUT, items_per_tile=-1, tile_run=-1)
+        ctx.tune(small_cut=xxtric_amd.DEFAULT_LIGHT_CUT, items_per_tile=-1, tile_run=-1, wave_owner=-1)
 
 
 def test_tile_knobs_reject_bad_values(ctx):

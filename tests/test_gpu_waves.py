@@ -82,11 +82,17 @@ def test_wave_table_spills(ctx, k):
     edges = [(a, b) for i, a in enumerate(ids) for b in ids[i + 1:]]
     off, tails = csr_from_edges(n, edges)
     exp = (k + 1) * k * (k - 1) // 6
+    # per-vertex (min-id credit): the j-th smallest id of the clique closes C(k - j, 2)
+    exp_pv = np.zeros(n, np.uint64)
+    for j, v in enumerate(ids):
+        exp_pv[v] = (k - j) * (k - j - 1) // 2
     ctx.tune(small_cut=1, wave_owner=128)
     try:
         for orient in ("degree", "id"):
             _say("spill clique", k, orient)
             assert ctx.count_csr(off, tails, orient=orient) == exp
+            tot, pv = ctx.count_csr(off, tails, orient=orient, per_vertex=True)
+            assert tot == exp and (pv == exp_pv).all()
     finally:
         _restore(ctx)
 
@@ -101,6 +107,8 @@ def test_book_graph_many_tiny_owners(ctx):
     try:
         _say("book graph", n)
         assert ctx.count_csr(off, tails) == n
+        tot, pv = ctx.count_csr(off, tails, per_vertex=True)   # every page's triangle belongs to vertex 0
+        assert tot == n and int(pv[0]) == n and int(pv.sum()) == n
         assert ctx.count_csr(off, tails, orient="id") == n
         o2, t2 = complete(40)
         assert ctx.count_csr(o2, t2) == 40 * 39 * 38 // 6
@@ -181,3 +189,27 @@ def test_wave_tiles_owned_ranges(ctx, wave_owner, small_cut):
     finally:
         _restore(ctx)
         g.close()
+
+
+@pytest.mark.parametrize("wave_owner", [1, 16, 64, 128])
+def test_wave_tiles_per_vertex(ctx, golden, wave_owner):
+    """Per-vertex counts with wave tiles (slot, partner and owner credits kept in
+    the wave's LDS and added when the owner or tile ends) equal the oracle's,
+    for every edge heavy (cut 1) and at the default cut; the batch's per-graph
+    totals with per-vertex output too."""
+    off, tails = O.rmat_csr(15, 16, 6, True)
+    exp, exp_pv = O.count_degree_ordered(off, tails, per_vertex=True)
+    try:
+        for cut in (1, 32):
+            ctx.tune(small_cut=cut, wave_owner=wave_owner)
+            for orient in ("degree", "id"):
+                _say("waves per-vertex", wave_owner, "cut", cut, orient)
+                tot, pv = ctx.count_csr(off, tails.astype(np.int64), per_vertex=True, orient=orient)
+                assert tot == exp and (pv == exp_pv).all()
+        g = golden["rgg_lcg"][3]
+        o2, t2 = xxtric_amd.rgg_generate(g["n"], seed=g["seed"])
+        e2, e2_pv = O.count_reference(o2, t2, per_vertex=True)
+        tot, pv = ctx.count_csr(o2, t2, per_vertex=True)
+        assert tot == e2 == g["triangles"] and (pv == e2_pv).all()
+    finally:
+        _restore(ctx)

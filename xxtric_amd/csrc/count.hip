@@ -1214,6 +1214,17 @@ __device__ __forceinline__ bool wave_tab_continue(const int4* tab, uint32_t b, i
   }
 }
 
+// The same, returning the slot index 4*bucket + s of w, or -1.
+__device__ __forceinline__ int wave_tab_continue_slot(const int4* tab, uint32_t b, int32_t w) {
+  for (;;) {
+    b = (b + 1) & (kWaveBuckets - 1);
+    const int4 t = tab[b];
+    const int sl = bucket_slot(t, b, w);
+    if (sl >= 0) return sl;
+    if (!bucket_spilled(t)) return -1;
+  }
+}
+
 // Orders this wave's LDS accesses before and after it (one wave owns the
 // table; LDS instructions of a wave execute in order).
 __device__ __forceinline__ void wave_lds_fence() {
@@ -1241,22 +1252,39 @@ struct WaveTile {
 struct WaveChunk {
   int4 v;
   int32_t r;
-  int32_t y;  // the item's partner (owned-range credit)
+  int32_t y;  // the item's partner (owned-range and per-vertex credit)
+  int32_t k;  // the item's index in the tile (per-vertex credit)
 };
 
-// CM: kCreditTotal, or kCreditRange (owned-range counts: each hit {x, y, w}
-// is split by its minimum and middle vertex, as in the group kernel).
-template <int CM, bool BATCH>
-__global__ __launch_bounds__(kWaveThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_count_waves(
+// CM: kCreditTotal; kCreditRange (owned-range counts: each hit {x, y, w} is
+// split by its minimum and middle vertex, as in the group kernel); or
+// kCreditVertex (per-vertex counts: each hit goes to min(x, y, w) -- to the
+// table id w through a 32-bit counter per table slot, kept while the owner
+// stays; to the partner y through a counter per item of the tile; to the
+// owner x in registers -- all added to HBM when the owner or tile ends).
+template <int CM, bool BATCH, int WPE>
+__global__ __launch_bounds__(kWaveThreads) __attribute__((amdgpu_waves_per_eu(WPE, 8))) void k_count_waves(
     const int32_t* __restrict__ ot, const Item* __restrict__ items, const TileDesc* __restrict__ tdesc,
     const int64_t* __restrict__ t_begin, const int64_t* __restrict__ t_end,
     uint64_t* __restrict__ counters, const int32_t* __restrict__ gid,
-    unsigned long long* __restrict__ gtotal, uint32_t tile_run, int64_t rlo, int64_t rhi) {
-  static_assert(CM == kCreditTotal || CM == kCreditRange, "plain or owned-range counts");
+    unsigned long long* __restrict__ gtotal, uint32_t tile_run, int64_t rlo, int64_t rhi,
+    unsigned long long* __restrict__ pv) {
   constexpr bool RANGE = CM == kCreditRange;
-  __shared__ int4 s_tab[kWaveThreads / kWave][kWaveBuckets];
+  constexpr bool PV = CM == kCreditVertex;
+  constexpr int kWaves = kWaveThreads / kWave;
+  __shared__ int4 s_tab[kWaves][kWaveBuckets];
+  __shared__ uint32_t s_wc[PV ? kWaves : 1][PV ? 4 * kWaveBuckets : 1];  // per table slot
+  __shared__ uint32_t s_yc[PV ? kWaves : 1][PV ? kWaveItems : 1];        // per item of the tile
   const int lane = threadIdx.x & (kWave - 1);
-  int4* tab = s_tab[threadIdx.x / kWave];
+  const int wv = threadIdx.x / kWave;
+  int4* tab = s_tab[wv];
+  uint32_t* wc = s_wc[PV ? wv : 0];
+  uint32_t* yc = s_yc[PV ? wv : 0];
+  if constexpr (PV) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) wc[lane + kWave * i] = 0;
+    yc[lane] = 0;
+  }
   const int64_t tb = *t_begin;
   const uint64_t nt = (uint64_t)(*t_end - tb);
   // One queue per XCD over an eighth of the tiles (consecutive tiles: the same
@@ -1338,8 +1366,8 @@ __global__ __launch_bounds__(kWaveThreads) __attribute__((amdgpu_waves_per_eu(8,
     const int64_t kb = __shfl(ib, kk, kWave);
     const int32_t kp = __shfl(pre, kk, kWave), kl = __shfl(il, kk, kWave);
     const int32_t q = c - kp;  // chunk of item kk
-    WaveChunk ch{make_int4(-1, -1, -1, -1), kl - 4 * q, 0};  // ids inside the list: <= 0 past the run's end
-    if constexpr (RANGE) ch.y = __shfl(iy, kk, kWave);
+    WaveChunk ch{make_int4(-1, -1, -1, -1), kl - 4 * q, 0, kk};  // ids inside the list: <= 0 past the run's end
+    if constexpr (RANGE || PV) ch.y = __shfl(iy, kk, kWave);
     if (c < total) {
       const v4i_a4 tq = *reinterpret_cast<const gv4i_a4*>(reinterpret_cast<uint64_t>(ot + kb + 4 * (int64_t)q));
       ch.v = make_int4(tq.x, tq.y, tq.z, tq.w);
@@ -1372,7 +1400,85 @@ __global__ __launch_bounds__(kWaveThreads) __attribute__((amdgpu_waves_per_eu(8,
     owned += (uint64_t)__popcll(own);
     local += (uint64_t)__popcll(own & __builtin_amdgcn_ballot_w64(mid < (int32_t)rhi));
   };
+  uint32_t at_x = 0;  // PV: hits credited to the owner (per lane, summed when the owner ends)
+  // PV: one hit of w at table slot `slot`, for an item with partner y and index k.
+  auto pv_hit = [&](int32_t w, int slot, int32_t y, uint32_t& ny) {
+    if (w < min(cur_x, y))
+      atomicAdd(&wc[slot], 1u);
+    else if (cur_x < y)
+      at_x++;
+    else
+      ny++;
+  };
+  auto probe_pv = [&](const WaveChunk& ch) {
+    const int32_t w[4] = {ch.v.x, ch.v.y, ch.v.z, ch.v.w};
+    int4 tt[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) tt[j] = tab[wave_bucket(w[j])];
+    uint32_t ny = 0;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const bool e0 = tt[j].x == w[j], e1 = tt[j].y == w[j], e2 = tt[j].z == w[j], e3 = tt[j].w == w[j];
+      const bool h = j < ch.r && (e0 | e1 | e2 | e3);
+      hits += (uint64_t)__popcll(__builtin_amdgcn_ballot_w64(h));
+      if (h) pv_hit(w[j], (int)(wave_bucket(w[j]) << 2) + ((e1 | e3) ? 1 : 0) + ((e2 | e3) ? 2 : 0), ch.y, ny);
+    }
+    if (spl && __builtin_amdgcn_ballot_w64((tt[0].w | tt[1].w | tt[2].w | tt[3].w) < 0)) {
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        int slot = -1;
+        if (j < ch.r && tt[j].w < 0) {
+          if ((tt[j].w & 0x7fffffff) == w[j])
+            slot = (int)(wave_bucket(w[j]) << 2) + 3;  // slot 3 of a spilled bucket: the raw compare missed it
+          else if (bucket_miss(tt[j], w[j]) != 0)
+            slot = wave_tab_continue_slot(tab, wave_bucket(w[j]), w[j]);
+        }
+        hits += (uint64_t)__popcll(__builtin_amdgcn_ballot_w64(slot >= 0));
+        if (slot >= 0) pv_hit(w[j], slot, ch.y, ny);
+      }
+    }
+    if (ny) atomicAdd(&yc[ch.k], ny);
+  };
+  // PV: partner credits of the tile just probed (lane k: item k, partner iy).
+  auto flush_items = [&](int n, int32_t iy) {
+    if constexpr (PV) {
+      wave_lds_fence();
+      if (lane < n) {
+        const uint32_t c = yc[lane];
+        if (c) {
+          atomicAdd(&pv[iy], (unsigned long long)c);
+          yc[lane] = 0;
+        }
+      }
+      wave_lds_fence();
+    }
+  };
+  // PV: the table's slot credits and the owner's credits, before the table goes.
+  auto flush_owner = [&]() {
+    if constexpr (PV) {
+      wave_lds_fence();
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int sl = lane + kWave * i;
+        const uint32_t c = wc[sl];
+        if (c) {
+          const int4 t = tab[sl >> 2];
+          const int32_t key = (sl & 3) == 0 ? t.x : (sl & 3) == 1 ? t.y : (sl & 3) == 2 ? t.z : (t.w & 0x7fffffff);
+          atomicAdd(&pv[key], (unsigned long long)c);
+          wc[sl] = 0;
+        }
+      }
+      const uint64_t ax = wave_sum_u64((uint64_t)at_x);  // lane 0 holds the sum
+      if (lane == 0 && ax) atomicAdd(&pv[cur_x], (unsigned long long)ax);
+      at_x = 0;
+      wave_lds_fence();
+    }
+  };
   auto probe = [&](const WaveChunk& ch) {
+    if constexpr (PV) {
+      probe_pv(ch);
+      return;
+    }
     const int32_t w[4] = {ch.v.x, ch.v.y, ch.v.z, ch.v.w};
     int4 tt[4];
 #pragma unroll
@@ -1434,7 +1540,7 @@ __global__ __launch_bounds__(kWaveThreads) __attribute__((amdgpu_waves_per_eu(8,
     int32_t c0 = 0;
     for (;;) {
       const bool same = c0 + kWave < total_c;  // uniform: the tile has another round
-      WaveChunk nxt{make_int4(-1, -1, -1, -1), 0, 0};
+      WaveChunk nxt{make_int4(-1, -1, -1, -1), 0, 0, 0};
       int32_t pre_n = 0, total_n = 0;
       const bool fresh = !same && dn.valid && dn.a0 != table_a0;  // the next tile needs a new table
       if (same) {
@@ -1450,10 +1556,12 @@ __global__ __launch_bounds__(kWaveThreads) __attribute__((amdgpu_waves_per_eu(8,
         c0 += kWave;
         continue;
       }
+      flush_items(dc.n, iy_c);
       if (!dn.valid) break;  // uniform
       // Tile boundary: dn becomes the current tile.
       enter_graph(dn);
       if (fresh) {
+        flush_owner();
         build_table(dn, o0, o1);
         table_a0 = dn.a0;
       }
@@ -1469,6 +1577,7 @@ __global__ __launch_bounds__(kWaveThreads) __attribute__((amdgpu_waves_per_eu(8,
       load_items(dn, ib_n, il_n, iy_n);
       dnn = dn.valid ? load_desc(draw()) : none;
     }
+    flush_owner();
   }
   if (lane == 0 && hits) {
     if constexpr (BATCH)
@@ -1552,7 +1661,7 @@ int count_oriented(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne_bound, const 
   XT_TRY(ensure(ctx, s.items, (size_t)ne_bound * sizeof(Item)));
   XT_TRY(ensure(ctx, s.tiledesc, (size_t)tiles_bound(nv, ne_bound) * sizeof(TileDesc)));
   // Owners with out-lists of at most wown ids go to wave tiles (plain counts).
-  const int wown = pv ? 0 : wave_owner_cut(ctx, batch);
+  const int wown = wave_owner_cut(ctx, batch);
   XT_TRY(ensure(ctx, s.deg, (size_t)(wown ? 2 : 1) * (size_t)nv * sizeof(uint32_t)));
   XT_TRY(ensure(ctx, s.tileoff, (size_t)((wown ? 2 : 1) * nv + 1) * sizeof(int64_t)));
   uint64_t* counters = s.counters.as<uint64_t>();
@@ -1653,11 +1762,16 @@ int count_oriented(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne_bound, const 
     const int64_t* tof = s.tileoff.as<int64_t>();
     const unsigned wblocks = (unsigned)(ctx->num_cus * kWaveBlocksPerCU);
 #define XT_WAVES(C, B)                                                                                    \
-  hipLaunchKernelGGL((k_count_waves<C, B>), dim3(wblocks), dim3(kWaveThreads), 0, st, ot,                      \
+  hipLaunchKernelGGL((k_count_waves<C, B, C == kCreditVertex ? 5 : C == kCreditRange ? 7 : 8>), dim3(wblocks),       \
+                     dim3(kWaveThreads), 0, st, ot,                                                             \
                      (const Item*)s.items.as<Item>(), (const TileDesc*)s.tiledesc.as<TileDesc>(), tof + nv,     \
-                     tof + 2 * nv, counters, gid, gt, (uint32_t)ctx->tile_run, rlo, rhi)
+                     tof + 2 * nv, counters, gid, gt, (uint32_t)ctx->tile_run, rlo, rhi, pvp)
     if (range)
       XT_WAVES(kCreditRange, false);
+    else if (pv && batch)
+      XT_WAVES(kCreditVertex, true);
+    else if (pv)
+      XT_WAVES(kCreditVertex, false);
     else if (batch)
       XT_WAVES(kCreditTotal, true);
     else
