@@ -90,10 +90,11 @@ int xt_ctx_tune(xt_ctx* ctx, int light_cut, int group_blocks, int group_round);
 int xt_ctx_tune_tiles(xt_ctx* ctx, int items_per_tile, int tile_run);
 /* Wave tiles for plain counts (0 keeps the current value, -1 restores the
  * defaults, -2 turns them off): owners whose out-list holds at most wave_owner
- * ids (1..128) are counted one wave per tile of at most 64 items, with a
- * 1 KB LDS table per wave, instead of by the 512-thread group kernel (default
- * 128 for one graph, 64 for batches and the server).  Per-vertex and
- * owned-range counts always use the group kernel.  Does not change a result.
+ * ids (1..512) are counted one wave per tile of at most 64 items, with an
+ * LDS table per wave of 1 KB (up to 128 ids), 2 KB (256) or 4 KB (512),
+ * instead of by the 512-thread group kernel (default 256 for one graph, 64
+ * for batches and the server).  Plain, per-vertex and owned-range counts all
+ * use them.  Does not change a result.
  * No counterpart in the reference, whose launch shapes are compile-time
  * macros (counting.cuh:150-176). */
 int xt_ctx_tune_waves(xt_ctx* ctx, int wave_owner);

@@ -23,7 +23,7 @@ def _restore(ctx):
     ctx.tune(small_cut=-1, group_blocks=4, group_round=2, items_per_tile=-1, tile_run=-1, wave_owner=-1)
 
 
-@pytest.mark.parametrize("wave_owner", [1, 16, 64, 128])
+@pytest.mark.parametrize("wave_owner", [1, 16, 64, 128, 256, 512])
 @pytest.mark.parametrize("small_cut", [1, 32])
 def test_wave_tiles_match_oracle(ctx, golden, wave_owner, small_cut):
     """R-MAT s8-s16 and RGG fixtures: plain counts with wave tiles equal the
@@ -63,21 +63,23 @@ def test_wave_tiles_tile_runs_and_grid(ctx, run):
 
 
 def _colliding_ids(count, start, bucket):
-    """count ids >= start whose wave-table home bucket (top 6 bits of
-    id * 0x9E3779B1 mod 2^32) is `bucket`."""
-    ids = np.arange(start, start + 200000, dtype=np.uint64)
-    home = ((ids * np.uint64(0x9E3779B1)) & np.uint64(0xFFFFFFFF)) >> np.uint64(26)
+    """count ids >= start whose home bucket is `bucket` in every wave table
+    (top 6, 7 and 8 bits of id * 0x9E3779B1 mod 2^32; bucket < 64 is the
+    6-bit index, so the 8-bit one is 4 * bucket)."""
+    ids = np.arange(start, start + 800000, dtype=np.uint64)
+    home = ((ids * np.uint64(0x9E3779B1)) & np.uint64(0xFFFFFFFF)) >> np.uint64(24)
+    bucket = 4 * bucket
     sel = ids[home == bucket][:count]
     assert len(sel) == count
     return [int(v) for v in sel]
 
 
-@pytest.mark.parametrize("k", [6, 20, 60])
-def test_wave_table_spills(ctx, k):
+@pytest.mark.parametrize("k,wave_owner", [(6, 128), (20, 128), (60, 128), (20, 256), (150, 256), (300, 512)])
+def test_wave_table_spills(ctx, k, wave_owner):
     """A clique on k+1 vertices whose ids all share one home bucket of the wave
     table: buckets spill, keys sit in marked slots and further buckets, and the
     count is still C(k+1, 3); the same graph with shuffled labels gives the same."""
-    ids = [0] + _colliding_ids(k, 1000, 17)
+    ids = [0] + _colliding_ids(k, 1000, 17 if wave_owner <= 128 else 40)  # in every table size
     n = max(ids) + 1
     edges = [(a, b) for i, a in enumerate(ids) for b in ids[i + 1:]]
     off, tails = csr_from_edges(n, edges)
@@ -86,10 +88,10 @@ def test_wave_table_spills(ctx, k):
     exp_pv = np.zeros(n, np.uint64)
     for j, v in enumerate(ids):
         exp_pv[v] = (k - j) * (k - j - 1) // 2
-    ctx.tune(small_cut=1, wave_owner=128)
+    ctx.tune(small_cut=1, wave_owner=wave_owner)
     try:
         for orient in ("degree", "id"):
-            _say("spill clique", k, orient)
+            _say("spill clique", k, wave_owner, orient)
             assert ctx.count_csr(off, tails, orient=orient) == exp
             tot, pv = ctx.count_csr(off, tails, orient=orient, per_vertex=True)
             assert tot == exp and (pv == exp_pv).all()
@@ -131,7 +133,7 @@ def test_wave_tiles_wide_layout(ctx):
 
 
 def test_wave_knob_rejects_bad_values(ctx):
-    for w in (129, -3):
+    for w in (513, -3):
         with pytest.raises(xxtric_amd.XtError) as e:
             ctx.tune(wave_owner=w)
         assert e.value.code == -1
@@ -163,7 +165,7 @@ def test_batch_wave_settings_agree(ctx):
 
 
 
-@pytest.mark.parametrize("wave_owner", [-2, 16, 128])
+@pytest.mark.parametrize("wave_owner", [-2, 16, 128, 256, 512])
 @pytest.mark.parametrize("small_cut", [1, 32])
 def test_wave_tiles_owned_ranges(ctx, wave_owner, small_cut):
     """Owned-range counts with wave tiles (minimum and middle vertex split by
@@ -191,7 +193,7 @@ def test_wave_tiles_owned_ranges(ctx, wave_owner, small_cut):
         g.close()
 
 
-@pytest.mark.parametrize("wave_owner", [1, 16, 64, 128])
+@pytest.mark.parametrize("wave_owner", [1, 16, 64, 128, 256, 512])
 def test_wave_tiles_per_vertex(ctx, golden, wave_owner):
     """Per-vertex counts with wave tiles (slot, partner and owner credits kept in
     the wave's LDS and added when the owner or tile ends) equal the oracle's,

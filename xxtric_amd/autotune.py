@@ -40,13 +40,13 @@ import numpy as np
 DEFAULT_CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
 SPACE = {
     "light_cut": (4, 8, 16, 24, 32, 48, 64, 96),
-    "wave_owner": (-2, 32, 64, 128),
+    "wave_owner": (-2, 64, 128, 256, 512),
     "group_round": (1, 2, 4),
     "group_blocks": (2, 3, 4),
     "items_per_tile": (64, 128, 256),
     "tile_run": (4, 8, 16, 32, 64),
 }
-DEFAULTS = {"light_cut": 32, "wave_owner": 128, "group_round": 2, "group_blocks": 4, "items_per_tile": 256,
+DEFAULTS = {"light_cut": 32, "wave_owner": 256, "group_round": 2, "group_blocks": 4, "items_per_tile": 256,
             "tile_run": 16}
 BATCH_DEFAULTS = dict(DEFAULTS, light_cut=8, wave_owner=64)
 

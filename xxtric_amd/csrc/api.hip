@@ -354,8 +354,8 @@ int xt_ctx_tune_tiles(xt_ctx* ctx, int items_per_tile, int tile_run) {
 int xt_ctx_tune_waves(xt_ctx* ctx, int wave_owner) {
   return xt_guarded([&]() -> int {
     if (!ctx) return XT_ERR_ARG;
-    if (wave_owner < -2 || wave_owner > 128)
-      return set_err(ctx, XT_ERR_ARG, "wave_owner must be in 1..128 (0 keeps, -1 restores the defaults, -2 turns wave tiles off)");
+    if (wave_owner < -2 || wave_owner > 512)
+      return set_err(ctx, XT_ERR_ARG, "wave_owner must be in 1..512 (0 keeps, -1 restores the defaults, -2 turns wave tiles off)");
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     if (wave_owner > 0) ctx->wave_owner = wave_owner;
     if (wave_owner == -1) ctx->wave_owner = -1;

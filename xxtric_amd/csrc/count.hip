@@ -1164,7 +1164,8 @@ __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(W
 // ---------------------------------------------------------------------------
 // Wave tiles (plain counts).  An owner x with a short out-list gets one wave,
 // not a 512-thread workgroup with a 2048-bucket table: the wave hashes N+(x)
-// (at most kWaveOwnerMax ids) into its own 64-bucket LDS table (1 KB) and
+// (at most kWaveOwnerMax ids) into its own LDS table of 64, 128 or 256
+// buckets (1, 2 or 4 KB: at most half its slots filled) and
 // streams its tile's items -- at most 64, one per lane -- through it.  The
 // items' lists are concatenated into 16-byte chunks and dealt out 64 at a
 // time, one per lane, so lanes never wait on a longer list than their own
@@ -1175,21 +1176,22 @@ __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(W
 // the items starting at or before the round's first chunk are counted with
 // one ballot, those starting inside the round are bits of a 64-bit mask
 // OR-ed across the wave, and each lane adds the bits up to its own position.
-constexpr int kWaveBuckets = 64;  // per-wave LDS hash: 64 buckets of four ids
-constexpr int kWaveOwnerMax = 2 * kWaveBuckets;  // owner lists fill at most half the slots
+constexpr int kWaveLogMax = 8;  // per-wave LDS hash: up to 256 buckets of four ids
+constexpr int kWaveOwnerMax = 2 << kWaveLogMax;  // owner lists fill at most half the slots
 constexpr int kWaveThreads = 256;  // four independent waves per workgroup
 
-// Home bucket of w: a multiplicative hash, so ids sharing their low bits
-// (strided or clustered labels) still spread.
+// Home bucket of w among 2^LOGB: a multiplicative hash, so ids sharing their
+// low bits (strided or clustered labels) still spread.
+template <int LOGB>
 __device__ __forceinline__ uint32_t wave_bucket(int32_t w) {
-  return ((uint32_t)w * 0x9E3779B1u) >> (32 - 6);
+  return ((uint32_t)w * 0x9E3779B1u) >> (32 - LOGB);
 }
-static_assert(kWaveBuckets == 64, "wave_bucket yields 6 bits");
 
 // Inserts key; returns whether a full bucket had to be passed (and was
 // marked spilled, as in tab_insert).
+template <int LOGB>
 __device__ __forceinline__ bool wave_tab_insert(int4* tab, int32_t key) {
-  uint32_t b = wave_bucket(key);
+  uint32_t b = wave_bucket<LOGB>(key);
   bool passed = false;
   for (;;) {
     int* s = reinterpret_cast<int*>(tab + b);
@@ -1199,15 +1201,16 @@ __device__ __forceinline__ bool wave_tab_insert(int4* tab, int32_t key) {
     if (atomicCAS(s + 3, kEmpty, key) == kEmpty) return passed;
     atomicOr(s + 3, (int)0x80000000);
     passed = true;
-    b = (b + 1) & (kWaveBuckets - 1);
+    b = (b + 1) & ((1u << LOGB) - 1);
   }
 }
 
 // Continues a lookup past the spilled home bucket b; ends because at most half
 // the slots are used, so some bucket is not full and therefore not spilled.
+template <int LOGB>
 __device__ __forceinline__ bool wave_tab_continue(const int4* tab, uint32_t b, int32_t w) {
   for (;;) {
-    b = (b + 1) & (kWaveBuckets - 1);
+    b = (b + 1) & ((1u << LOGB) - 1);
     const int4 t = tab[b];
     if (bucket_miss(t, w) == 0) return true;
     if (!bucket_spilled(t)) return false;
@@ -1215,9 +1218,10 @@ __device__ __forceinline__ bool wave_tab_continue(const int4* tab, uint32_t b, i
 }
 
 // The same, returning the slot index 4*bucket + s of w, or -1.
+template <int LOGB>
 __device__ __forceinline__ int wave_tab_continue_slot(const int4* tab, uint32_t b, int32_t w) {
   for (;;) {
-    b = (b + 1) & (kWaveBuckets - 1);
+    b = (b + 1) & ((1u << LOGB) - 1);
     const int4 t = tab[b];
     const int sl = bucket_slot(t, b, w);
     if (sl >= 0) return sl;
@@ -1262,7 +1266,7 @@ struct WaveChunk {
 // table id w through a 32-bit counter per table slot, kept while the owner
 // stays; to the partner y through a counter per item of the tile; to the
 // owner x in registers -- all added to HBM when the owner or tile ends).
-template <int CM, bool BATCH, int WPE>
+template <int CM, bool BATCH, int WPE, int LOGB>
 __global__ __launch_bounds__(kWaveThreads) __attribute__((amdgpu_waves_per_eu(WPE, 8))) void k_count_waves(
     const int32_t* __restrict__ ot, const Item* __restrict__ items, const TileDesc* __restrict__ tdesc,
     const int64_t* __restrict__ t_begin, const int64_t* __restrict__ t_end,
@@ -1272,8 +1276,11 @@ __global__ __launch_bounds__(kWaveThreads) __attribute__((amdgpu_waves_per_eu(WP
   constexpr bool RANGE = CM == kCreditRange;
   constexpr bool PV = CM == kCreditVertex;
   constexpr int kWaves = kWaveThreads / kWave;
-  __shared__ int4 s_tab[kWaves][kWaveBuckets];
-  __shared__ uint32_t s_wc[PV ? kWaves : 1][PV ? 4 * kWaveBuckets : 1];  // per table slot
+  constexpr int kB = 1 << LOGB;           // buckets of the wave's table
+  constexpr int kOwnIds = 2 * kB / kWave;  // owner ids per lane (owner lists of at most 2 kB ids)
+  static_assert(LOGB >= 6 && LOGB <= kWaveLogMax, "64 to 256 buckets");
+  __shared__ int4 s_tab[kWaves][kB];
+  __shared__ uint32_t s_wc[PV ? kWaves : 1][PV ? 4 * kB : 1];  // per table slot
   __shared__ uint32_t s_yc[PV ? kWaves : 1][PV ? kWaveItems : 1];        // per item of the tile
   const int lane = threadIdx.x & (kWave - 1);
   const int wv = threadIdx.x / kWave;
@@ -1282,7 +1289,7 @@ __global__ __launch_bounds__(kWaveThreads) __attribute__((amdgpu_waves_per_eu(WP
   uint32_t* yc = s_yc[PV ? wv : 0];
   if constexpr (PV) {
 #pragma unroll
-    for (int i = 0; i < 4; i++) wc[lane + kWave * i] = 0;
+    for (int i = 0; i < 4 * kB / kWave; i++) wc[lane + kWave * i] = 0;
     yc[lane] = 0;
   }
   const int64_t tb = *t_begin;
@@ -1374,18 +1381,20 @@ __global__ __launch_bounds__(kWaveThreads) __attribute__((amdgpu_waves_per_eu(WP
     }
     return ch;
   };
-  // An owner list of at most kWaveOwnerMax ids: two ids per lane.
-  auto load_owner = [&](const WaveTile& d, int32_t& o0, int32_t& o1) {
-    o0 = lane < d.dx ? ot[d.a0 + lane] : 0;
-    o1 = lane + kWave < d.dx ? ot[d.a0 + kWave + lane] : 0;
+  // An owner list of at most 2 kB ids: kOwnIds ids per lane.
+  auto load_owner = [&](const WaveTile& d, int32_t (&o)[kOwnIds]) {
+#pragma unroll
+    for (int i = 0; i < kOwnIds; i++) o[i] = lane + kWave * i < d.dx ? ot[d.a0 + kWave * i + lane] : 0;
   };
   bool spl = false;  // some bucket of the table spilled (uniform)
-  auto build_table = [&](const WaveTile& d, int32_t o0, int32_t o1) {
-    tab[lane] = make_int4(kEmpty, kEmpty, kEmpty, kEmpty);
+  auto build_table = [&](const WaveTile& d, const int32_t (&o)[kOwnIds]) {
+#pragma unroll
+    for (int i = 0; i < kB / kWave; i++) tab[lane + kWave * i] = make_int4(kEmpty, kEmpty, kEmpty, kEmpty);
     wave_lds_fence();
     bool passed = false;
-    if (lane < d.dx) passed |= wave_tab_insert(tab, o0);
-    if (lane + kWave < d.dx) passed |= wave_tab_insert(tab, o1);
+#pragma unroll
+    for (int i = 0; i < kOwnIds; i++)
+      if (lane + kWave * i < d.dx) passed |= wave_tab_insert<LOGB>(tab, o[i]);
     spl = __builtin_amdgcn_ballot_w64(passed) != 0;
     wave_lds_fence();
   };
@@ -1414,14 +1423,14 @@ __global__ __launch_bounds__(kWaveThreads) __attribute__((amdgpu_waves_per_eu(WP
     const int32_t w[4] = {ch.v.x, ch.v.y, ch.v.z, ch.v.w};
     int4 tt[4];
 #pragma unroll
-    for (int j = 0; j < 4; j++) tt[j] = tab[wave_bucket(w[j])];
+    for (int j = 0; j < 4; j++) tt[j] = tab[wave_bucket<LOGB>(w[j])];
     uint32_t ny = 0;
 #pragma unroll
     for (int j = 0; j < 4; j++) {
       const bool e0 = tt[j].x == w[j], e1 = tt[j].y == w[j], e2 = tt[j].z == w[j], e3 = tt[j].w == w[j];
       const bool h = j < ch.r && (e0 | e1 | e2 | e3);
       hits += (uint64_t)__popcll(__builtin_amdgcn_ballot_w64(h));
-      if (h) pv_hit(w[j], (int)(wave_bucket(w[j]) << 2) + ((e1 | e3) ? 1 : 0) + ((e2 | e3) ? 2 : 0), ch.y, ny);
+      if (h) pv_hit(w[j], (int)(wave_bucket<LOGB>(w[j]) << 2) + ((e1 | e3) ? 1 : 0) + ((e2 | e3) ? 2 : 0), ch.y, ny);
     }
     if (spl && __builtin_amdgcn_ballot_w64((tt[0].w | tt[1].w | tt[2].w | tt[3].w) < 0)) {
 #pragma unroll
@@ -1429,9 +1438,9 @@ __global__ __launch_bounds__(kWaveThreads) __attribute__((amdgpu_waves_per_eu(WP
         int slot = -1;
         if (j < ch.r && tt[j].w < 0) {
           if ((tt[j].w & 0x7fffffff) == w[j])
-            slot = (int)(wave_bucket(w[j]) << 2) + 3;  // slot 3 of a spilled bucket: the raw compare missed it
+            slot = (int)(wave_bucket<LOGB>(w[j]) << 2) + 3;  // slot 3 of a spilled bucket: the raw compare missed it
           else if (bucket_miss(tt[j], w[j]) != 0)
-            slot = wave_tab_continue_slot(tab, wave_bucket(w[j]), w[j]);
+            slot = wave_tab_continue_slot<LOGB>(tab, wave_bucket<LOGB>(w[j]), w[j]);
         }
         hits += (uint64_t)__popcll(__builtin_amdgcn_ballot_w64(slot >= 0));
         if (slot >= 0) pv_hit(w[j], slot, ch.y, ny);
@@ -1458,7 +1467,7 @@ __global__ __launch_bounds__(kWaveThreads) __attribute__((amdgpu_waves_per_eu(WP
     if constexpr (PV) {
       wave_lds_fence();
 #pragma unroll
-      for (int i = 0; i < 4; i++) {
+      for (int i = 0; i < 4 * kB / kWave; i++) {
         const int sl = lane + kWave * i;
         const uint32_t c = wc[sl];
         if (c) {
@@ -1482,7 +1491,7 @@ __global__ __launch_bounds__(kWaveThreads) __attribute__((amdgpu_waves_per_eu(WP
     const int32_t w[4] = {ch.v.x, ch.v.y, ch.v.z, ch.v.w};
     int4 tt[4];
 #pragma unroll
-    for (int j = 0; j < 4; j++) tt[j] = tab[wave_bucket(w[j])];
+    for (int j = 0; j < 4; j++) tt[j] = tab[wave_bucket<LOGB>(w[j])];
 #pragma unroll
     for (int j = 0; j < 4; j++) {
       const uint64_t hit = (__builtin_amdgcn_ballot_w64(tt[j].x == w[j]) | __builtin_amdgcn_ballot_w64(tt[j].y == w[j]) |
@@ -1499,7 +1508,7 @@ __global__ __launch_bounds__(kWaveThreads) __attribute__((amdgpu_waves_per_eu(WP
         bool h = false;
         if (j < ch.r && tt[j].w < 0) {
           h = (tt[j].w & 0x7fffffff) == w[j];
-          if (!h && bucket_miss(tt[j], w[j]) != 0) h = wave_tab_continue(tab, wave_bucket(w[j]), w[j]);
+          if (!h && bucket_miss(tt[j], w[j]) != 0) h = wave_tab_continue<LOGB>(tab, wave_bucket<LOGB>(w[j]), w[j]);
         }
         const uint64_t hm = __builtin_amdgcn_ballot_w64(h);
         hits += (uint64_t)__popcll(hm);
@@ -1524,14 +1533,14 @@ __global__ __launch_bounds__(kWaveThreads) __attribute__((amdgpu_waves_per_eu(WP
   WaveTile dc = load_desc(draw());
   if (dc.valid) {
     int64_t ib_c, ib_n;
-    int32_t il_c, il_n, iy_c, iy_n, o0, o1;
+    int32_t il_c, il_n, iy_c, iy_n, own[kOwnIds];
     load_items(dc, ib_c, il_c, iy_c);
-    load_owner(dc, o0, o1);
+    load_owner(dc, own);
     WaveTile dn = load_desc(draw());
     load_items(dn, ib_n, il_n, iy_n);
     WaveTile dnn = dn.valid ? load_desc(draw()) : none;
     enter_graph(dc);
-    build_table(dc, o0, o1);
+    build_table(dc, own);
     int64_t table_a0 = dc.a0;
     cur_x = dc.x;
     int32_t pre_c, total_c;
@@ -1548,7 +1557,7 @@ __global__ __launch_bounds__(kWaveThreads) __attribute__((amdgpu_waves_per_eu(WP
       } else if (dn.valid) {
         chunk_starts(il_n, pre_n, total_n);
         nxt = load_round(ib_n, il_n, iy_n, pre_n, dn.n, total_n, 0);
-        if (fresh) load_owner(dn, o0, o1);
+        if (fresh) load_owner(dn, own);
       }
       probe(cur);
       cur = nxt;
@@ -1562,7 +1571,7 @@ __global__ __launch_bounds__(kWaveThreads) __attribute__((amdgpu_waves_per_eu(WP
       enter_graph(dn);
       if (fresh) {
         flush_owner();
-        build_table(dn, o0, o1);
+        build_table(dn, own);
         table_a0 = dn.a0;
       }
       dc = dn;
@@ -1761,11 +1770,20 @@ int count_oriented(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne_bound, const 
   if (wown) {
     const int64_t* tof = s.tileoff.as<int64_t>();
     const unsigned wblocks = (unsigned)(ctx->num_cus * kWaveBlocksPerCU);
-#define XT_WAVES(C, B)                                                                                    \
-  hipLaunchKernelGGL((k_count_waves<C, B, C == kCreditVertex ? 5 : C == kCreditRange ? 7 : 8>), dim3(wblocks),       \
+#define XT_WAVES_L(C, B, L)                                                                               \
+  hipLaunchKernelGGL((k_count_waves<C, B, C == kCreditVertex ? 5 : C == kCreditRange ? 7 : 8, L>), dim3(wblocks),    \
                      dim3(kWaveThreads), 0, st, ot,                                                             \
                      (const Item*)s.items.as<Item>(), (const TileDesc*)s.tiledesc.as<TileDesc>(), tof + nv,     \
                      tof + 2 * nv, counters, gid, gt, (uint32_t)ctx->tile_run, rlo, rhi, pvp)
+#define XT_WAVES(C, B)            \
+  do {                            \
+    if (wown <= 128)              \
+      XT_WAVES_L(C, B, 6);        \
+    else if (wown <= 256)         \
+      XT_WAVES_L(C, B, 7);        \
+    else                          \
+      XT_WAVES_L(C, B, 8);        \
+  } while (0)
     if (range)
       XT_WAVES(kCreditRange, false);
     else if (pv && batch)
@@ -1777,6 +1795,7 @@ int count_oriented(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne_bound, const 
     else
       XT_WAVES(kCreditTotal, false);
 #undef XT_WAVES
+#undef XT_WAVES_L
     XT_HIP(ctx, hipGetLastError());
   }
   return XT_OK;

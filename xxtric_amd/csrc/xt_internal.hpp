@@ -26,7 +26,7 @@ constexpr int kBatchLightCut = 8;   // default light cut, batches and the server
 constexpr int kOffCap = 2048;
 // Wave tiles (k_count_waves): owners whose out-list has at most this many ids
 // are counted one wave per tile; 0 turns them off.
-constexpr int kWaveOwner = 128;      // default, one graph
+constexpr int kWaveOwner = 256;      // default, one graph
 constexpr int kBatchWaveOwner = 64;  // default, batches and the server
 constexpr int kWaveBlocksPerCU = 8;  // persistent 256-thread workgroups per CU
 
