@@ -65,7 +65,8 @@ def rmat_workload(ctx, scale, steps, warmup, seed=1, tune=True):
     med = lambda k: float(np.median([e[k] for e in ev]))  # noqa: E731
     st = g.stats()
     out = {"triangles": tri, "nv": g.nv, "m": m, "ms_per_step": ms_step, "gen_s": gen_s,
-           "n_heavy": ev[-1]["n_heavy"], "n_tiles": ev[-1]["n_tiles"], "tuning": tuning}
+           "n_heavy": ev[-1]["n_heavy"], "n_tiles": ev[-1]["n_tiles"], "n_wave_tiles": ev[-1]["n_wave_tiles"],
+           "tuning": tuning}
     for k in ("orient_ms", "count_ms", "total_ms", "light_ms", "items_ms", "group_ms"):
         out[k] = med(k)
     out.update(st)
@@ -484,7 +485,7 @@ def main():
         "teps_reference_style": round(2 * r["m"] / (r["ms_per_step"] * 1e-3), 1),
         "edges_per_s_device": round(r["m"] / (r["total_ms"] * 1e-3), 1),
         "ne_plus": r["ne_plus"], "sum_min": r["sum_min"], "heavy_edges": r["heavy_edges"],
-        "sum_min_heavy": r["sum_min_heavy"], "group_tiles": r["n_tiles"],
+        "sum_min_heavy": r["sum_min_heavy"], "group_tiles": r["n_tiles"], "wave_tiles": r["n_wave_tiles"],
         "tuning": r["tuning"],
     }
     line["cpu_baseline"] = cpu_baseline(g, args.cpu_seconds)

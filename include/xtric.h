@@ -62,6 +62,7 @@ typedef struct xt_timing {
   int64_t ne_plus;    /* oriented edges |E+| */
   int64_t n_heavy;    /* oriented edges intersected by the group kernel */
   int64_t n_tiles;    /* group tiles drawn from the device work queue */
+  int64_t n_wave_tiles; /* wave tiles (owners with short out-lists, k_count_waves) */
 } xt_timing;
 
 const char* xt_version(void);
@@ -74,7 +75,8 @@ int xt_last_timing(const xt_ctx* ctx, xt_timing* out);
 /* Tuning knobs (set by xxtric_amd.autotune from measurements; 0 keeps the
  * current value): light_cut -- edges whose two out-lists are both at most this
  * long are intersected one per thread, the rest by the group kernel or wave
- * tiles (default 32 for one graph, 8 for batches and the server; a value set here applies
+ * tiles (default 32 for one graph; for batches and the server 8 from 2^22
+ * entries on, where wave tiles take the rest, and 64 below; a value set here applies
  * to both, -1 restores the defaults); group_blocks -- persistent group-kernel workgroups per CU (1..4,
  * default 4); group_round -- 16-byte list chunks each group-kernel thread has
  * in flight per streaming round (1, 2 or 4, default 2).  None of them changes
@@ -92,9 +94,11 @@ int xt_ctx_tune_tiles(xt_ctx* ctx, int items_per_tile, int tile_run);
  * defaults, -2 turns them off): owners whose out-list holds at most wave_owner
  * ids (1..512) are counted one wave per tile of at most 64 items, with an
  * LDS table per wave of 1 KB (up to 128 ids), 2 KB (256) or 4 KB (512),
- * instead of by the 512-thread group kernel (default 256 for one graph, 64
- * for batches and the server).  Plain, per-vertex and owned-range counts all
- * use them.  Does not change a result.
+ * instead of by the 512-thread group kernel (default 256 for one graph of
+ * 2^28 or more entries, 64 for batches and the server from 2^22 entries on,
+ * none for smaller inputs, where the extra launch costs more than it saves; a
+ * value set here applies at every size).  Plain, per-vertex and owned-range
+ * counts all use them.  Does not change a result.
  * No counterpart in the reference, whose launch shapes are compile-time
  * macros (counting.cuh:150-176). */
 int xt_ctx_tune_waves(xt_ctx* ctx, int wave_owner);

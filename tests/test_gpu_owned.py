@@ -66,7 +66,7 @@ def test_owned_range_on_both_kernels(ctx, small_cut):
     _, pv = O.count_reference(off, tails, per_vertex=True)
     us, vs, cs = min_mid_pairs(off, tails)
     g = ctx.graph(off, tails)
-    ctx.tune(small_cut=small_cut)
+    ctx.tune(small_cut=small_cut, wave_owner=-2)  # heavy edges through the group kernel
     try:
         for lo, hi in [(0, n // 8), (n // 3, n // 2), (n - 100, n)]:
             _say("cut", small_cut, lo, hi)
@@ -74,7 +74,7 @@ def test_owned_range_on_both_kernels(ctx, small_cut):
             assert owned == int(pv[lo:hi].sum())
             assert local == int(cs[(us >= lo) & (us < hi) & (vs < hi)].sum())
     finally:
-        ctx.tune(small_cut=xxtric_amd.DEFAULT_LIGHT_CUT, group_blocks=4, group_round=2)
+        ctx.tune(small_cut=xxtric_amd.DEFAULT_LIGHT_CUT, group_blocks=4, group_round=2, wave_owner=-1)
 
 
 def test_slice_has_the_rank_view(ctx):

@@ -129,7 +129,7 @@ def test_star_hub(ctx):
 def test_kernel_split_does_not_change_counts(ctx, small_cut):
     off, tails = O.rmat_csr(14, 16, 1, True)
     exp, exp_pv = O.count_degree_ordered(off, tails, per_vertex=True)
-    ctx.tune(small_cut=small_cut)
+    ctx.tune(small_cut=small_cut, wave_owner=-2)  # heavy edges through the group kernel
     try:
         _say("small_cut", small_cut)
         tot, pv = ctx.count_csr(off, tails.astype(np.int64), per_vertex=True)
@@ -145,7 +145,7 @@ def test_kernel_split_does_not_change_counts(ctx, small_cut):
             tot, pv = ctx.count_csr(off, tails.astype(np.int64), per_vertex=True)
             assert tot == exp and (pv == exp_pv).all()
     finally:
-        ctx.tune(small_cut=xxtric_amd.DEFAULT_LIGHT_CUT, group_blocks=4, group_round=2)
+        ctx.tune(small_cut=xxtric_amd.DEFAULT_LIGHT_CUT, group_blocks=4, group_round=2, wave_owner=-1)
 
 
 @pytest.mark.parametrize("ipt,run", [(64, 1), (128, 64), (256, 7), (64, 16)])
@@ -491,7 +491,7 @@ def test_wide_layout_gives_the_same_counts(ctx, small_cut):
     hub_o, hub_t = wheel(70000)  # one vertex of degree 69999: its saturated key ties and breaks by id
     hub_exp = O.count_degree_ordered(hub_o, hub_t)
     hub_exp = hub_exp[0] if isinstance(hub_exp, tuple) else hub_exp
-    ctx.tune(small_cut=small_cut)
+    ctx.tune(small_cut=small_cut, wave_owner=-2)  # the group kernel's wide layout (wave tiles: test_gpu_waves)
     ctx.force_wide(True)
     try:
         _say("wide layout, cut", small_cut)
@@ -513,7 +513,7 @@ def test_wide_layout_gives_the_same_counts(ctx, small_cut):
         b.close()
     finally:
         ctx.force_wide(False)
-        ctx.tune(small_cut=xxtric_amd.DEFAULT_LIGHT_CUT, group_blocks=4, group_round=2)
+        ctx.tune(small_cut=xxtric_amd.DEFAULT_LIGHT_CUT, group_blocks=4, group_round=2, wave_owner=-1)
 
 
 def test_rgg_connectivity_radius_strict_1e5(ctx):

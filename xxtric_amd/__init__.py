@@ -17,7 +17,7 @@ from .capi import (XT_BATCH_HIPGRAPH, XT_BATCH_PER_GRAPH, XT_BATCH_PERSISTENT,  
                    XT_CHECK_SYMMETRY, XT_ORIENT_DEGREE, XT_ORIENT_ID, XT_PER_VERTEX,
                    XT_VALIDATE, XtError)
 
-DEFAULT_LIGHT_CUT = -1  # tune() value that restores the defaults: light = both out-lists <= 32 (8 in batches)
+DEFAULT_LIGHT_CUT = -1  # tune() value that restores the defaults: light = both out-lists <= 32 (8 in batches of 4M+ entries, else 64)
 
 __all__ = ["Context", "DeviceGraph", "TriangleCountingSingle", "rgg_generate", "rgg_radius", "rgg_radius_connect",
            "vite_read", "vite_write", "edgelist_to_csr", "batch_params", "batch_graph_csr",

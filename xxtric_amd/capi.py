@@ -57,7 +57,7 @@ _pp = ctypes.POINTER(ctypes.c_void_p)
 class XtTiming(ctypes.Structure):
     _fields_ = [("orient_ms", _dbl), ("count_ms", _dbl), ("total_ms", _dbl),
                 ("light_ms", _dbl), ("items_ms", _dbl), ("group_ms", _dbl),
-                ("ne_plus", _i64), ("n_heavy", _i64), ("n_tiles", _i64)]
+                ("ne_plus", _i64), ("n_heavy", _i64), ("n_tiles", _i64), ("n_wave_tiles", _i64)]
 
 
 class XtGraphStats(ctypes.Structure):

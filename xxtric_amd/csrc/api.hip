@@ -195,7 +195,7 @@ static int finish_count(xt_ctx* ctx, int64_t nv, uint32_t flags, hipStream_t st,
   XT_HIP(ctx, hipStreamSynchronize(st));
   if (h[kBadIds]) return set_err(ctx, XT_ERR_INPUT, "neighbour id out of range [0, nv)");
   if (total) *total = h[kTotal];
-  int64_t nep = 0, nheavy = 0, ntiles = 0;
+  int64_t nep = 0, nheavy = 0, ntiles = 0, nwave = 0;
   const bool worked = nv > 0 && ctx->s.itcnt.p != nullptr && ctx->s.ooff.p != nullptr;
   if (nv > 0)
     XT_HIP(ctx, hipMemcpy(&nep, ctx->s.ooff.as<int64_t>() + nv, sizeof(int64_t), hipMemcpyDeviceToHost));
@@ -203,11 +203,17 @@ static int finish_count(xt_ctx* ctx, int64_t nv, uint32_t flags, hipStream_t st,
   if (split) {
     nheavy = (int64_t)h[kHeavyItems];
     XT_HIP(ctx, hipMemcpy(&ntiles, ctx->s.tileoff.as<int64_t>() + nv, sizeof(int64_t), hipMemcpyDeviceToHost));
+    if (ctx->last_wave_owner > 0) {
+      int64_t wend = 0;
+      XT_HIP(ctx, hipMemcpy(&wend, ctx->s.tileoff.as<int64_t>() + 2 * nv, sizeof(int64_t), hipMemcpyDeviceToHost));
+      nwave = wend - ntiles;
+    }
   }
   ctx->timing = xt_timing{};
   ctx->timing.ne_plus = nep;
   ctx->timing.n_heavy = nheavy;
   ctx->timing.n_tiles = ntiles;
+  ctx->timing.n_wave_tiles = nwave;
   if (timed) {
     float a = 0, b = 0, c = 0;
     XT_HIP(ctx, hipEventElapsedTime(&a, ctx->ev[kEvStart], ctx->ev[kEvOriented]));

@@ -603,7 +603,9 @@ int xt_batch_stats_get(xt_batch* b, uint32_t flags, xt_graph_stats* out) {
     // The union's orientation is each graph's own (graphs share no edge).
     XT_TRY(fill_u64(ctx, b->s.counters.as<uint64_t>(), kNumSlots, 0, st));
     XT_TRY(orient<int32_t>(ctx, b->s, b->V, b->E, b->goff, b->gtails, mode, st));
-    XT_TRY(alg_stats(ctx, b->s, b->V, b->E, st, ctx->small_cut > 0 ? ctx->small_cut : kBatchLightCut));
+    XT_TRY(alg_stats(ctx, b->s, b->V, b->E, st,
+                     ctx->small_cut > 0 ? ctx->small_cut
+                     : b->E >= kBatchWaveMinEntries ? kBatchLightCut : kBatchLightCutSmall));
     uint64_t h[kNumSlots];
     int64_t nep = 0;
     XT_HIP(ctx, hipMemcpyAsync(h, b->s.counters.p, sizeof(h), hipMemcpyDeviceToHost, st));

@@ -1294,6 +1294,9 @@ __global__ __launch_bounds__(kWaveThreads) __attribute__((amdgpu_waves_per_eu(WP
   }
   const int64_t tb = *t_begin;
   const uint64_t nt = (uint64_t)(*t_end - tb);
+  // Waves beyond the number of tiles have nothing to draw and leave at once
+  // (each would otherwise add to all eight queue counters on its way out).
+  if ((uint64_t)blockIdx.x * (kWaveThreads / kWave) + threadIdx.x / kWave >= nt) return;  // uniform per wave
   // One queue per XCD over an eighth of the tiles (consecutive tiles: the same
   // owners, and in a batch the same graphs, so they stay in that XCD's L2);
   // a wave whose XCD's share is drawn goes on to the next XCD's queue.
@@ -1303,12 +1306,12 @@ __global__ __launch_bounds__(kWaveThreads) __attribute__((amdgpu_waves_per_eu(WP
   auto draw = [&]() -> uint64_t {
     while (run_next >= run_end) {
       if (qleft == 0) return nt;  // every queue drawn: no tile
+      const uint64_t qlo = nt * (uint64_t)q / kXcds, qhi = nt * (uint64_t)(q + 1) / kXcds;
       unsigned long long v = 0;
       if (lane == 0) v = atomicAdd(&queues[q], (unsigned long long)tile_run);
       const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
       const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
       const uint64_t got = ((uint64_t)hi << 32) | lo;
-      const uint64_t qlo = nt * (uint64_t)q / kXcds, qhi = nt * (uint64_t)(q + 1) / kXcds;
       if (qlo + got < qhi) {
         run_next = qlo + got;
         run_end = min(run_next + tile_run, qhi);
@@ -1646,8 +1649,18 @@ static int64_t tiles_bound(int64_t nv, int64_t ne) {
 }
 
 // Largest owner out-list counted in wave tiles, 0 for none.
-static int wave_owner_cut(const xt_ctx* ctx, bool batch) {
-  const int w = ctx->wave_owner == -1 ? (batch ? kBatchWaveOwner : kWaveOwner) : ctx->wave_owner;
+// By default wave tiles (and, in batches, the low light cut that sends most
+// edges to them) are used from kWaveMinEntries (one graph) or
+// kBatchWaveMinEntries (a batch) entries on: below that the extra launch and
+// the longer tile scan cost more than the small tiles do.  A value set with
+// xt_ctx_tune / xt_ctx_tune_waves applies at every size.
+static bool wave_size(int64_t ne, bool batch) {
+  return ne >= (batch ? kBatchWaveMinEntries : kWaveMinEntries);
+}
+static int wave_owner_cut(const xt_ctx* ctx, bool batch, int64_t ne) {
+  const int w = ctx->wave_owner != -1 ? ctx->wave_owner
+                : !wave_size(ne, batch) ? 0
+                : (batch ? kBatchWaveOwner : kWaveOwner);
   return w < 0 ? 0 : (w > kWaveOwnerMax ? kWaveOwnerMax : w);
 }
 
@@ -1670,7 +1683,8 @@ int count_oriented(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne_bound, const 
   XT_TRY(ensure(ctx, s.items, (size_t)ne_bound * sizeof(Item)));
   XT_TRY(ensure(ctx, s.tiledesc, (size_t)tiles_bound(nv, ne_bound) * sizeof(TileDesc)));
   // Owners with out-lists of at most wown ids go to wave tiles (plain counts).
-  const int wown = wave_owner_cut(ctx, batch);
+  const int wown = wave_owner_cut(ctx, batch, ne_bound);
+  ctx->last_wave_owner = wown;
   XT_TRY(ensure(ctx, s.deg, (size_t)(wown ? 2 : 1) * (size_t)nv * sizeof(uint32_t)));
   XT_TRY(ensure(ctx, s.tileoff, (size_t)((wown ? 2 : 1) * nv + 1) * sizeof(int64_t)));
   uint64_t* counters = s.counters.as<uint64_t>();
@@ -1682,7 +1696,10 @@ int count_oriented(xt_ctx* ctx, Scratch& s, int64_t nv, int64_t ne_bound, const 
   const uint8_t* oflags = s.oflags.as<uint8_t>();
   unsigned long long* pvp = (unsigned long long*)pv;
   unsigned long long* gt = (unsigned long long*)gtotal;
-  const int cut = ctx->small_cut > 0 ? ctx->small_cut : (gid ? kBatchLightCut : kLightCut);
+  const int cut = ctx->small_cut > 0 ? ctx->small_cut
+                  : !gid                   ? kLightCut
+                  : wave_size(ne_bound, true) ? kBatchLightCut
+                                              : kBatchLightCutSmall;
   int64_t g = (nv + 1 + 255) / 256;
   if (g > 65536) g = 65536;
   // every oriented offset and out-degree fits 32 bits (unless the wide layout is forced)

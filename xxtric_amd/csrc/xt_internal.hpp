@@ -22,13 +22,17 @@ constexpr int kChunk = kBlock * kPerThread;  // 2048
 // last chunk of the last list may read up to three ids past |E+|.
 constexpr int64_t kOtailsPad = 4;
 constexpr int kLightCut = 32;       // default light cut, one graph
-constexpr int kBatchLightCut = 8;   // default light cut, batches and the server
+constexpr int kBatchLightCut = 8;   // default light cut, batches and the server (with wave tiles)
+constexpr int kBatchLightCutSmall = 64;  // ... below kBatchWaveMinEntries (no wave tiles)
 constexpr int kOffCap = 2048;
 // Wave tiles (k_count_waves): owners whose out-list has at most this many ids
 // are counted one wave per tile; 0 turns them off.
 constexpr int kWaveOwner = 256;      // default, one graph
 constexpr int kBatchWaveOwner = 64;  // default, batches and the server
 constexpr int kWaveBlocksPerCU = 8;  // persistent 256-thread workgroups per CU
+// Default sizes (directed entries) from which wave tiles are used.
+constexpr int64_t kWaveMinEntries = int64_t(1) << 28;       // one graph
+constexpr int64_t kBatchWaveMinEntries = int64_t(1) << 22;  // a batch
 
 // Counter slots in Scratch::counters (uint64).
 enum Slot : int {
@@ -103,6 +107,7 @@ struct xt_ctx {
   int items_per_tile = 256;  // items of one group tile (64, 128 or 256)
   int tile_run = 16;         // consecutive tiles a group-kernel workgroup draws at once (1..64)
   int wave_owner = -1;       // wave tiles for owner lists up to this long; -1 default, 0 off
+  int last_wave_owner = 0;  // wave-owner cut of the last count (tiles [tileoff[nv], tileoff[2nv]) when > 0)
   bool force_wide = false;  // use the 64-bit offset layout of graphs with >= 2^32 entries everywhere
   int fail_allocs = 0;      // failure injection: scratch allocations left to fail
 };
