@@ -327,6 +327,35 @@ extern "C" uint64_t or_count_reference32(int64_t nv, const int64_t* off, const i
 // Reference count (xxtric.hpp:160-206) restricted to the listed vertices:
 // per_vertex[k] receives tcount of vertices[k].  Lets tests check per-vertex
 // counts of a sample of a graph too large for a full reference run.
+// Per-vertex count of listed vertices (smallest-vertex credit, xxtric.hpp:165)
+// by a different route from the reference loop, for vertices it is too slow
+// for (hubs: its pair loop over N(v) costs deg(v)^2 searches): mark N(v), then
+// for every neighbour u > v count the marked w > u in N(u) -- each triangle
+// v < u < w once, at the edge (v, u).  Rows must be ascending.  Work: the sum
+// of deg(u) over u in N(v); the loop over N(v) runs on nthreads threads.
+extern "C" void or_vertex_min_credit32(int64_t nv, const int64_t* off, const int32_t* tails,
+                                       const int64_t* vertices, int64_t nlist, uint64_t* out,
+                                       int nthreads) {
+  std::vector<uint8_t> mark((size_t)nv, 0);
+  if (nthreads < 1) nthreads = 1;
+  for (int64_t k = 0; k < nlist; k++) {
+    const int64_t v = vertices[k];
+    const int64_t e0 = off[v], e1 = off[v + 1];
+    for (int64_t e = e0; e < e1; e++) mark[(size_t)tails[e]] = 1;
+    uint64_t c = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : c) num_threads(nthreads)
+    for (int64_t e = e0; e < e1; e++) {
+      const int32_t u = tails[e];
+      if ((int64_t)u <= v) continue;
+      const int32_t* r0 = tails + off[u];
+      const int32_t* r1 = tails + off[u + 1];
+      for (const int32_t* q = std::upper_bound(r0, r1, u); q < r1; q++) c += mark[(size_t)*q];
+    }
+    for (int64_t e = e0; e < e1; e++) mark[(size_t)tails[e]] = 0;
+    out[k] = c;
+  }
+}
+
 extern "C" void or_count_reference32_list(int64_t nv, const int64_t* off, const int32_t* tails,
                                           const int64_t* vertices, int64_t nlist,
                                           uint64_t* per_vertex, int nthreads) {

@@ -15,7 +15,12 @@
  *                           scipy row sums of (U*U).*U for per-vertex counts.
  *   or_count_degree_ordered equality with or_count_reference (global + per-vertex)
  *                           and with the closed forms above.
+ *   or_vertex_min_credit32  equality with or_count_reference's per-vertex counts on
+ *                           every vertex of R-MAT, RGG and ER fixtures, and the K_n
+ *                           closed form C(n-1-i, 2).
  *   or_rgg_lcg              golden edge and triangle counts from the reference binary.
+ *   or_rgg_lcg_opts         strict test: a radius equal to one pair's distance drops
+ *                           that pair; both tests equal an all-pairs numpy enumeration.
  *   or_rgg_lcg_ranks        the CSR (sha256) the reference's generator builds on P
  *                           MPI ranks (oracle/_ref/ref_rgg_ranks under mpiexec); at
  *                           P = 1 equal to or_rgg_lcg.
@@ -80,6 +85,11 @@ uint64_t or_count_reference32(int64_t nv, const int64_t* off, const int32_t* tai
 
 /* or_count_reference32 restricted to a vertex list: per_vertex[k] = tcount of
  * vertices[k] (xxtric.hpp:160-206). */
+/* Per-vertex counts (smallest-vertex credit) of the listed vertices by marking
+ * N(v) and scanning N(u) above u for each neighbour u > v (rows ascending):
+ * the check for hubs, where the reference loop is too slow. */
+void or_vertex_min_credit32(int64_t nv, const int64_t* off, const int32_t* tails,
+                            const int64_t* vertices, int64_t nlist, uint64_t* out, int nthreads);
 void or_count_reference32_list(int64_t nv, const int64_t* off, const int32_t* tails,
                                const int64_t* vertices, int64_t nlist, uint64_t* per_vertex,
                                int nthreads);

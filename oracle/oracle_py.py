@@ -43,6 +43,7 @@ def lib():
             "or_count_degree_ordered": (_u64, [_i64, _p, _p, _p, _int]),
             "or_count_reference32_budget": (_i64, [_i64, _p, _p, _int, _dbl, _p, _p]),
             "or_count_reference32_list": (None, [_i64, _p, _p, _p, _i64, _p, _int]),
+            "or_vertex_min_credit32": (None, [_i64, _p, _p, _p, _i64, _p, _int]),
             "or_vite_write": (_int, [ctypes.c_char_p, _i64, _p, _p, _int]),
             "or_vite_read": (_int, [ctypes.c_char_p, _int, _p, _p, _p, _p]),
             "or_mix64": (_u64, [_u64]),
@@ -140,6 +141,18 @@ def count_reference_list(off, tails, vertices, nthreads=None):
     lib().or_count_reference32_list(len(off) - 1, _ptr(off), _ptr(tails), _ptr(vertices),
                                     len(vertices), _ptr(pv), nt)
     return pv
+
+
+def vertex_min_credit(off, tails, vertices, nthreads=None):
+    """Per-vertex counts (smallest-id credit) of the listed vertices by marking
+    N(v) and scanning N(u) above u for each neighbour u > v (rows ascending)."""
+    off = np.ascontiguousarray(off, np.int64)
+    tails = np.ascontiguousarray(tails, np.int32)
+    vertices = np.ascontiguousarray(vertices, np.int64)
+    out = np.zeros(len(vertices), np.uint64)
+    lib().or_vertex_min_credit32(len(off) - 1, _ptr(off), _ptr(tails), _ptr(vertices), len(vertices),
+                                 _ptr(out), nthreads or nthreads_default())
+    return out
 
 
 def count_reference_budget(off, tails, seconds, nthreads=None):

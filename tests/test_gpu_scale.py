@@ -3,8 +3,9 @@
 R-MAT scale 22 (the c1 workload): full global and per-vertex parity against
 the oracle's independent degree-ordered counter, and orientation invariance.
 R-MAT scale 26 (the benchmarked graph): size-independent properties --
-determinism, per-vertex counts summing to the total, and per-vertex counts of
-a sample of vertices equal to the reference algorithm run on those vertices.
+determinism, per-vertex counts summing to the total, per-vertex counts of a
+sample of vertices equal to the reference algorithm run on those vertices, and
+of the hubs and longest out-lists equal to the oracle's mark-based count.
 """
 import numpy as np
 import pytest
@@ -66,4 +67,21 @@ def test_rmat_s26_properties(ctx):
     _say("rmat s26 sampled reference", len(verts))
     ref = O.count_reference_list(off, tails, verts)
     assert (pv[verts] == ref).all() and ref.sum() > 0
+    # The vertices the hard paths serve, checked by the oracle's mark-based
+    # per-vertex count (too slow for hubs by the reference loop): the
+    # highest-degree vertices (owners of many tiles, their tables kept across
+    # them), the longest out-lists among the 30,000 highest-degree vertices
+    # (the group kernel's fullest tables; beyond 4096 ids they are searched in
+    # HBM), and vertices at degree ranks 10 .. 10^6.
+    deg = np.diff(off)
+    order = np.argsort(-deg, kind="stable")
+    cand = order[:30000]
+    key = (np.minimum(deg, 65535).astype(np.int64) << 32) | np.arange(len(deg), dtype=np.int64)
+    dplus = np.array([int((key[tails[off[v]:off[v + 1]]] > key[v]).sum()) for v in cand])
+    longest = cand[np.argsort(-dplus, kind="stable")[:8]]
+    ranks = order[[10, 100, 1000, 10000, 100000, 1000000]]
+    hard = np.unique(np.concatenate([order[:8], longest, ranks]))
+    _say("rmat s26 hubs and long out-lists", len(hard), "largest d+", int(dplus.max()))
+    exp_hard = O.vertex_min_credit(off, tails, hard)
+    assert (pv[hard] == exp_hard).all(), (hard, pv[hard], exp_hard)
     g.close()

@@ -154,3 +154,28 @@ def test_rgg_connect_radius_formula():
     import xxtric_amd
     for n in (1000, 100000):
         assert xxtric_amd.rgg_radius_connect(n) == sqrt(2 * log(n) / (np.pi * n))
+
+
+@pytest.mark.parametrize("name", ["rmat11p", "rmat12", "rgg4096", "er", "k40"])
+def test_vertex_min_credit_equals_reference(name):
+    """or_vertex_min_credit32 (mark N(v), scan N(u) above u) equals the reference
+    restatement's per-vertex counts on every vertex, and K_n's closed form."""
+    if name == "rmat11p":
+        off, tails = O.rmat_csr(11, 16, 3, True)
+    elif name == "rmat12":
+        off, tails = O.rmat_csr(12, 16, 5, False)
+    elif name == "rgg4096":
+        off, tails = O.rgg_lcg(4096, 1)
+    elif name == "er":
+        off, tails = O.er_csr(3000, 15, O.er_threshold(0.01))
+    else:
+        off, tails = complete(40)
+    off = np.asarray(off, np.int64)
+    tails = np.asarray(tails, np.int64)
+    n = len(off) - 1
+    _, pv = O.count_reference(off, tails, per_vertex=True)
+    got = O.vertex_min_credit(off, tails, np.arange(n))
+    assert (got == pv).all()
+    if name == "k40":
+        i = np.arange(n)
+        assert (got == (n - 1 - i) * (n - 2 - i) // 2).all()
