@@ -350,6 +350,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8))) 
     u0 = tile_row_of(r, s_off, off, e0);
     {
       auto dplus_of = [&](int64_t uu) -> int64_t {
+        if constexpr (NARROW && !PRUNE) return (int64_t)(vrec[uu] >> 32);  // one load: the row's own record
         const int64_t d = ooff[uu + 1] - ooff[uu];
         return NARROW ? d : (d < 65535 ? d : 65535);
       };
@@ -444,8 +445,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8))) 
     if (live) {
       cur = tile_off(r, s_off, off, u);
       nxt = tile_off(r, s_off, off, u + 1);
-      a0 = ooff[u];
-      a1 = ooff[u + 1];
+      if constexpr (NARROW && !PRUNE) {  // the row's own record: offset and out-degree in one load
+        const uint64_t rec = vrec[u];
+        a0 = (int64_t)(uint32_t)rec;
+        a1 = a0 + (int64_t)(rec >> 32);
+      } else {
+        a0 = ooff[u];
+        a1 = ooff[u + 1];
+      }
     }
     const int lane = threadIdx.x & (kWave - 1);
 #pragma unroll
@@ -457,8 +464,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8))) 
           ++u;
           cur = nxt;
           nxt = tile_off(r, s_off, off, u + 1);
-          a0 = a1;
-          a1 = ooff[u + 1];
+          if constexpr (NARROW && !PRUNE) {
+            const uint64_t rec = vrec[u];
+            a0 = (int64_t)(uint32_t)rec;
+            a1 = a0 + (int64_t)(rec >> 32);
+          } else {
+            a0 = a1;
+            a1 = ooff[u + 1];
+          }
         }
         const bool inner = cur >= eb && nxt <= ee;
         if (inner && e == cur) {  // the row's first entry publishes its item count
