@@ -268,8 +268,11 @@ __device__ __forceinline__ void load_ids(const int32_t* __restrict__ ot, int64_t
 // both rows of an edge pick the same owner.  Otherwise a 2-byte gather of the
 // saturated out-degree classifies every entry (both rows see the same
 // saturated pair) and the owned and light entries gather their offsets.
+// Six waves per SIMD for batches (dependent-load latency of small graphs'
+// rows: 11.7 -> 11.4 ms on 4096 graphs), five otherwise (R-MAT s26's gathers:
+// 42.6 ms at five, 46.3 at six).
 template <typename T, bool NARROW, int CM, bool BATCH>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_light_items(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(BATCH && CM == kCreditTotal ? 6 : 5, 8))) void k_light_items(
     int64_t nv, int64_t ne, const int64_t* __restrict__ off, const T* __restrict__ tails,
     const uint8_t* __restrict__ oflags, const int64_t* __restrict__ ooff, const uint64_t* __restrict__ vrec,
     const uint4* __restrict__ vrange, const uint16_t* __restrict__ dp16, const int32_t* __restrict__ ot,
