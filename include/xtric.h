@@ -95,7 +95,7 @@ int xt_ctx_tune_tiles(xt_ctx* ctx, int items_per_tile, int tile_run);
  * ids (1..512) are counted one wave per tile of at most 64 items, with an
  * LDS table per wave of 1 KB (up to 128 ids), 2 KB (256) or 4 KB (512),
  * instead of by the 512-thread group kernel (default 256 for one graph of
- * 2^28 or more entries, 64 for batches and the server from 2^22 entries on,
+ * 2^27 or more entries, 64 for batches and the server from 2^22 entries on,
  * none for smaller inputs, where the extra launch costs more than it saves; a
  * value set here applies at every size).  Plain, per-vertex and owned-range
  * counts all use them.  Does not change a result.
