@@ -244,6 +244,10 @@ def test_native_replay(ctx):
     assert bad == 0 and (lat >= 0).all()
     st = srv.stats()
     assert st["completed"] == n and st["rejected"] == 0
+    # the worker's time breakdown: every part measured, the batches' device time positive
+    for k in ("admit_s", "stage_s", "launch_s", "complete_s", "device_s"):
+        assert st[k] >= 0.0, k
+    assert st["device_s"] > 0.0 and st["launch_s"] > 0.0
     # a wrong expectation is reported, not hidden
     bad2, _ = srv.replay(gs, exp + np.uint64(1), order[:5], srv.now_us() + np.arange(5) * 10.0)
     assert bad2 == 5
