@@ -1380,17 +1380,25 @@ __global__ __launch_bounds__(kWaveThreads) __attribute__((amdgpu_waves_per_eu(WP
     pre = inc - nc;
     total = __builtin_amdgcn_readlane(inc, kWave - 1);
   };
-  // Chunk c0 + lane of a tile: its item kk has pre[kk] <= c < pre[kk+1].
+  // Chunk c0 + lane of a tile: its item kk is the last with pre[kk] <= c,
+  // found by a binary search over the lanes' pre values (lane k holds pre[k],
+  // ascending; items 0..n-1), six shuffles.
   auto load_round = [&](int64_t ib, int32_t il, int32_t iy, int32_t pre, int n, int32_t total,
                         int32_t c0) -> WaveChunk {
     const int32_t c = c0 + lane;
-    const int k0 = __popcll(__builtin_amdgcn_ballot_w64(lane < n && pre <= c0)) - 1;
-    uint64_t sb = (lane < n && pre > c0 && pre < c0 + kWave) ? (1ull << (pre - c0)) : 0ull;
+    int kk = 0;
+    int32_t kp = 0;  // pre[0]
 #pragma unroll
-    for (int o = 1; o < kWave; o <<= 1) sb |= __shfl_xor(sb, o, kWave);
-    const int kk = k0 + __popcll(sb & ((2ull << lane) - 1ull));
+    for (int step = kWave / 2; step > 0; step >>= 1) {
+      const int cand = kk + step;  // <= 63
+      const int32_t pc = __shfl(pre, cand, kWave);
+      if (cand < n && pc <= c) {
+        kk = cand;
+        kp = pc;
+      }
+    }
     const int64_t kb = __shfl(ib, kk, kWave);
-    const int32_t kp = __shfl(pre, kk, kWave), kl = __shfl(il, kk, kWave);
+    const int32_t kl = __shfl(il, kk, kWave);
     const int32_t q = c - kp;  // chunk of item kk
     WaveChunk ch{make_int4(-1, -1, -1, -1), kl - 4 * q, 0, kk};  // ids inside the list: <= 0 past the run's end
     if constexpr (RANGE || PV) ch.y = __shfl(iy, kk, kWave);
