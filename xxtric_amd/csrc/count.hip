@@ -1188,10 +1188,8 @@ __global__ __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(W
 // chunk.  Small sparse graphs (batches) have almost only such owners, and a
 // workgroup-wide tile per owner cost more in set-up than its lists.
 // Chunk c of a round belongs to the item k with pre[k] <= c < pre[k+1]
-// (pre: exclusive scan of the items' chunk counts, lane k holding pre[k]):
-// the items starting at or before the round's first chunk are counted with
-// one ballot, those starting inside the round are bits of a 64-bit mask
-// OR-ed across the wave, and each lane adds the bits up to its own position.
+// (pre: exclusive scan of the items' chunk counts, lane k holding pre[k]),
+// found by a binary search over the lanes' pre values with six shuffles.
 constexpr int kWaveLogMax = 8;  // per-wave LDS hash: up to 256 buckets of four ids
 constexpr int kWaveOwnerMax = 2 << kWaveLogMax;  // owner lists fill at most half the slots
 constexpr int kWaveThreads = 256;  // four independent waves per workgroup
