@@ -529,6 +529,16 @@ def main():
             line["serve"] = serve_workload(ctx, args.serve_rate)
         except Exception as e:
             line["serve"] = {"error": str(e)}
+        try:  # the same at other offered rates: latency below saturation, capacity above it
+            sweep = []
+            for rate in (1000.0, 10000.0, 20000.0, 100000.0):
+                r = serve_workload(ctx, rate, n_req=max(2000, min(40000, int(rate))))
+                sweep.append({k: r[k] for k in ("rate_graphs_per_s", "graphs_per_s", "p50_us", "p99_us",
+                                                "counts_equal_solo")})
+                sweep[-1]["small_p99_us"] = r["small"]["p99_us"]
+            line["serve_sweep"] = sweep
+        except Exception as e:
+            line["serve_sweep"] = {"error": str(e)}
     print(json.dumps(line), flush=True)
     ctx.close()
 
