@@ -460,6 +460,8 @@ def main():
                     help="only run the batched 4096-graph workload")
     ap.add_argument("--ceilings", action="store_true",
                     help="only measure the streaming-read and random-gather ceilings of this GPU")
+    ap.add_argument("--per-vertex-only", action="store_true",
+                    help="only run the per-vertex count of the R-MAT graph (--scale), library defaults")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the library's default knobs instead of tuning them on each workload first")
     args = ap.parse_args()
@@ -481,6 +483,13 @@ def main():
                   flush=True)
         return
     tune = not args.no_autotune
+    if args.per_vertex_only:
+        g = ctx.rmat(args.scale, 16, 1, True)
+        tri = g.count()
+        out = per_vertex_count(ctx, g, tri)
+        out.update({"workload": "rmat_s%d_ef16_whole_graph" % args.scale, "triangles": tri})
+        print(json.dumps(out), flush=True)
+        return
     if args.batch_only:
         print(json.dumps(batch_workload(ctx, tune=tune)), flush=True)
         return
