@@ -25,6 +25,13 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+# HIP runs a process's streams on GPU_MAX_HW_QUEUES hardware queues (4 by
+# default).  The serving run has the context's stream and four slot streams
+# (and the default stream): on four queues two slots share one, and a small
+# batch in one of them waits behind a large request in the other.  Eight give
+# every stream a queue of its own; set before anything initialises the GPU.
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
+
 METRIC = "edges/s + %HBM roofline, R-MAT s26 (1 GPU); batched graphs/s, p99 latency"
 HBM_PEAK_GBS = 8000.0
 # rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload; its line
@@ -394,6 +401,7 @@ def serve_workload(ctx, rate, n_req=4000, pool=400, seed=7, inflight=4):
             "batches": st["batches"], "isolated_batches": st["isolated_batches"],
             "rejected": st["rejected"], "counts_equal_solo": bad == 0 and st["rejected"] == 0,
             "submitter": "native (xt_serve_replay)",
+            "hip_hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")),
             "worker_s": {k: round(st[k], 4) for k in ("admit_s", "stage_s", "launch_s", "complete_s", "device_s")},
             # the same latencies split by request class: the pool's small graphs and the
             # 1 % R-MAT s18-s20 requests
