@@ -465,7 +465,8 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the 1 GiB copy kernel used to scale PMC byte counters")
     ap.add_argument("--serve-sweep", action="store_true",
-                    help="only run the serving workload at 1k..50k graphs/s and print one line per rate")
+                    help="only run the serving workload at 1k..100k graphs/s offered, 60 s or 100k requests "
+                         "per rate, and print one line per rate")
     ap.add_argument("--batch-only", action="store_true",
                     help="only run the batched 4096-graph workload")
     ap.add_argument("--ceilings", action="store_true",
@@ -489,8 +490,7 @@ def main():
         return
     if args.serve_sweep:
         for rate in (1000, 2000, 5000, 10000, 20000, 50000, 100000):
-            print(json.dumps(serve_workload(ctx, float(rate), n_req=max(2000, min(40000, rate)))),
-                  flush=True)
+            print(json.dumps(serve_workload(ctx, float(rate), n_req=min(100000, 60 * rate))), flush=True)
         return
     tune = not args.no_autotune
     if args.per_vertex_only:
