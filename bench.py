@@ -134,28 +134,33 @@ def small_configs(ctx, steps=5, cpu=True, cpu_seconds=10.0):
     O = _oracle() if cpu else None
     cores = O.nthreads_default() if cpu else 0
     out = {}
-    off, tails = xxtric_amd.rgg_generate(100000)
-    g = ctx.graph(off, tails)
-    tot, pv = g.count(per_vertex=True)
-    ms = []
-    for _ in range(steps):
-        g.count()
-        ms.append(ctx.timing()["total_ms"])
-    m = len(tails) // 2
-    out["rgg_1e5"] = {"nv": g.nv, "directed_entries": g.ne, "triangles": tot,
-                      "per_vertex_sum_equals_total": int(pv.sum(dtype=np.uint64)) == tot,
-                      "ms": round(float(np.median(ms)), 3),
-                      "edges_per_s": round(m / (float(np.median(ms)) * 1e-3), 1)}
-    g.close()
-    if cpu:  # the whole count, one thread and all cores, best of 3
-        t1, c1 = _timed(lambda: O.count_reference(off, tails, nthreads=1), 3)
-        tn, cn = _timed(lambda: O.count_reference(off, tails, nthreads=cores), 3)
-        out["rgg_1e5"]["cpu_baseline"] = {
-            "kind": "port", "unit": "edges/s", "cores": cores, "value": round(m / tn, 1), "s": round(tn, 4),
-            "single_thread_value": round(m / t1, 1), "single_thread_s": round(t1, 4),
-            "counts_equal": c1 == cn == tot,
-            "sample": "whole graph: the reference algorithm (oracle restatement of xxtric.hpp:60-208), "
-                      "best of 3 runs"}
+    # rgg_1e5: the reference's radius (graph.hpp:789-791) and test (distance <= r);
+    # rgg_1e5_connectivity_radius: r = sqrt(2 ln n / (pi n)) and distance < r
+    for name, strict, connect in (("rgg_1e5", False, False), ("rgg_1e5_connectivity_radius", True, True)):
+        off, tails = xxtric_amd.rgg_generate(100000, strict=strict, connect=connect)
+        g = ctx.graph(off, tails)
+        tot, pv = g.count(per_vertex=True)
+        ms = []
+        for _ in range(steps):
+            g.count()
+            ms.append(ctx.timing()["total_ms"])
+        m = len(tails) // 2
+        out[name] = {"nv": g.nv, "directed_entries": g.ne, "triangles": tot,
+                     "per_vertex_sum_equals_total": int(pv.sum(dtype=np.uint64)) == tot,
+                     "ms": round(float(np.median(ms)), 3),
+                     "edges_per_s": round(m / (float(np.median(ms)) * 1e-3), 1)}
+        if connect:
+            out[name]["radius"] = float(xxtric_amd.rgg_radius_connect(100000))
+        g.close()
+        if cpu:  # the whole count, one thread and all cores, best of 3
+            t1, c1 = _timed(lambda: O.count_reference(off, tails, nthreads=1), 3)
+            tn, cn = _timed(lambda: O.count_reference(off, tails, nthreads=cores), 3)
+            out[name]["cpu_baseline"] = {
+                "kind": "port", "unit": "edges/s", "cores": cores, "value": round(m / tn, 1), "s": round(tn, 4),
+                "single_thread_value": round(m / t1, 1), "single_thread_s": round(t1, 4),
+                "counts_equal": c1 == cn == tot,
+                "sample": "whole graph: the reference algorithm (oracle restatement of xxtric.hpp:60-208), "
+                          "best of 3 runs"}
     g = ctx.rmat(22, 16, 1, True)
     ms = []
     for _ in range(steps + 1):
