@@ -187,6 +187,23 @@ def small_configs(ctx, steps=5, cpu=True, cpu_seconds=10.0):
                       "extrapolated by adjacency fraction; beside it the oracle's exact degree-ordered counter "
                       "over the whole graph" % cpu_seconds}
     g.close()
+    # the same tuples without the seeded label permutation: hubs at low ids
+    g = ctx.rmat(22, 16, 1, False)
+    ms = []
+    for _ in range(steps + 1):
+        tot = g.count()
+        ms.append(ctx.timing()["total_ms"])
+    med = float(np.median(ms[1:]))
+    out["rmat_s22_unpermuted"] = {"nv": g.nv, "directed_entries": g.ne, "triangles": tot, "ms": round(med, 3),
+                                  "edges_per_s": round((g.ne // 2) / (med * 1e-3), 1)}
+    if cpu:
+        ou, tu = g.download()
+        td, cd = _timed(lambda: O.count_degree_ordered(ou, tu, nthreads=cores))
+        out["rmat_s22_unpermuted"]["cpu_baseline"] = {
+            "kind": "port", "unit": "edges/s", "cores": cores, "value": round((len(tu) // 2) / td, 1),
+            "s": round(td, 3), "counts_equal": cd == tot,
+            "sample": "the oracle's exact degree-ordered counter over the whole graph"}
+    g.close()
     return out
 
 
@@ -304,7 +321,8 @@ def cpu_baseline(g, seconds):
 def default_knobs(ctx):
     """The library's own defaults (which depend on the input's size) in place of
     knobs an earlier workload's tuning set explicitly."""
-    ctx.tune(small_cut=-1, group_blocks=4, group_round=2, items_per_tile=-1, tile_run=-1, wave_owner=-1)
+    ctx.tune(small_cut=-1, group_blocks=4, group_round=2, items_per_tile=-1, tile_run=-1, wave_owner=-1,
+             group_hash=-1)
 
 
 def batch_workload(ctx, n_graphs=4096, runs=5, tune=True):

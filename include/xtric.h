@@ -102,6 +102,13 @@ int xt_ctx_tune_tiles(xt_ctx* ctx, int items_per_tile, int tile_run);
  * No counterpart in the reference, whose launch shapes are compile-time
  * macros (counting.cuh:150-176). */
 int xt_ctx_tune_waves(xt_ctx* ctx, int wave_owner);
+/* The group kernel's LDS hash of an owner's out-list: 1 (default) the top 11
+ * bits of id * 0x9E3779B1, which spreads ids whatever their bit pattern
+ * (labels that share low bits, such as an R-MAT graph's without a
+ * relabelling); 2 the low 11 bits of the id, one multiply fewer per streamed
+ * id, for randomly labelled graphs.  0 keeps, -1 restores the default.  Does
+ * not change a result.  No counterpart in the reference. */
+int xt_ctx_tune_hash(xt_ctx* ctx, int group_hash);
 /* on != 0: count every graph with the layout used for graphs of 2^32 or more
  * entries (64-bit list offsets gathered by the light pass, 16-bit saturated
  * out-degrees for classification) -- for tests and comparisons on graphs of

@@ -221,6 +221,91 @@ def test_complete_graph_fills_the_lds_hash(ctx, n):
     assert ctx.count_csr(off, tails) == tot
 
 
+def _group_colliding_ids(count, start, bucket):
+    """count ids >= start whose home bucket in the group kernel's 2048-bucket
+    table is `bucket` (top 11 bits of id * 0x9E3779B1 mod 2^32)."""
+    ids = np.arange(start, start + 2_000_000, dtype=np.uint64)
+    home = ((ids * np.uint64(0x9E3779B1)) & np.uint64(0xFFFFFFFF)) >> np.uint64(21)
+    sel = ids[home == bucket][:count]
+    assert len(sel) == count
+    return [int(v) for v in sel]
+
+
+@pytest.mark.parametrize("group_hash", [1, 2])
+@pytest.mark.parametrize("k", [5, 40, 300])
+def test_group_table_spills(ctx, k, group_hash):
+    """A clique on k+1 vertices whose ids share one home bucket of the group
+    kernel's table (under its multiplicative hash, 1, or its low-bits hash, 2:
+    ids 2048 apart): the bucket spills, keys sit in its marked slot and in the
+    buckets after it (up to 75 of them), and lookups walk the chain.  Closed
+    forms: C(k+1, 3) triangles, the j-th smallest id credited C(k - j, 2)."""
+    coll = _group_colliding_ids(k, 1000, 1234) if group_hash == 1 else [1000 + 2048 * i for i in range(k)]
+    ids = [0] + coll
+    n = max(ids) + 1
+    edges = [(a, b) for i, a in enumerate(ids) for b in ids[i + 1:]]
+    off, tails = csr_from_edges(n, edges)
+    exp = (k + 1) * k * (k - 1) // 6
+    exp_pv = np.zeros(n, np.uint64)
+    for j, v in enumerate(ids):
+        exp_pv[v] = (k - j) * (k - j - 1) // 2
+    ctx.tune(small_cut=1, wave_owner=-2, group_hash=group_hash)  # every edge heavy, every owner grouped
+    try:
+        for orient in ("degree", "id"):
+            _say("group spill clique", k, group_hash, orient)
+            assert ctx.count_csr(off, tails, orient=orient) == exp
+            tot, pv = ctx.count_csr(off, tails, orient=orient, per_vertex=True)
+            assert tot == exp and (pv == exp_pv).all()
+    finally:
+        ctx.tune(small_cut=-1, wave_owner=-1, group_hash=-1)
+
+
+@pytest.mark.parametrize("group_hash", [1, 2])
+def test_group_hash_modes_match_oracle(ctx, golden, group_hash):
+    """Both table hashes give the golden counts on the R-MAT fixtures (both
+    orientations), the oracle's per-vertex counts at s14 and the oracle's
+    owned-range sums on a device graph."""
+    ctx.tune(group_hash=group_hash)
+    try:
+        for g in golden["rmat"]:
+            off, tails = O.rmat_csr(g["scale"], g["edgefactor"], g["seed"], bool(g["permute"]))
+            for orient in ("degree", "id"):
+                _say("hash", group_hash, "rmat", g["scale"], g["seed"], orient)
+                assert ctx.count_csr(off, tails.astype(np.int64), orient=orient) == g["triangles"]
+        off, tails = O.rmat_csr(14, 16, 1, True)
+        exp, exp_pv = O.count_degree_ordered(off, tails, per_vertex=True)
+        tot, pv = ctx.count_csr(off, tails.astype(np.int64), per_vertex=True)
+        assert tot == exp and (pv == exp_pv).all()
+        dg = ctx.rmat(13, 16, 4, True)
+        o13, t13 = dg.download()
+        e13, pv13 = O.count_degree_ordered(o13, t13, per_vertex=True)
+        lo, hi = 1000, 3000
+        owned, _, tot13 = dg.count_owned(lo, hi)
+        assert tot13 == e13 and owned == int(pv13[lo:hi].sum())
+        dg.close()
+    finally:
+        ctx.tune(group_hash=-1)
+
+
+def test_rmat_unpermuted_labels(ctx):
+    """R-MAT s16 without the label permutation (ids mostly zero bits, hubs at
+    low ids): counts and per-vertex counts equal the oracle's, in both
+    orientations, with and without wave tiles, under both table hashes (the
+    low-bits hash spills heavily on these labels)."""
+    off, tails = O.rmat_csr(16, 16, 2, False)
+    exp, exp_pv = O.count_degree_ordered(off, tails, per_vertex=True)
+    t64 = tails.astype(np.int64)
+    try:
+        for gh in (1, 2):
+            for wo in (-2, 256):
+                ctx.tune(wave_owner=wo, group_hash=gh)
+                for orient in ("degree", "id"):
+                    _say("rmat16 unpermuted", gh, wo, orient)
+                    tot, pv = ctx.count_csr(off, t64, per_vertex=True, orient=orient)
+                    assert tot == exp and (pv == exp_pv).all()
+    finally:
+        ctx.tune(wave_owner=-1, group_hash=-1)
+
+
 def test_per_vertex_owner_with_many_tiles(ctx):
     """Id orientation gives vertex 2000, joined to every other vertex, an
     out-list of 4000 ids (inside the LDS hash) and 6000 items: 24 group tiles of

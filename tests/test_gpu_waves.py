@@ -20,7 +20,8 @@ def _say(*a):
 
 
 def _restore(ctx):
-    ctx.tune(small_cut=-1, group_blocks=4, group_round=2, items_per_tile=-1, tile_run=-1, wave_owner=-1)
+    ctx.tune(small_cut=-1, group_blocks=4, group_round=2, items_per_tile=-1, tile_run=-1, wave_owner=-1,
+             group_hash=-1)
 
 
 @pytest.mark.parametrize("wave_owner", [1, 16, 64, 128, 256, 512])

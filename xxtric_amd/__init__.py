@@ -85,17 +85,21 @@ class Context:
         except Exception:
             pass
 
-    def tune(self, small_cut=0, group_blocks=0, group_round=0, items_per_tile=0, tile_run=0, wave_owner=0):
+    def tune(self, small_cut=0, group_blocks=0, group_round=0, items_per_tile=0, tile_run=0, wave_owner=0,
+             group_hash=0):
         """Set the light-edge cut, the group kernel's workgroups per CU, its chunks in
         flight per thread, the items per group tile, the tiles a workgroup draws at
-        once and/or the longest owner list counted in wave tiles (0 = keep; -1
-        restores the default of small_cut, items_per_tile, tile_run, wave_owner;
+        once, the longest owner list counted in wave tiles and/or the group
+        kernel's table hash (1 multiplicative, 2 low bits) (0 = keep; -1 restores
+        the default of small_cut, items_per_tile, tile_run, wave_owner, group_hash;
         wave_owner -2 turns wave tiles off).  No setting changes a result."""
         self._check(self._L.xt_ctx_tune(self.h, small_cut, group_blocks, group_round))
         if items_per_tile or tile_run:
             self._check(self._L.xt_ctx_tune_tiles(self.h, items_per_tile, tile_run))
         if wave_owner:
             self._check(self._L.xt_ctx_tune_waves(self.h, wave_owner))
+        if group_hash:
+            self._check(self._L.xt_ctx_tune_hash(self.h, group_hash))
 
     def fail_allocs(self, n):
         """Failure injection: the next n scratch allocations fail with XT_ERR_NOMEM."""

@@ -1,6 +1,6 @@
 """Autotuner for the counting pipeline's launch knobs.
 
-Six knobs decide how the work is split; none changes a result:
+Seven knobs decide how the work is split; none changes a result:
   * light_cut      -- edges whose out-lists are both at most this long are
                       intersected in the light pass; longer ones go to the
                       group kernel (LDS-staged list, streamed partner lists)
@@ -14,10 +14,13 @@ Six knobs decide how the work is split; none changes a result:
   * items_per_tile -- heavy edges of one owner per group tile (64, 128, 256);
   * tile_run       -- consecutive tiles a workgroup draws from the work queue
                       at once (an owner's tiles are consecutive, so longer runs
-                      keep its staged table more often).
+                      keep its staged table more often);
+  * group_hash     -- the group kernel's table hash: 1 multiplicative (spreads
+                      any labels), 2 the id's low bits (one multiply fewer per
+                      streamed id, enough for randomly labelled graphs).
 The reference fixes its launch shapes with compile-time macros
 (counting.cuh:150-176); here they are context knobs (xt_ctx_tune,
-xt_ctx_tune_tiles, xt_ctx_tune_waves) chosen by measurement.
+xt_ctx_tune_tiles, xt_ctx_tune_waves, xt_ctx_tune_hash) chosen by measurement.
 
 The search is coordinate descent from the library defaults: each knob in turn
 is swept with the others held at the best setting so far, for at most
@@ -45,9 +48,10 @@ SPACE = {
     "group_blocks": (2, 3, 4),
     "items_per_tile": (64, 128, 256),
     "tile_run": (4, 8, 16, 32, 64),
+    "group_hash": (1, 2),
 }
 DEFAULTS = {"light_cut": 32, "wave_owner": 256, "group_round": 2, "group_blocks": 4, "items_per_tile": 256,
-            "tile_run": 16}
+            "tile_run": 16, "group_hash": 1}
 BATCH_DEFAULTS = dict(DEFAULTS, light_cut=8, wave_owner=64)
 
 
@@ -83,7 +87,8 @@ def _time_batch(b, reps):
 def apply(ctx, knobs):
     ctx.tune(small_cut=knobs["light_cut"], group_blocks=knobs["group_blocks"],
              group_round=knobs["group_round"], items_per_tile=knobs["items_per_tile"],
-             tile_run=knobs["tile_run"], wave_owner=knobs.get("wave_owner", -1))
+             tile_run=knobs["tile_run"], wave_owner=knobs.get("wave_owner", -1),
+             group_hash=knobs.get("group_hash", -1))
 
 
 def search(ctx, runner, start, space=SPACE, passes=2, log=print, label=""):

@@ -92,6 +92,7 @@ _SIGS = {
     "xt_ctx_tune": (_int, [_p, _int, _int, _int]),
     "xt_ctx_tune_tiles": (_int, [_p, _int, _int]),
     "xt_ctx_tune_waves": (_int, [_p, _int]),
+    "xt_ctx_tune_hash": (_int, [_p, _int]),
     "xt_ctx_force_wide": (_int, [_p, _int]),
     "xt_ctx_fail_allocs": (_int, [_p, _int]),
     "xt_dev_alloc": (_int, [_p, ctypes.c_size_t, _pp]),

@@ -370,6 +370,18 @@ int xt_ctx_tune_waves(xt_ctx* ctx, int wave_owner) {
   });
 }
 
+int xt_ctx_tune_hash(xt_ctx* ctx, int group_hash) {
+  return xt_guarded([&]() -> int {
+    if (!ctx) return XT_ERR_ARG;
+    if (group_hash < -1 || group_hash > 2)
+      return set_err(ctx, XT_ERR_ARG, "group_hash must be 1 (multiplicative) or 2 (low bits); 0 keeps, -1 restores the default");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    if (group_hash > 0) ctx->group_hash = group_hash;
+    if (group_hash == -1) ctx->group_hash = kHashMix;
+    return XT_OK;
+  });
+}
+
 int xt_ctx_fail_allocs(xt_ctx* ctx, int n) {
   return xt_guarded([&]() -> int {
     if (!ctx || n < 0) return XT_ERR_ARG;

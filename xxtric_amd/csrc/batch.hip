@@ -118,6 +118,7 @@ struct xt_batch {
   int exec_items_per_tile = -1;
   int exec_tile_run = -1;
   int exec_wave_owner = -2;
+  int exec_group_hash = 0;
   // pinned host mirror of the local arena for staged host adds
   int64_t* h_loff = nullptr;
   int32_t* h_ltails = nullptr;
@@ -262,7 +263,7 @@ int batch_launch(xt_batch* b, int mode, uint32_t flags) {
         b->exec_flags != flags || b->exec_small_cut != ctx->small_cut ||
         b->exec_group_blocks != ctx->group_blocks || b->exec_group_round != ctx->group_round ||
         b->exec_items_per_tile != ctx->items_per_tile || b->exec_tile_run != ctx->tile_run ||
-        b->exec_wave_owner != ctx->wave_owner) {
+        b->exec_wave_owner != ctx->wave_owner || b->exec_group_hash != ctx->group_hash) {
       if (b->exec) {
         XT_HIP(ctx, hipGraphExecDestroy(b->exec));
         b->exec = nullptr;
@@ -292,6 +293,7 @@ int batch_launch(xt_batch* b, int mode, uint32_t flags) {
       b->exec_items_per_tile = ctx->items_per_tile;
       b->exec_tile_run = ctx->tile_run;
       b->exec_wave_owner = ctx->wave_owner;
+      b->exec_group_hash = ctx->group_hash;
     }
     XT_HIP(ctx, hipEventRecord(b->ev0, st));
     XT_HIP(ctx, hipGraphLaunch(b->exec, st));

@@ -27,6 +27,8 @@ constexpr int kBatchLightCutSmall = 64;  // ... below kBatchWaveMinEntries (no w
 constexpr int kOffCap = 2048;
 // Wave tiles (k_count_waves): owners whose out-list has at most this many ids
 // are counted one wave per tile; 0 turns them off.
+constexpr int kHashMix = 1;      // group-kernel table hash: top bits of id * 0x9E3779B1 (default)
+constexpr int kHashLowBits = 2;  // ... or id mod 2048
 constexpr int kWaveOwner = 256;      // default, one graph
 constexpr int kBatchWaveOwner = 64;  // default, batches and the server
 constexpr int kWaveBlocksPerCU = 8;  // persistent 256-thread workgroups per CU
@@ -104,6 +106,7 @@ struct xt_ctx {
   int small_cut = 0;
   int group_blocks = 4;   // persistent group-kernel workgroups per CU
   int group_round = 2;    // 16-byte chunks per thread per group-kernel streaming round (1, 2, 4)
+  int group_hash = 1;     // group-kernel LDS hash: kHashMix (multiplicative) or kHashLowBits (id mod 2048)
   int items_per_tile = 256;  // items of one group tile (64, 128 or 256)
   int tile_run = 16;         // consecutive tiles a group-kernel workgroup draws at once (1..64)
   int wave_owner = -1;       // wave tiles for owner lists up to this long; -1 default, 0 off
